@@ -1,0 +1,163 @@
+/*
+ * pfhip.h -- C ABI of libpfhip.so, the gfx950 FDTD stepper behind
+ * photonfdtd's Simulation(backend="hip").
+ *
+ * The reference binds its GPU stepper through PyO3 (rust/src/lib.rs:139-152,
+ * rust/src/cuda.rs:508-574: CudaStepper.new_f32/new_f64, run_steps,
+ * read_field, device_info, CUDA_BUILT). This header is the plain-C
+ * replacement: every entry point takes plain pointers and sizes, returns 0 on
+ * success and a negative PF_E* code on failure, with a message available from
+ * pf_last_error(). Handles are not thread-safe; one HIP stream per handle.
+ *
+ * Array conventions: fields are C-order (nx, ny, nz), z fastest. "dtype"
+ * arrays are float (PF_F32) or double (PF_F64) according to pf_desc.dtype.
+ */
+#ifndef PFHIP_H
+#define PFHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFHIP_ABI_VERSION 1
+
+enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
+enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
+enum pf_status {
+  PF_OK = 0,
+  PF_E_ARG = -1,      /* invalid argument / shape mismatch                 */
+  PF_E_HIP = -2,      /* HIP runtime error (allocation, launch, copy)      */
+  PF_E_STATE = -3,    /* call not valid in the handle's current state      */
+  PF_E_LIMIT = -4     /* request exceeds a documented size limit           */
+};
+
+/* A strided sub-box of the grid: nodes start[a] + n*stride[a], n < count[a]. */
+typedef struct pf_box {
+  int64_t start[3];
+  int64_t count[3];
+  int64_t stride[3];
+} pf_box;
+
+/* One dispersive material (simulation.py:572-636): the cells it owns (flat
+ * C-order indices, ascending) and, per pole, (a, b, c) of
+ * P^{n+1} = a P^n + b P^{n-1} + c E^n. E is corrected by
+ * -inv_eps * sum_p (P^{n+1} - P^n) (simulation.py:645-669). */
+typedef struct pf_ade_desc {
+  int64_t n_cells;
+  const int64_t* cells;
+  int32_t n_poles;
+  const double* abc;      /* [n_poles][3] */
+  double inv_eps;         /* 1 / (eps0 * eps_inf) */
+} pf_ade_desc;
+
+/* Running DFT over a box (simulation.py:1023-1042,1116-1133):
+ * acc[c][f] += phase[s][f] * field_c * dt for every sampled step s, E
+ * components with phase_e (t = (n+1) dt), H components with phase_h
+ * (t = (n+1/2) dt). Steps n with n % interval == 0 are sampled, in order. */
+typedef struct pf_dft_desc {
+  int32_t n_comp;
+  int32_t comps[6];
+  pf_box box;
+  int32_t n_freq;
+  int32_t interval;
+  int64_t n_samples;          /* number of sampled steps over the whole run */
+  const double* phase_e;      /* [n_samples][n_freq][2] (re, im) */
+  const double* phase_h;      /* [n_samples][n_freq][2] */
+  double dt;                  /* Riemann weight */
+} pf_dft_desc;
+
+/* Time-domain Poynting flux through the plane index `index` normal to
+ * `axis` (simulation.py:1440-1459): total += sum(S . n) * dA * dt per step. */
+typedef struct pf_flux_desc {
+  int32_t axis;
+  int64_t index;
+  double dA;
+  double dt;
+} pf_flux_desc;
+
+/* Everything the stepper needs; host arrays are copied during pf_create.
+ * Mirrors the tables rustbackend._build_tables builds (rustbackend.py:135-166)
+ * plus the ADE, DFT and flux state the reference GPU path keeps on the host. */
+typedef struct pf_desc {
+  int32_t dtype;              /* pf_dtype */
+  int32_t exact;              /* 1: the reference NumPy operation order
+                                 (difference / cell size, no contraction);
+                                 0: multiply by 1/cell size, fused kernels */
+  int64_t shape[3];
+  double cell[3];             /* cell size per axis (1.0 on collapsed axes) */
+  double ch;                  /* dt / mu0 */
+  const void* ce;             /* [nx*ny*nz] dt/(eps_r eps0), dtype */
+  const double* b_e[3];       /* CPML coefficients per axis, [shape[a]] */
+  const double* c_e[3];
+  const double* b_h[3];
+  const double* c_h[3];
+  int64_t n_steps;            /* length of each waveform row */
+  /* soft sources, sorted by (component, cell); equal targets are applied in
+   * array order (the reference's source order) */
+  int32_t n_src;
+  const int32_t* src_comp;
+  const int64_t* src_cell;
+  const double* src_amp;
+  const int32_t* src_wave;    /* row of `wave` holding w(t) at the Yee time */
+  int32_t n_wave;
+  const double* wave;         /* [n_wave][n_steps] */
+  int32_t n_ade;
+  const pf_ade_desc* ade;
+  int32_t n_dft;
+  const pf_dft_desc* dft;
+  int32_t n_flux;
+  const pf_flux_desc* flux;
+} pf_desc;
+
+typedef struct pf_handle pf_handle;
+
+/* Kernel classes reported by pf_timing_read. */
+enum pf_kernel_class {
+  PF_K_YEE_H = 0,      /* H half-step (curl E + CPML)                       */
+  PF_K_YEE_E = 1,      /* E half-step (curl H + CPML)                       */
+  PF_K_YEE_FUSED = 2,  /* one launch doing a whole H+E step                 */
+  PF_K_AUX = 3,        /* per-step sources, ADE, flux and DFT capture/fold  */
+  PF_NKCLASS = 4
+};
+
+/* Replaces CudaStepper.new_f32/new_f64 (rust/src/cuda.rs:424-540). */
+int pf_create(const pf_desc* desc, pf_handle** out);
+/* Frees every device allocation of the handle. */
+int pf_destroy(pf_handle* h);
+/* Replaces CudaStepper.run_steps (rust/src/cuda.rs:298-341,543-548):
+ * advances steps [step0, step0 + n), synchronous on return. */
+int pf_advance(pf_handle* h, int64_t step0, int64_t n);
+/* Replaces CudaStepper.read_field (rust/src/cuda.rs:354-356): copies one
+ * component (comp 0..5 = Ex..Hz) into host memory of nx*ny*nz dtype. */
+int pf_read_field(pf_handle* h, int32_t comp, void* dst);
+/* Strided sub-box read for FieldMonitor downsample / plane_z
+ * (simulation.py:1062-1078); dst holds count[0]*count[1]*count[2] dtype. */
+int pf_read_box(pf_handle* h, int32_t comp, const pf_box* box, void* dst);
+/* Overwrites one component from host memory (nx*ny*nz dtype). */
+int pf_write_field(pf_handle* h, int32_t comp, const void* src);
+/* DFT accumulator of monitor `mon`, component slot `slot` (index into its
+ * comps[]): complex128 interleaved [n_freq][box cells]. Replaces the host
+ * accumulation of rustbackend.py:384-397. */
+int pf_dft_read(pf_handle* h, int32_t mon, int32_t slot, double* dst);
+/* Accumulated flux of every flux monitor, [n_flux] doubles. */
+int pf_flux_read(pf_handle* h, double* dst);
+/* Event timing (no reference counterpart; used by the benchmark): when on,
+ * each kernel-class launch group of pf_advance is bracketed by HIP events on
+ * the handle's stream. Enabling resets the totals. pf_timing_read returns,
+ * per class, total milliseconds and launch counts ([PF_NKCLASS] each). */
+int pf_timing_enable(pf_handle* h, int32_t on);
+int pf_timing_read(pf_handle* h, double* ms, int64_t* launches);
+/* Replaces CudaStepper.device_info (rust/src/cuda.rs:560-573). */
+int pf_device_info(char* name, int32_t name_len, int64_t* free_mb, int64_t* total_mb);
+/* Thread-local description of the last failure on this thread. */
+const char* pf_last_error(void);
+/* PFHIP_ABI_VERSION of the loaded library (the CUDA_BUILT analogue). */
+int32_t pf_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFHIP_H */

@@ -1,0 +1,107 @@
+"""ctypes binding of ``libpfhip.so`` (declarations in ``include/pfhip.h``).
+
+The loader fails loudly: there is no CPU fallback. It plays the role of the
+reference's ``_rust_module()`` gate (``src/photonfdtd/rustbackend.py:45-58``),
+including its ``RuntimeError`` with build instructions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
+ABI_VERSION = 1
+
+PF_F32, PF_F64 = 0, 1
+KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux")
+COMP_CODE = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+
+_i32, _i64, _f64, _vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p
+_P = C.POINTER
+
+
+class PfBox(C.Structure):
+    _fields_ = [("start", _i64 * 3), ("count", _i64 * 3), ("stride", _i64 * 3)]
+
+
+class PfAdeDesc(C.Structure):
+    _fields_ = [("n_cells", _i64), ("cells", _P(_i64)), ("n_poles", _i32), ("abc", _P(_f64)),
+                ("inv_eps", _f64)]
+
+
+class PfDftDesc(C.Structure):
+    _fields_ = [("n_comp", _i32), ("comps", _i32 * 6), ("box", PfBox), ("n_freq", _i32),
+                ("interval", _i32), ("n_samples", _i64), ("phase_e", _P(_f64)),
+                ("phase_h", _P(_f64)), ("dt", _f64)]
+
+
+class PfFluxDesc(C.Structure):
+    _fields_ = [("axis", _i32), ("index", _i64), ("dA", _f64), ("dt", _f64)]
+
+
+class PfDesc(C.Structure):
+    _fields_ = [("dtype", _i32), ("exact", _i32), ("shape", _i64 * 3), ("cell", _f64 * 3),
+                ("ch", _f64), ("ce", _vp),
+                ("b_e", _P(_f64) * 3), ("c_e", _P(_f64) * 3), ("b_h", _P(_f64) * 3),
+                ("c_h", _P(_f64) * 3),
+                ("n_steps", _i64), ("n_src", _i32), ("src_comp", _P(_i32)), ("src_cell", _P(_i64)),
+                ("src_amp", _P(_f64)), ("src_wave", _P(_i32)), ("n_wave", _i32),
+                ("wave", _P(_f64)), ("n_ade", _i32), ("ade", _P(PfAdeDesc)), ("n_dft", _i32),
+                ("dft", _P(PfDftDesc)), ("n_flux", _i32), ("flux", _P(PfFluxDesc))]
+
+
+# Exported symbols and their signatures; tests check every one is present.
+SIGNATURES = {
+    "pf_create": (_i32, [_P(PfDesc), _P(_vp)]),
+    "pf_destroy": (_i32, [_vp]),
+    "pf_advance": (_i32, [_vp, _i64, _i64]),
+    "pf_read_field": (_i32, [_vp, _i32, _vp]),
+    "pf_read_box": (_i32, [_vp, _i32, _P(PfBox), _vp]),
+    "pf_write_field": (_i32, [_vp, _i32, _vp]),
+    "pf_dft_read": (_i32, [_vp, _i32, _i32, _P(_f64)]),
+    "pf_flux_read": (_i32, [_vp, _P(_f64)]),
+    "pf_timing_enable": (_i32, [_vp, _i32]),
+    "pf_timing_read": (_i32, [_vp, _P(_f64), _P(_i64)]),
+    "pf_device_info": (_i32, [C.c_char_p, _i32, _P(_i64), _P(_i64)]),
+    "pf_last_error": (C.c_char_p, []),
+    "pf_abi_version": (_i32, []),
+}
+
+_LIB = None
+
+
+def load_library(path: os.PathLike = LIB_PATH) -> C.CDLL:
+    """Load and type the library; ``RuntimeError`` when it is missing or stale."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = Path(path)
+    if not path.exists():
+        raise RuntimeError(
+            f"backend='hip' needs the compiled library {path}, which is missing. "
+            "Build it with 'python -m photonfdtd_amd.build' (hipcc, gfx950).")
+    lib = C.CDLL(str(path))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.pf_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{path} has ABI {lib.pf_abi_version()}, expected {ABI_VERSION}; "
+                           "rebuild with 'python -m photonfdtd_amd.build'")
+    _LIB = lib
+    return lib
+
+
+def check(lib: C.CDLL, status: int, what: str) -> None:
+    """Map a non-zero status to ``RuntimeError("HIP error: ...")`` as the
+    reference maps CUDA failures (``rust/src/cuda.rs:267-269``)."""
+    if status != 0:
+        msg = lib.pf_last_error().decode(errors="replace")
+        if status == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what} failed ({status}): {msg}")
+
+
+def as_ptr(arr, ctype):
+    return arr.ctypes.data_as(_P(ctype))

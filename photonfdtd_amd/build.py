@@ -1,0 +1,58 @@
+"""Build libpfhip.so for gfx950 in-tree with hipcc (no JIT cache).
+
+``python -m photonfdtd_amd.build`` compiles every HIP translation unit under
+``photonfdtd_amd/csrc`` and links ``photonfdtd_amd/libpfhip.so``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OUT = PKG / "libpfhip.so"
+BUILD = PKG.parent / "_build"
+
+# -packed-fp32-ops keeps the compiler from emitting packed FP32 adds/muls/fmas;
+# -ffp-contract=off keeps the reference operation order (no implicit FMA) --
+# kernels that want FMAs call fma() explicitly.
+COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-Wall", "-Wno-unused-function", f"-I{PKG.parent / 'include'}"]
+
+SOURCES = ["kernels_generic.hip", "pfhip.cpp"]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found; the HIP library cannot be built")
+
+
+def build(verbose: bool = False) -> Path:
+    cc = hipcc()
+    BUILD.mkdir(exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        obj = BUILD / (Path(src).stem + ".o")
+        lang = ["-x", "hip"]
+        cmd = [cc, *COMMON, *lang, "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(str(obj))
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

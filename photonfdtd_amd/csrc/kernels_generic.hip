@@ -1,0 +1,408 @@
+// Generic Yee + CPML, source, ADE, DFT and flux kernels for gfx950.
+//
+// These kernels handle every grid shape the scene API allows (1-D, 2-D in any
+// plane, 3-D) and are the reference-order ("exact") path: each field update
+// performs the operations of the reference NumPy stepper in the same order
+// (simulation.py:1252-1309), so a float64 run reproduces the oracle's
+// trajectory bit for bit. This file is compiled with -ffp-contract=off.
+//
+// Thread mapping: blockIdx.y = x index i, blockIdx.x * 256 + tid = flat (j, k)
+// index, z fastest, so every wavefront reads 64 consecutive z nodes.
+#include "pfhip_internal.h"
+
+namespace pfhip {
+
+// Flat psi index of a compact PML array whose axis `ax` holds only the
+// `pc` nodes with a non-zero CPML c coefficient (rust/src/kernel.rs:92-103).
+__device__ __forceinline__ int64_t psi_index(int ax, int m, int i, int j, int k, int ny,
+                                             int nz, int pc) {
+  if (ax == 0) return ((int64_t)m * ny + j) * nz + k;
+  if (ax == 1) return ((int64_t)i * pc + m) * nz + k;
+  return ((int64_t)i * ny + j) * pc + m;
+}
+
+template <typename real, bool EXACT>
+__device__ __forceinline__ real diff(real hi, real lo, real d, real inv_d) {
+  if constexpr (EXACT) return (hi - lo) / d;
+  else return (hi - lo) * inv_d;
+}
+
+// One derivative term of a curl: present flag, value, and PML state.
+template <typename real>
+struct Term {
+  bool on;
+  real d;
+  real* psi;    // nullptr when the node is outside the PML slab
+  int64_t pid;
+  real b, c;
+};
+
+// F += K * (dA - dB), then F += K psiA, F -= K psiB on PML nodes, where
+// psi <- b psi + c d first. Same order as the reference's bulk-then-slab
+// update (simulation.py:1263-1309).
+template <typename real>
+__device__ __forceinline__ void apply_terms(real* __restrict__ F, int64_t id, real K,
+                                            const Term<real>& A, const Term<real>& B) {
+  if (!A.on && !B.on) return;
+  real cb;
+  if (A.on) {
+    cb = A.d;
+    if (B.on) cb = cb - B.d;
+  } else {
+    cb = -B.d;
+  }
+  real f = F[id];
+  f = f + cb * K;
+  if (A.on && A.psi) {
+    real p = A.b * A.psi[A.pid] + A.c * A.d;
+    A.psi[A.pid] = p;
+    f = f + K * p;
+  }
+  if (B.on && B.psi) {
+    real p = B.b * B.psi[B.pid] + B.c * B.d;
+    B.psi[B.pid] = p;
+    f = f - K * p;
+  }
+  F[id] = f;
+}
+
+template <typename real>
+__device__ __forceinline__ void pml_term(Term<real>& t, const StepArgs<real>& a, bool e_side,
+                                         int ax, int psi_slot, int i, int j, int k) {
+  const int node = ax == 0 ? i : (ax == 1 ? j : k);
+  const int* map = e_side ? a.map_e[ax] : a.map_h[ax];
+  const int m = map[node];
+  if (m < 0) {
+    t.psi = nullptr;
+    return;
+  }
+  const int pc = e_side ? a.pe[ax] : a.ph[ax];
+  t.psi = a.psi[psi_slot];
+  t.pid = psi_index(ax, m, i, j, k, a.ny, a.nz, pc);
+  t.b = e_side ? a.b_e[ax][node] : a.b_h[ax][node];
+  t.c = e_side ? a.c_e[ax][node] : a.c_h[ax][node];
+}
+
+template <typename real, bool EXACT>
+__global__ __launch_bounds__(256) void k_update_h(StepArgs<real> a) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= a.nyz) return;
+  const int i = blockIdx.y;
+  const int j = (int)(q / a.nz);
+  const int k = (int)(q - (int64_t)j * a.nz);
+  const int64_t id = (int64_t)i * a.nyz + q;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int64_t sx = a.nyz, sy = nz;
+  const real K = -a.ch;
+  real* Ex = a.f[0]; real* Ey = a.f[1]; real* Ez = a.f[2];
+  const bool xi = nx == 1 || i < nx - 1;
+  const bool yj = ny == 1 || j < ny - 1;
+  const bool zk = nz == 1 || k < nz - 1;
+  Term<real> A, B;
+  // Hx -= ch (d_y Ez - d_z Ey)
+  if (yj && zk) {
+    A.on = ny > 1;
+    B.on = nz > 1;
+    if (A.on) { A.d = diff<real, EXACT>(Ez[id + sy], Ez[id], a.d[1], a.inv_d[1]); pml_term(A, a, false, 1, PSI_HXY, i, j, k); }
+    if (B.on) { B.d = diff<real, EXACT>(Ey[id + 1], Ey[id], a.d[2], a.inv_d[2]); pml_term(B, a, false, 2, PSI_HXZ, i, j, k); }
+    apply_terms(a.f[3], id, K, A, B);
+  }
+  // Hy -= ch (d_z Ex - d_x Ez)
+  if (zk && xi) {
+    A.on = nz > 1;
+    B.on = nx > 1;
+    if (A.on) { A.d = diff<real, EXACT>(Ex[id + 1], Ex[id], a.d[2], a.inv_d[2]); pml_term(A, a, false, 2, PSI_HYZ, i, j, k); }
+    if (B.on) { B.d = diff<real, EXACT>(Ez[id + sx], Ez[id], a.d[0], a.inv_d[0]); pml_term(B, a, false, 0, PSI_HYX, i, j, k); }
+    apply_terms(a.f[4], id, K, A, B);
+  }
+  // Hz -= ch (d_x Ey - d_y Ex)
+  if (xi && yj) {
+    A.on = nx > 1;
+    B.on = ny > 1;
+    if (A.on) { A.d = diff<real, EXACT>(Ey[id + sx], Ey[id], a.d[0], a.inv_d[0]); pml_term(A, a, false, 0, PSI_HZX, i, j, k); }
+    if (B.on) { B.d = diff<real, EXACT>(Ex[id + sy], Ex[id], a.d[1], a.inv_d[1]); pml_term(B, a, false, 1, PSI_HZY, i, j, k); }
+    apply_terms(a.f[5], id, K, A, B);
+  }
+}
+
+template <typename real, bool EXACT>
+__global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= a.nyz) return;
+  const int i = blockIdx.y;
+  const int j = (int)(q / a.nz);
+  const int k = (int)(q - (int64_t)j * a.nz);
+  const int64_t id = (int64_t)i * a.nyz + q;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int64_t sx = a.nyz, sy = nz;
+  const real K = a.ce[id];
+  real* Hx = a.f[3]; real* Hy = a.f[4]; real* Hz = a.f[5];
+  const bool xi = nx == 1 || i >= 1;
+  const bool yj = ny == 1 || j >= 1;
+  const bool zk = nz == 1 || k >= 1;
+  Term<real> A, B;
+  // Ex += ce (d_y Hz - d_z Hy)
+  if (yj && zk) {
+    A.on = ny > 1;
+    B.on = nz > 1;
+    if (A.on) { A.d = diff<real, EXACT>(Hz[id], Hz[id - sy], a.d[1], a.inv_d[1]); pml_term(A, a, true, 1, PSI_EXY, i, j, k); }
+    if (B.on) { B.d = diff<real, EXACT>(Hy[id], Hy[id - 1], a.d[2], a.inv_d[2]); pml_term(B, a, true, 2, PSI_EXZ, i, j, k); }
+    apply_terms(a.f[0], id, K, A, B);
+  }
+  // Ey += ce (d_z Hx - d_x Hz)
+  if (zk && xi) {
+    A.on = nz > 1;
+    B.on = nx > 1;
+    if (A.on) { A.d = diff<real, EXACT>(Hx[id], Hx[id - 1], a.d[2], a.inv_d[2]); pml_term(A, a, true, 2, PSI_EYZ, i, j, k); }
+    if (B.on) { B.d = diff<real, EXACT>(Hz[id], Hz[id - sx], a.d[0], a.inv_d[0]); pml_term(B, a, true, 0, PSI_EYX, i, j, k); }
+    apply_terms(a.f[1], id, K, A, B);
+  }
+  // Ez += ce (d_x Hy - d_y Hx)
+  if (xi && yj) {
+    A.on = nx > 1;
+    B.on = ny > 1;
+    if (A.on) { A.d = diff<real, EXACT>(Hy[id], Hy[id - sx], a.d[0], a.inv_d[0]); pml_term(A, a, true, 0, PSI_EZX, i, j, k); }
+    if (B.on) { B.d = diff<real, EXACT>(Hx[id], Hx[id - sy], a.d[1], a.inv_d[1]); pml_term(B, a, true, 1, PSI_EZY, i, j, k); }
+    apply_terms(a.f[2], id, K, A, B);
+  }
+}
+
+// Soft sources: one thread per distinct target node; equal targets are
+// summed in source order with a rounding to the field dtype after every add,
+// which is what `F[i, j, k] += amp * w(t)` does per dipole in the reference
+// loop (simulation.py:1366-1390).
+template <typename real>
+__global__ void k_inject(real* const* __restrict__ fields, const int32_t* __restrict__ g_comp,
+                         const int64_t* __restrict__ g_cell, const int32_t* __restrict__ g_begin,
+                         const double* __restrict__ amp, const int32_t* __restrict__ wave_row,
+                         const double* __restrict__ wave, int64_t n_steps, int64_t step,
+                         int32_t n_groups) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_groups) return;
+  real* F = fields[g_comp[g]];
+  const int64_t cell = g_cell[g];
+  real v = F[cell];
+  for (int s = g_begin[g]; s < g_begin[g + 1]; ++s) {
+    const double add = amp[s] * wave[(int64_t)wave_row[s] * n_steps + step];
+    v = (real)((double)v + add);
+  }
+  F[cell] = v;
+}
+
+// E^n on a dispersive region, captured before the curl update
+// (simulation.py:638-643).
+template <typename real>
+__global__ void k_ade_capture(real* const* __restrict__ fields, const int64_t* __restrict__ cells,
+                              int64_t n, real* __restrict__ En) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t c = cells[t];
+  En[t] = fields[0][c];
+  En[n + t] = fields[1][c];
+  En[2 * n + t] = fields[2][c];
+}
+
+// Pole recursion and E correction (simulation.py:645-669): per component,
+// delta = sum_p (P_p^{n+1} - P_p^n), E -= inv_eps * delta.
+template <typename real>
+__global__ void k_ade_apply(real* const* __restrict__ fields, const int64_t* __restrict__ cells,
+                            int64_t n, const real* __restrict__ En, real* __restrict__ Pcur,
+                            real* __restrict__ Pprev, const real* __restrict__ abc, int n_poles,
+                            real inv_eps) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || n_poles == 0) return;
+  const int64_t cell = cells[t];
+  for (int c = 0; c < 3; ++c) {
+    const real e = En[c * n + t];
+    real delta = 0;
+    for (int p = 0; p < n_poles; ++p) {
+      const int64_t off = ((int64_t)p * 3 + c) * n + t;
+      const real pc = Pcur[off];
+      const real pp = Pprev[off];
+      const real pn = abc[3 * p] * pc + abc[3 * p + 1] * pp + abc[3 * p + 2] * e;
+      const real inc = pn - pc;
+      delta = p == 0 ? inc : delta + inc;
+      Pprev[off] = pc;
+      Pcur[off] = pn;
+    }
+    real* E = fields[c];
+    E[cell] = E[cell] - inv_eps * delta;
+  }
+}
+
+// Copy a strided box of one or more components into a dense buffer:
+// out[(c * nbox) + flat box index] = field_c[node].
+template <typename real>
+__global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __restrict__ comps,
+                             int n_comp, BoxDev box, int64_t nyz, int nz, real* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
+  if (t >= nbox * n_comp) return;
+  const int c = (int)(t / nbox);
+  const int64_t r = t - (int64_t)c * nbox;
+  const int64_t bk = r % box.count[2];
+  const int64_t bj = (r / box.count[2]) % box.count[1];
+  const int64_t bi = r / (box.count[2] * box.count[1]);
+  const int64_t node = (box.start[0] + bi * box.stride[0]) * nyz +
+                       (box.start[1] + bj * box.stride[1]) * nz + (box.start[2] + bk * box.stride[2]);
+  out[t] = fields[comps[c]][node];
+}
+
+// Fold `count` buffered samples into the complex128 accumulators, one sample
+// at a time in step order: acc += (phase * f) * dt, exactly the sequence of
+// the reference's per-step `acc += phase * snap * dt` (simulation.py:1133).
+template <typename real>
+__global__ void k_dft_fold(const real* __restrict__ ring, int64_t nbox, int n_comp,
+                           const int32_t* __restrict__ is_h, int count, int64_t sample0,
+                           const double* __restrict__ phase_e, const double* __restrict__ phase_h,
+                           int n_freq, double dt, double* __restrict__ acc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbox * n_comp) return;
+  const int c = (int)(t / nbox);
+  const int64_t r = t - (int64_t)c * nbox;
+  const double* ph = is_h[c] ? phase_h : phase_e;
+  double* A = acc + (int64_t)c * n_freq * nbox * 2;
+  for (int f = 0; f < n_freq; ++f) {
+    double re = A[2 * ((int64_t)f * nbox + r)];
+    double im = A[2 * ((int64_t)f * nbox + r) + 1];
+    for (int s = 0; s < count; ++s) {
+      const double v = (double)ring[((int64_t)s * n_comp + c) * nbox + r];
+      const double* p = ph + 2 * ((sample0 + s) * n_freq + f);
+      re = re + (p[0] * v) * dt;
+      im = im + (p[1] * v) * dt;
+    }
+    A[2 * ((int64_t)f * nbox + r)] = re;
+    A[2 * ((int64_t)f * nbox + r) + 1] = im;
+  }
+}
+
+// Plane Poynting flux, stage 1: per-block partial sums of S . n.
+template <typename real>
+__global__ __launch_bounds__(256) void k_flux_partial(const real* __restrict__ a1,
+                                                      const real* __restrict__ a2,
+                                                      const real* __restrict__ b1,
+                                                      const real* __restrict__ b2, int axis,
+                                                      int64_t index, int nx, int ny, int nz,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int64_t n1 = axis == 0 ? ny : nx;
+  const int64_t n2 = axis == 2 ? ny : nz;
+  const int64_t total = n1 * n2;
+  double s = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int64_t u = t / n2, v = t - (t / n2) * n2;
+    int64_t node;
+    if (axis == 0) node = (index * ny + u) * nz + v;
+    else if (axis == 1) node = (u * ny + index) * nz + v;
+    else node = (u * ny + v) * nz + index;
+    s += (double)a1[node] * (double)a2[node] - (double)b1[node] * (double)b2[node];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Stage 2: fixed-order sum of the partials, total += (sum * dA) * dt.
+__global__ void k_flux_finish(const double* __restrict__ partial, int n_part, double dA, double dt,
+                              double* __restrict__ total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int p = 0; p < n_part; ++p) s += partial[p];
+  *total = *total + (s * dA) * dt;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (called from pfhip.cpp with the handle's stream).
+// ---------------------------------------------------------------------------
+template <typename real>
+void launch_generic_h(const StepArgs<real>& a, bool exact, hipStream_t s) {
+  dim3 grid((unsigned)((a.nyz + 255) / 256), (unsigned)a.nx);
+  if (exact) k_update_h<real, true><<<grid, 256, 0, s>>>(a);
+  else k_update_h<real, false><<<grid, 256, 0, s>>>(a);
+}
+
+template <typename real>
+void launch_generic_e(const StepArgs<real>& a, bool exact, hipStream_t s) {
+  dim3 grid((unsigned)((a.nyz + 255) / 256), (unsigned)a.nx);
+  if (exact) k_update_e<real, true><<<grid, 256, 0, s>>>(a);
+  else k_update_e<real, false><<<grid, 256, 0, s>>>(a);
+}
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename real>
+void launch_inject(real* const* fields_dev, const SourceGroupsDev& g, int64_t n_steps, int64_t step,
+                   hipStream_t s) {
+  if (g.n_groups == 0) return;
+  k_inject<real><<<blocks_for(g.n_groups, 64), 64, 0, s>>>(fields_dev, g.comp, g.cell, g.begin,
+                                                           g.amp, g.wave_row, g.wave, n_steps, step,
+                                                           g.n_groups);
+}
+
+template <typename real>
+void launch_ade_capture(real* const* fields_dev, const int64_t* cells, int64_t n, real* En,
+                        hipStream_t s) {
+  k_ade_capture<real><<<blocks_for(n, 256), 256, 0, s>>>(fields_dev, cells, n, En);
+}
+
+template <typename real>
+void launch_ade_apply(real* const* fields_dev, const int64_t* cells, int64_t n, const real* En,
+                      real* Pcur, real* Pprev, const real* abc, int n_poles, real inv_eps,
+                      hipStream_t s) {
+  k_ade_apply<real><<<blocks_for(n, 256), 256, 0, s>>>(fields_dev, cells, n, En, Pcur, Pprev, abc,
+                                                       n_poles, inv_eps);
+}
+
+template <typename real>
+void launch_gather_box(real* const* fields_dev, const int32_t* comps, int n_comp, const BoxDev& box,
+                       int64_t nyz, int nz, real* out, hipStream_t s) {
+  const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
+  k_gather_box<real><<<blocks_for(nbox * n_comp, 256), 256, 0, s>>>(fields_dev, comps, n_comp, box,
+                                                                     nyz, nz, out);
+}
+
+template <typename real>
+void launch_dft_fold(const real* ring, int64_t nbox, int n_comp, const int32_t* is_h, int count,
+                     int64_t sample0, const double* phase_e, const double* phase_h, int n_freq,
+                     double dt, double* acc, hipStream_t s) {
+  k_dft_fold<real><<<blocks_for(nbox * n_comp, 256), 256, 0, s>>>(
+      ring, nbox, n_comp, is_h, count, sample0, phase_e, phase_h, n_freq, dt, acc);
+}
+
+template <typename real>
+void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int ny, int nz,
+                 double dA, double dt, double* partial, int n_part, double* total, hipStream_t s) {
+  // S . n: x: Ey Hz - Ez Hy; y: Ez Hx - Ex Hz; z: Ex Hy - Ey Hx (simulation.py:1446-1457)
+  static const int t1[3][4] = {{1, 5, 2, 4}, {2, 3, 0, 5}, {0, 4, 1, 3}};
+  const int* c = t1[axis];
+  k_flux_partial<real><<<n_part, 256, 0, s>>>(fields_host[c[0]], fields_host[c[1]],
+                                              fields_host[c[2]], fields_host[c[3]], axis, index, nx,
+                                              ny, nz, partial);
+  k_flux_finish<<<1, 64, 0, s>>>(partial, n_part, dA, dt, total);
+}
+
+#define PF_INSTANTIATE(real)                                                                       \
+  template void launch_generic_h<real>(const StepArgs<real>&, bool, hipStream_t);                  \
+  template void launch_generic_e<real>(const StepArgs<real>&, bool, hipStream_t);                  \
+  template void launch_inject<real>(real* const*, const SourceGroupsDev&, int64_t, int64_t,        \
+                                    hipStream_t);                                                  \
+  template void launch_ade_capture<real>(real* const*, const int64_t*, int64_t, real*,             \
+                                         hipStream_t);                                             \
+  template void launch_ade_apply<real>(real* const*, const int64_t*, int64_t, const real*, real*,  \
+                                       real*, const real*, int, real, hipStream_t);                \
+  template void launch_gather_box<real>(real* const*, const int32_t*, int, const BoxDev&, int64_t, \
+                                        int, real*, hipStream_t);                                  \
+  template void launch_dft_fold<real>(const real*, int64_t, int, const int32_t*, int, int64_t,     \
+                                      const double*, const double*, int, double, double*,          \
+                                      hipStream_t);                                                \
+  template void launch_flux<real>(real* const*, int, int64_t, int, int, int, double, double,       \
+                                  double*, int, double*, hipStream_t);
+
+PF_INSTANTIATE(float)
+PF_INSTANTIATE(double)
+
+}  // namespace pfhip

@@ -1,0 +1,660 @@
+// libpfhip: C ABI (include/pfhip.h) over the gfx950 FDTD kernels.
+//
+// A handle owns every device buffer of one simulation -- fields, the E
+// coefficient, compact CPML psi, source tables, ADE state, DFT accumulators
+// and flux totals -- on its own HIP stream (default priority). The host
+// crosses into device memory only at pf_create (upload), pf_read_* /
+// pf_dft_read (download) and pf_write_field, mirroring the reference's
+// VRAM-resident CudaStepper (rust/src/cuda.rs:271-574) but with ADE, DFT and
+// flux accumulation kept on the device as well.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pfhip.h"
+#include "pfhip_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+struct HipError {
+  int code;
+  std::string msg;
+};
+
+#define HIP_CHECK(expr)                                                                    \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      throw HipError{PF_E_HIP, std::string("HIP error: ") + hipGetErrorString(_e) + " (" + \
+                                   #expr + ")"};                                           \
+  } while (0)
+
+struct ArgError {
+  std::string msg;
+};
+
+// Device allocations of one handle, freed together.
+class DeviceArena {
+ public:
+  ~DeviceArena() { release(); }
+  void* alloc(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, bytes));
+    ptrs_.push_back(p);
+    return p;
+  }
+  template <typename T>
+  T* upload(const T* host, size_t n, hipStream_t s) {
+    if (n == 0) return nullptr;
+    T* d = static_cast<T*>(alloc(n * sizeof(T)));
+    (void)s;
+    // synchronous: host tables may be temporaries of the caller
+    HIP_CHECK(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+  template <typename T>
+  T* zeros(size_t n, hipStream_t s) {
+    if (n == 0) return nullptr;
+    T* d = static_cast<T*>(alloc(n * sizeof(T)));
+    HIP_CHECK(hipMemsetAsync(d, 0, n * sizeof(T), s));
+    return d;
+  }
+  void release() {
+    for (void* p : ptrs_) (void)hipFree(p);
+    ptrs_.clear();
+  }
+
+ private:
+  std::vector<void*> ptrs_;
+};
+
+using pfhip::BoxDev;
+using pfhip::SourceGroupsDev;
+using pfhip::StepArgs;
+
+BoxDev check_box(const pf_box& b, const int64_t shape[3]) {
+  BoxDev out;
+  for (int a = 0; a < 3; ++a) {
+    if (b.count[a] < 1 || b.stride[a] < 1 || b.start[a] < 0 ||
+        b.start[a] + (b.count[a] - 1) * b.stride[a] >= shape[a])
+      throw ArgError{"box exceeds the grid on axis " + std::to_string(a)};
+    out.start[a] = b.start[a];
+    out.count[a] = b.count[a];
+    out.stride[a] = b.stride[a];
+  }
+  return out;
+}
+
+}  // namespace
+
+struct pf_handle {
+  virtual ~pf_handle() = default;
+  virtual void advance(int64_t step0, int64_t n) = 0;
+  virtual void read_box(int comp, const pf_box& box, void* dst) = 0;
+  virtual void write_field(int comp, const void* src) = 0;
+  virtual void dft_read(int mon, int slot, double* dst) = 0;
+  virtual void flux_read(double* dst) = 0;
+  virtual void timing_enable(bool on) = 0;
+  virtual void timing_read(double* ms, int64_t* launches) = 0;
+  int64_t shape[3];
+};
+
+namespace {
+
+template <typename real>
+class Stepper final : public pf_handle {
+ public:
+  explicit Stepper(const pf_desc& d);
+  ~Stepper() override {
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      for (Timed& t : pending_) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+      }
+      for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+      arena_.release();
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+  void advance(int64_t step0, int64_t n) override;
+  void read_box(int comp, const pf_box& box, void* dst) override;
+  void write_field(int comp, const void* src) override;
+  void dft_read(int mon, int slot, double* dst) override;
+  void flux_read(double* dst) override;
+  void timing_enable(bool on) override;
+  void timing_read(double* ms, int64_t* launches) override;
+
+ private:
+  struct Ade {
+    int64_t n;
+    int n_poles;
+    int64_t* cells;
+    real* abc;
+    real inv_eps;
+    real* En;
+    real* Pcur;
+    real* Pprev;
+  };
+  struct Dft {
+    int n_comp;
+    int32_t comps[6];
+    int32_t* comps_dev;
+    int32_t* is_h_dev;
+    BoxDev box;
+    int64_t nbox;
+    int n_freq;
+    int interval;
+    int64_t n_samples;
+    double* phase_e;
+    double* phase_h;
+    double dt;
+    double* acc;     // [n_comp][n_freq][nbox][2]
+    real* ring;      // [K][n_comp][nbox]
+    int K;
+    int count;       // buffered samples
+    int64_t sample0; // sample index of ring slot 0
+  };
+  struct Flux {
+    int axis;
+    int64_t index;
+    double dA, dt;
+    int n_part;
+    double* partial;
+    double* total;
+  };
+
+  void step_once(int64_t step);
+  void capture_dft(int64_t step);
+  void fold(Dft& m);
+  real* staging(size_t elems);
+  template <typename F>
+  void timed(int cls, F&& launch);
+  void collect_timing();
+
+  DeviceArena arena_;
+  hipStream_t stream_ = nullptr;
+  bool exact_;
+  int64_t n_steps_;
+  StepArgs<real> args_;
+  real** fields_dev_ = nullptr;  // device copy of the six field pointers
+  SourceGroupsDev src_h_, src_e_;
+  std::vector<Ade> ade_;
+  std::vector<Dft> dft_;
+  std::vector<Flux> flux_;
+  real* stage_ = nullptr;
+  size_t stage_elems_ = 0;
+  int32_t* comp_index_dev_ = nullptr;  // {0, 1, 2, 3, 4, 5}
+  // event timing
+  struct Timed {
+    hipEvent_t a, b;
+    int cls;
+  };
+  bool timing_ = false;
+  std::vector<Timed> pending_;
+  std::vector<hipEvent_t> pool_;
+  double t_ms_[PF_NKCLASS] = {0, 0, 0, 0};
+  int64_t t_count_[PF_NKCLASS] = {0, 0, 0, 0};
+};
+
+template <typename real>
+Stepper<real>::Stepper(const pf_desc& d) {
+  for (int a = 0; a < 3; ++a) {
+    shape[a] = d.shape[a];
+    if (d.shape[a] < 1) throw ArgError{"shape must be positive"};
+  }
+  if (d.shape[0] * d.shape[1] * d.shape[2] >= (int64_t(1) << 31))
+    throw ArgError{"grids of 2^31 nodes or more are not supported"};
+  if (d.shape[1] * d.shape[2] >= (int64_t(1) << 31) || d.shape[0] > 65535 * 16)
+    throw ArgError{"grid extent beyond the launch limits"};
+  if (!d.ce) throw ArgError{"ce is required"};
+  for (int a = 0; a < 3; ++a)
+    if (!d.b_e[a] || !d.c_e[a] || !d.b_h[a] || !d.c_h[a])
+      throw ArgError{"CPML coefficient arrays are required on every axis"};
+  if (d.n_steps < 0 || d.n_src < 0 || d.n_wave < 0 || d.n_ade < 0 || d.n_dft < 0 || d.n_flux < 0)
+    throw ArgError{"negative table length"};
+  if (d.n_src > 0 && (!d.src_comp || !d.src_cell || !d.src_amp || !d.src_wave || !d.wave))
+    throw ArgError{"source tables missing"};
+
+  HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  exact_ = d.exact != 0;
+  n_steps_ = d.n_steps;
+  const int nx = (int)d.shape[0], ny = (int)d.shape[1], nz = (int)d.shape[2];
+  const int64_t ncell = (int64_t)nx * ny * nz;
+  std::memset(&args_, 0, sizeof(args_));
+  args_.nx = nx;
+  args_.ny = ny;
+  args_.nz = nz;
+  args_.nyz = (int64_t)ny * nz;
+  for (int a = 0; a < 3; ++a) {
+    args_.d[a] = (real)d.cell[a];
+    args_.inv_d[a] = (real)(1.0 / d.cell[a]);
+  }
+  args_.ch = (real)d.ch;
+  for (int c = 0; c < 6; ++c) args_.f[c] = arena_.zeros<real>(ncell, stream_);
+  args_.ce = arena_.upload(static_cast<const real*>(d.ce), ncell, stream_);
+
+  // CPML tables: coefficients rounded to the field dtype, compact maps.
+  const int n_axis[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    const int n = n_axis[a];
+    std::vector<real> be(n), ce(n), bh(n), ch(n);
+    std::vector<int> me(n, -1), mh(n, -1);
+    int pe = 0, ph = 0;
+    for (int i = 0; i < n; ++i) {
+      be[i] = (real)d.b_e[a][i];
+      ce[i] = (real)d.c_e[a][i];
+      bh[i] = (real)d.b_h[a][i];
+      ch[i] = (real)d.c_h[a][i];
+      if (d.c_e[a][i] != 0.0) me[i] = pe++;
+      if (d.c_h[a][i] != 0.0) mh[i] = ph++;
+    }
+    args_.b_e[a] = arena_.upload(be.data(), n, stream_);
+    args_.c_e[a] = arena_.upload(ce.data(), n, stream_);
+    args_.b_h[a] = arena_.upload(bh.data(), n, stream_);
+    args_.c_h[a] = arena_.upload(ch.data(), n, stream_);
+    args_.map_e[a] = arena_.upload(me.data(), n, stream_);
+    args_.map_h[a] = arena_.upload(mh.data(), n, stream_);
+    args_.pe[a] = pe;
+    args_.ph[a] = ph;
+  }
+  // psi slot -> derivative axis
+  static const int slot_axis[12] = {1, 2, 2, 0, 0, 1, 1, 2, 2, 0, 0, 1};
+  for (int s = 0; s < pfhip::PSI_COUNT; ++s) {
+    const int ax = slot_axis[s];
+    const int pc = s < 6 ? args_.pe[ax] : args_.ph[ax];
+    const int64_t n = ncell / n_axis[ax] * pc;
+    args_.psi[s] = arena_.zeros<real>(n, stream_);
+  }
+  fields_dev_ = arena_.upload(args_.f, 6, stream_);
+  const int32_t comp_index[6] = {0, 1, 2, 3, 4, 5};
+  comp_index_dev_ = arena_.upload(comp_index, 6, stream_);
+
+  // Sources: consecutive equal (comp, cell) entries form one group.
+  std::vector<int32_t> gcomp[2], gbeg[2];
+  std::vector<int64_t> gcell[2];
+  std::vector<double> amp[2];
+  std::vector<int32_t> wrow[2];
+  for (int s = 0; s < d.n_src; ++s) {
+    const int comp = d.src_comp[s];
+    if (comp < 0 || comp > 5) throw ArgError{"source component out of range"};
+    if (d.src_cell[s] < 0 || d.src_cell[s] >= ncell) throw ArgError{"source cell out of range"};
+    if (d.src_wave[s] < 0 || d.src_wave[s] >= d.n_wave) throw ArgError{"source waveform row out of range"};
+    const int side = comp >= 3 ? 1 : 0;
+    if (gcomp[side].empty() || gcomp[side].back() != comp || gcell[side].back() != d.src_cell[s]) {
+      gcomp[side].push_back(comp);
+      gcell[side].push_back(d.src_cell[s]);
+      gbeg[side].push_back((int32_t)amp[side].size());
+    }
+    amp[side].push_back(d.src_amp[s]);
+    wrow[side].push_back(d.src_wave[s]);
+  }
+  double* wave = arena_.upload(d.wave, (size_t)d.n_wave * d.n_steps, stream_);
+  SourceGroupsDev* groups[2] = {&src_e_, &src_h_};
+  for (int side = 0; side < 2; ++side) {
+    SourceGroupsDev& g = *groups[side];
+    g.n_groups = (int32_t)gcomp[side].size();
+    if (g.n_groups == 0) continue;
+    gbeg[side].push_back((int32_t)amp[side].size());
+    g.comp = arena_.upload(gcomp[side].data(), gcomp[side].size(), stream_);
+    g.cell = arena_.upload(gcell[side].data(), gcell[side].size(), stream_);
+    g.begin = arena_.upload(gbeg[side].data(), gbeg[side].size(), stream_);
+    g.amp = arena_.upload(amp[side].data(), amp[side].size(), stream_);
+    g.wave_row = arena_.upload(wrow[side].data(), wrow[side].size(), stream_);
+    g.wave = wave;
+  }
+
+  // ADE media.
+  for (int m = 0; m < d.n_ade; ++m) {
+    const pf_ade_desc& a = d.ade[m];
+    if (a.n_cells <= 0) continue;
+    if (a.n_poles < 0 || (a.n_poles > 0 && !a.abc) || !a.cells) throw ArgError{"bad ADE table"};
+    for (int64_t t = 0; t < a.n_cells; ++t)
+      if (a.cells[t] < 0 || a.cells[t] >= ncell) throw ArgError{"ADE cell out of range"};
+    Ade st;
+    st.n = a.n_cells;
+    st.n_poles = a.n_poles;
+    st.cells = arena_.upload(a.cells, a.n_cells, stream_);
+    std::vector<real> abc(3 * (size_t)a.n_poles);
+    for (size_t i = 0; i < abc.size(); ++i) abc[i] = (real)a.abc[i];
+    st.abc = arena_.upload(abc.data(), abc.size(), stream_);
+    st.inv_eps = (real)a.inv_eps;
+    st.En = arena_.zeros<real>(3 * a.n_cells, stream_);
+    st.Pcur = arena_.zeros<real>(3 * (size_t)a.n_poles * a.n_cells, stream_);
+    st.Pprev = arena_.zeros<real>(3 * (size_t)a.n_poles * a.n_cells, stream_);
+    ade_.push_back(st);
+  }
+
+  // DFT monitors: ring of K samples, folded into complex128 accumulators.
+  for (int m = 0; m < d.n_dft; ++m) {
+    const pf_dft_desc& s = d.dft[m];
+    Dft st;
+    if (s.n_comp < 1 || s.n_comp > 6 || s.n_freq < 1 || s.interval < 1 || s.n_samples < 0)
+      throw ArgError{"bad DFT monitor description"};
+    st.n_comp = s.n_comp;
+    int32_t is_h[6];
+    for (int c = 0; c < s.n_comp; ++c) {
+      if (s.comps[c] < 0 || s.comps[c] > 5) throw ArgError{"DFT component out of range"};
+      st.comps[c] = s.comps[c];
+      is_h[c] = s.comps[c] >= 3;
+    }
+    st.comps_dev = arena_.upload(st.comps, s.n_comp, stream_);
+    st.is_h_dev = arena_.upload(is_h, s.n_comp, stream_);
+    st.box = check_box(s.box, d.shape);
+    st.nbox = st.box.count[0] * st.box.count[1] * st.box.count[2];
+    st.n_freq = s.n_freq;
+    st.interval = s.interval;
+    st.n_samples = s.n_samples;
+    st.phase_e = arena_.upload(s.phase_e, (size_t)s.n_samples * s.n_freq * 2, stream_);
+    st.phase_h = arena_.upload(s.phase_h, (size_t)s.n_samples * s.n_freq * 2, stream_);
+    st.dt = s.dt;
+    st.acc = arena_.zeros<double>((size_t)s.n_comp * s.n_freq * st.nbox * 2, stream_);
+    const int64_t sample_bytes = (int64_t)s.n_comp * st.nbox * (int64_t)sizeof(real);
+    const int64_t budget = 256ll << 20;
+    st.K = (int)std::max<int64_t>(1, std::min<int64_t>(32, budget / std::max<int64_t>(sample_bytes, 1)));
+    st.ring = static_cast<real*>(arena_.alloc((size_t)st.K * sample_bytes));
+    st.count = 0;
+    st.sample0 = 0;
+    dft_.push_back(st);
+  }
+
+  for (int m = 0; m < d.n_flux; ++m) {
+    const pf_flux_desc& s = d.flux[m];
+    if (s.axis < 0 || s.axis > 2 || s.index < 0 || s.index >= d.shape[s.axis])
+      throw ArgError{"flux plane outside the grid"};
+    Flux st;
+    st.axis = s.axis;
+    st.index = s.index;
+    st.dA = s.dA;
+    st.dt = s.dt;
+    const int64_t plane = ncell / d.shape[s.axis];
+    st.n_part = (int)std::min<int64_t>(512, (plane + 255) / 256);
+    st.partial = arena_.zeros<double>(st.n_part, stream_);
+    st.total = arena_.zeros<double>(1, stream_);
+    flux_.push_back(st);
+  }
+
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <typename real>
+template <typename F>
+void Stepper<real>::timed(int cls, F&& launch) {
+  if (!timing_) {
+    launch();
+    return;
+  }
+  hipEvent_t ev[2];
+  for (hipEvent_t& e : ev) {
+    if (pool_.empty()) {
+      HIP_CHECK(hipEventCreate(&e));
+    } else {
+      e = pool_.back();
+      pool_.pop_back();
+    }
+  }
+  HIP_CHECK(hipEventRecord(ev[0], stream_));
+  launch();
+  HIP_CHECK(hipEventRecord(ev[1], stream_));
+  pending_.push_back({ev[0], ev[1], cls});
+}
+
+template <typename real>
+void Stepper<real>::collect_timing() {
+  if (pending_.empty()) return;
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  for (Timed& t : pending_) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+    t_ms_[t.cls] += ms;
+    t_count_[t.cls] += 1;
+    pool_.push_back(t.a);
+    pool_.push_back(t.b);
+  }
+  pending_.clear();
+}
+
+template <typename real>
+void Stepper<real>::timing_enable(bool on) {
+  collect_timing();
+  timing_ = on;
+  for (int c = 0; c < PF_NKCLASS; ++c) {
+    t_ms_[c] = 0.0;
+    t_count_[c] = 0;
+  }
+}
+
+template <typename real>
+void Stepper<real>::timing_read(double* ms, int64_t* launches) {
+  collect_timing();
+  for (int c = 0; c < PF_NKCLASS; ++c) {
+    if (ms) ms[c] = t_ms_[c];
+    if (launches) launches[c] = t_count_[c];
+  }
+}
+
+template <typename real>
+void Stepper<real>::step_once(int64_t step) {
+  timed(PF_K_YEE_H, [&] { pfhip::launch_generic_h<real>(args_, exact_, stream_); });
+  timed(PF_K_AUX, [&] {
+    pfhip::launch_inject<real>(fields_dev_, src_h_, n_steps_, step, stream_);
+    for (Ade& a : ade_) pfhip::launch_ade_capture<real>(fields_dev_, a.cells, a.n, a.En, stream_);
+  });
+  timed(PF_K_YEE_E, [&] { pfhip::launch_generic_e<real>(args_, exact_, stream_); });
+  timed(PF_K_AUX, [&] {
+    for (Ade& a : ade_)
+      pfhip::launch_ade_apply<real>(fields_dev_, a.cells, a.n, a.En, a.Pcur, a.Pprev, a.abc,
+                                    a.n_poles, a.inv_eps, stream_);
+    pfhip::launch_inject<real>(fields_dev_, src_e_, n_steps_, step, stream_);
+    for (Flux& f : flux_)
+      pfhip::launch_flux<real>(args_.f, f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA, f.dt,
+                               f.partial, f.n_part, f.total, stream_);
+  });
+  timed(PF_K_AUX, [&] { capture_dft(step); });
+}
+
+template <typename real>
+void Stepper<real>::capture_dft(int64_t step) {
+  for (Dft& m : dft_) {
+    if (step % m.interval != 0) continue;
+    const int64_t sample = step / m.interval;
+    if (sample >= m.n_samples) throw ArgError{"DFT sample beyond the phase table"};
+    if (m.count == 0) m.sample0 = sample;
+    else if (sample != m.sample0 + m.count) throw ArgError{"DFT samples must be consecutive"};
+    pfhip::launch_gather_box<real>(fields_dev_, m.comps_dev, m.n_comp, m.box, args_.nyz, args_.nz,
+                                   m.ring + (int64_t)m.count * m.n_comp * m.nbox, stream_);
+    if (++m.count == m.K) fold(m);
+  }
+}
+
+template <typename real>
+void Stepper<real>::fold(Dft& m) {
+  if (m.count == 0) return;
+  pfhip::launch_dft_fold<real>(m.ring, m.nbox, m.n_comp, m.is_h_dev, m.count, m.sample0, m.phase_e,
+                               m.phase_h, m.n_freq, m.dt, m.acc, stream_);
+  m.sample0 += m.count;
+  m.count = 0;
+}
+
+template <typename real>
+void Stepper<real>::advance(int64_t step0, int64_t n) {
+  if (step0 < 0 || n < 0) throw ArgError{"negative step range"};
+  if ((src_h_.n_groups || src_e_.n_groups) && step0 + n > n_steps_)
+    throw ArgError{"step range exceeds the source waveform table"};
+  for (int64_t s = step0; s < step0 + n; ++s) {
+    step_once(s);
+    if (timing_ && pending_.size() > 4096) collect_timing();
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  collect_timing();
+}
+
+template <typename real>
+real* Stepper<real>::staging(size_t elems) {
+  if (elems > stage_elems_) {
+    stage_ = static_cast<real*>(arena_.alloc(elems * sizeof(real)));
+    stage_elems_ = elems;
+  }
+  return stage_;
+}
+
+template <typename real>
+void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
+  if (comp < 0 || comp > 5) throw ArgError{"component out of range"};
+  if (!dst) throw ArgError{"null destination"};
+  const BoxDev b = check_box(box, shape);
+  const int64_t nbox = b.count[0] * b.count[1] * b.count[2];
+  real* out = staging(nbox);
+  pfhip::launch_gather_box<real>(fields_dev_, comp_index_dev_ + comp, 1, b, args_.nyz, args_.nz, out, stream_);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(dst, out, nbox * sizeof(real), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <typename real>
+void Stepper<real>::write_field(int comp, const void* src) {
+  if (comp < 0 || comp > 5) throw ArgError{"component out of range"};
+  if (!src) throw ArgError{"null source"};
+  const int64_t ncell = shape[0] * shape[1] * shape[2];
+  HIP_CHECK(hipMemcpyAsync(args_.f[comp], src, ncell * sizeof(real), hipMemcpyHostToDevice, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <typename real>
+void Stepper<real>::dft_read(int mon, int slot, double* dst) {
+  if (mon < 0 || mon >= (int)dft_.size()) throw ArgError{"DFT monitor index out of range"};
+  Dft& m = dft_[mon];
+  if (slot < 0 || slot >= m.n_comp) throw ArgError{"DFT component slot out of range"};
+  if (!dst) throw ArgError{"null destination"};
+  fold(m);
+  const size_t n = (size_t)m.n_freq * m.nbox * 2;
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(dst, m.acc + (size_t)slot * n, n * sizeof(double),
+                           hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <typename real>
+void Stepper<real>::flux_read(double* dst) {
+  if (flux_.empty()) return;
+  if (!dst) throw ArgError{"null destination"};
+  for (size_t m = 0; m < flux_.size(); ++m)
+    HIP_CHECK(hipMemcpyAsync(dst + m, flux_[m].total, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return PF_OK;
+  } catch (const HipError& e) {
+    return fail(e.code, e.msg);
+  } catch (const ArgError& e) {
+    return fail(PF_E_ARG, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(PF_E_LIMIT, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(PF_E_STATE, e.what());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_create(const pf_desc* desc, pf_handle** out) {
+  if (!desc || !out) return fail(PF_E_ARG, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    if (desc->dtype == PF_F32) *out = new Stepper<float>(*desc);
+    else if (desc->dtype == PF_F64) *out = new Stepper<double>(*desc);
+    else throw ArgError{"dtype must be PF_F32 or PF_F64"};
+  });
+}
+
+int pf_destroy(pf_handle* h) {
+  return guarded([&] { delete h; });
+}
+
+int pf_advance(pf_handle* h, int64_t step0, int64_t n) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->advance(step0, n); });
+}
+
+int pf_read_field(pf_handle* h, int32_t comp, void* dst) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  pf_box b;
+  for (int a = 0; a < 3; ++a) {
+    b.start[a] = 0;
+    b.count[a] = h->shape[a];
+    b.stride[a] = 1;
+  }
+  return guarded([&] { h->read_box(comp, b, dst); });
+}
+
+int pf_read_box(pf_handle* h, int32_t comp, const pf_box* box, void* dst) {
+  if (!h || !box) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { h->read_box(comp, *box, dst); });
+}
+
+int pf_write_field(pf_handle* h, int32_t comp, const void* src) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->write_field(comp, src); });
+}
+
+int pf_dft_read(pf_handle* h, int32_t mon, int32_t slot, double* dst) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->dft_read(mon, slot, dst); });
+}
+
+int pf_flux_read(pf_handle* h, double* dst) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->flux_read(dst); });
+}
+
+int pf_timing_enable(pf_handle* h, int32_t on) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->timing_enable(on != 0); });
+}
+
+int pf_timing_read(pf_handle* h, double* ms, int64_t* launches) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->timing_read(ms, launches); });
+}
+
+int pf_device_info(char* name, int32_t name_len, int64_t* free_mb, int64_t* total_mb) {
+  return guarded([&] {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    size_t fr = 0, tot = 0;
+    HIP_CHECK(hipMemGetInfo(&fr, &tot));
+    if (name && name_len > 0) {
+      std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (free_mb) *free_mb = (int64_t)(fr >> 20);
+    if (total_mb) *total_mb = (int64_t)(tot >> 20);
+  });
+}
+
+const char* pf_last_error(void) { return g_last_error.c_str(); }
+
+int32_t pf_abi_version(void) { return PFHIP_ABI_VERSION; }
+
+}  // extern "C"

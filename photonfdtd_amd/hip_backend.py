@@ -1,0 +1,376 @@
+"""``backend="hip"``: the Python driver of the gfx950 stepper.
+
+Same shape as the reference's Rust/CUDA driver (``src/photonfdtd/rustbackend.py``):
+
+* ``validate``  -- feature gate with ``NotImplementedError`` (``:85-120``);
+* ``build_tables`` -- ce, CPML tables, soft-source tables (``:135-166``), plus
+  the ADE, DFT-phase and flux tables the reference keeps on the host;
+* ``HipEngine`` -- ``advance(step0, n)`` / ``get(comp)`` with a cache cleared
+  on every advance (``:222-235,297-304``);
+* ``run_loop`` -- chunked stepping with progress ticks and FieldMonitor
+  recording (``:310-447``). Unlike the reference, DFT and flux accumulate on
+  the device every step, so the loop stops only at FieldMonitor steps.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+from typing import Dict, List
+
+import numpy as np
+
+from . import _native as N
+from .constants import EPS_0, MU_0
+from .monitors import DFTMonitor, FieldMonitor, FluxMonitor
+from .simulation import COMPUTE_KEYS, Result
+
+_AXIS = {"x": 0, "y": 1, "z": 2}
+
+
+def validate(sim) -> np.dtype:
+    """Feature gate; returns the single compute dtype."""
+    dts = {np.dtype(sim.dtypes[k]) for k in COMPUTE_KEYS}
+    if dts not in ({np.dtype(np.float64)}, {np.dtype(np.float32)}):
+        raise NotImplementedError("backend='hip' needs one uniform compute precision "
+                                  "('float32' or 'float64'; monitor storage may differ).")
+    if sim.subpixel:
+        raise NotImplementedError("subpixel=True is not supported on backend='hip'.")
+    if sim.particle_sources:
+        raise NotImplementedError("ChargedParticle sources are not supported on backend='hip'.")
+    for m in sim.monitors:
+        if isinstance(m, FieldMonitor) and m.compression is not None:
+            raise NotImplementedError("FieldMonitor(compression=) is a host-side store not "
+                                      "wired to backend='hip'; drop compression.")
+    for s in sim.sources:
+        if s.component not in N.COMP_CODE:
+            raise ValueError(f"unknown source component {s.component!r}")
+    return dts.pop()
+
+
+def exact_mode(dtype) -> bool:
+    """Reference operation order: on for float64, off for float32 unless
+    ``PHOTONFDTD_HIP_EXACT`` says otherwise."""
+    env = os.environ.get("PHOTONFDTD_HIP_EXACT")
+    if env is not None:
+        return env == "1"
+    return np.dtype(dtype) == np.float64
+
+
+def _snap_box(shape, ds, plane=None, zplane=None):
+    """Box of ``field[::ds, ::ds, ::ds]`` with an optional single plane."""
+    start = [0, 0, 0]
+    count = [(n + ds - 1) // ds for n in shape]
+    stride = [ds, ds, ds]
+    if zplane is not None:                 # FieldMonitor plane_z keeps x/y stride
+        start[2], count[2], stride[2] = zplane, 1, 1
+    if plane is not None:
+        ax, ci = plane
+        start[ax], count[ax], stride[ax] = ci, 1, 1
+    return start, count, stride
+
+
+def build_tables(sim, npdt) -> Dict:
+    grid = sim.grid
+    dt = sim.dt
+    n_steps = sim.n_steps
+    nx, ny, nz = grid.shape
+    eps = np.asarray(sim.eps_r, dtype=sim.dtypes["eps_r"])
+    t: Dict = {}
+    t["ce"] = np.ascontiguousarray((dt / (eps * EPS_0)).astype(npdt, copy=False))
+    t["cell"] = tuple(d if d > 0 else 1.0 for d in grid.cell_size)
+    t["cpml"] = [np.ascontiguousarray(a, dtype=np.float64)
+                 for arrs in (sim._b_e, sim._c_e, sim._b_h, sim._c_h) for a in arrs]
+
+    # Soft sources: waveform rows keyed by (waveform object, E/H Yee time).
+    steps = np.arange(n_steps)
+    rows, row_of = [], {}
+    entries = []
+    for order, src in enumerate(sim.sources):
+        i, j, k = grid.index_at(src.position)
+        comp = N.COMP_CODE[src.component]
+        is_h = comp >= 3
+        key = (id(src.waveform), is_h)
+        if key not in row_of:
+            row_of[key] = len(rows)
+            tt = (steps + (0.5 if is_h else 1.0)) * dt
+            rows.append(np.asarray(src.waveform(tt), dtype=np.float64))
+        entries.append((comp, (i * ny + j) * nz + k, order, float(src.amplitude), row_of[key]))
+    entries.sort(key=lambda e: (e[0], e[1], e[2]))
+    t["src_comp"] = np.array([e[0] for e in entries], dtype=np.int32)
+    t["src_cell"] = np.array([e[1] for e in entries], dtype=np.int64)
+    t["src_amp"] = np.array([e[3] for e in entries], dtype=np.float64)
+    t["src_wave"] = np.array([e[4] for e in entries], dtype=np.int32)
+    t["wave"] = (np.ascontiguousarray(np.stack(rows)) if rows
+                 else np.zeros((1, max(n_steps, 1)), dtype=np.float64))
+
+    t["ade"] = sim.ade_media()
+
+    dfts = []
+    for m in sim.monitors:
+        if not isinstance(m, DFTMonitor):
+            continue
+        pl = m.plane()
+        plane = None if pl is None else (pl[0], grid.nearest_index(pl[0], pl[1]))
+        omega = 2.0 * np.pi * np.asarray(m.freqs, dtype=np.float64)
+        sample_steps = range(0, n_steps, m.interval)
+        ph_e = np.empty((len(sample_steps), omega.size), dtype=np.complex128)
+        ph_h = np.empty_like(ph_e)
+        for q, step in enumerate(sample_steps):
+            ph_e[q] = np.exp(1j * omega * ((step + 1.0) * dt))
+            ph_h[q] = np.exp(1j * omega * ((step + 0.5) * dt))
+        dfts.append(dict(mon=m, box=_snap_box(grid.shape, m.downsample, plane=plane),
+                         n_samples=len(sample_steps), phase_e=ph_e, phase_h=ph_h))
+    t["dft"] = dfts
+
+    fluxes = []
+    for m in sim.monitors:
+        if not isinstance(m, FluxMonitor):
+            continue
+        ax = _AXIS[m.plane_axis]
+        idx = 0 if grid.shape[ax] == 1 else grid.nearest_index(ax, m.plane_position)
+        ext = [grid.cell_size[a] if grid.shape[a] > 1 else 1.0 for a in range(3)]
+        dA = float(np.prod([ext[a] for a in range(3) if a != ax]))
+        fluxes.append(dict(mon=m, axis=ax, index=idx, dA=dA))
+    t["flux"] = fluxes
+    return t
+
+
+class HipEngine:
+    """One device-resident simulation behind the C ABI."""
+
+    def __init__(self, sim, npdt, exact: bool):
+        self.lib = N.load_library()
+        self.npdt = np.dtype(npdt).type
+        self.shape = sim.grid.shape
+        self.t = t = build_tables(sim, self.npdt)
+        keep: List = []                       # host arrays alive until pf_create returns
+        d = N.PfDesc()
+        d.dtype = N.PF_F64 if self.npdt == np.float64 else N.PF_F32
+        d.exact = 1 if exact else 0
+        d.shape[:] = self.shape
+        d.cell[:] = t["cell"]
+        d.ch = sim.dt / MU_0
+        d.ce = t["ce"].ctypes.data
+        cp = t["cpml"]
+        for a in range(3):
+            d.b_e[a] = N.as_ptr(cp[a], C.c_double)
+            d.c_e[a] = N.as_ptr(cp[3 + a], C.c_double)
+            d.b_h[a] = N.as_ptr(cp[6 + a], C.c_double)
+            d.c_h[a] = N.as_ptr(cp[9 + a], C.c_double)
+        d.n_steps = t["wave"].shape[1]
+        d.n_src = t["src_comp"].size
+        d.src_comp = N.as_ptr(t["src_comp"], C.c_int32)
+        d.src_cell = N.as_ptr(t["src_cell"], C.c_int64)
+        d.src_amp = N.as_ptr(t["src_amp"], C.c_double)
+        d.src_wave = N.as_ptr(t["src_wave"], C.c_int32)
+        d.n_wave = t["wave"].shape[0]
+        d.wave = N.as_ptr(t["wave"], C.c_double)
+
+        ade = (N.PfAdeDesc * max(len(t["ade"]), 1))()
+        for n_, med in enumerate(t["ade"]):
+            cells = np.ascontiguousarray(med.cells, dtype=np.int64)
+            abc = np.ascontiguousarray(med.abc, dtype=np.float64)
+            keep += [cells, abc]
+            ade[n_].n_cells = cells.size
+            ade[n_].cells = N.as_ptr(cells, C.c_int64)
+            ade[n_].n_poles = abc.shape[0]
+            ade[n_].abc = N.as_ptr(abc, C.c_double)
+            ade[n_].inv_eps = med.inv_eps
+        d.n_ade = len(t["ade"])
+        d.ade = ade
+
+        dft = (N.PfDftDesc * max(len(t["dft"]), 1))()
+        self.dft_slots = {}
+        for n_, spec in enumerate(t["dft"]):
+            m = spec["mon"]
+            comps = [N.COMP_CODE[c] for c in m.components]
+            if len(set(comps)) != len(comps) or not 1 <= len(comps) <= 6:
+                raise ValueError(f"DFTMonitor {m.name!r}: components must be distinct field names")
+            ph_e = np.ascontiguousarray(spec["phase_e"]).view(np.float64)
+            ph_h = np.ascontiguousarray(spec["phase_h"]).view(np.float64)
+            keep += [ph_e, ph_h]
+            dd = dft[n_]
+            dd.n_comp = len(comps)
+            for s, c in enumerate(comps):
+                dd.comps[s] = c
+            start, count, stride = spec["box"]
+            dd.box.start[:], dd.box.count[:], dd.box.stride[:] = start, count, stride
+            dd.n_freq = len(m.freqs)
+            dd.interval = m.interval
+            dd.n_samples = spec["n_samples"]
+            dd.phase_e = N.as_ptr(ph_e, C.c_double)
+            dd.phase_h = N.as_ptr(ph_h, C.c_double)
+            dd.dt = sim.dt
+            self.dft_slots[m.name] = (n_, list(m.components), tuple(count), len(m.freqs))
+        d.n_dft = len(t["dft"])
+        d.dft = dft
+
+        flux = (N.PfFluxDesc * max(len(t["flux"]), 1))()
+        self.flux_names = []
+        for n_, spec in enumerate(t["flux"]):
+            flux[n_].axis = spec["axis"]
+            flux[n_].index = spec["index"]
+            flux[n_].dA = spec["dA"]
+            flux[n_].dt = sim.dt
+            self.flux_names.append(spec["mon"].name)
+        d.n_flux = len(t["flux"])
+        d.flux = flux
+
+        h = C.c_void_p()
+        N.check(self.lib, self.lib.pf_create(C.byref(d), C.byref(h)), "pf_create")
+        self.h = h
+        self._cache: Dict[str, np.ndarray] = {}
+        del keep
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self, step0: int, n: int) -> None:
+        self._cache.clear()
+        N.check(self.lib, self.lib.pf_advance(self.h, int(step0), int(n)), "pf_advance")
+
+    def get(self, comp: str) -> np.ndarray:
+        if comp not in self._cache:
+            out = np.empty(self.shape, dtype=self.npdt)
+            N.check(self.lib, self.lib.pf_read_field(self.h, N.COMP_CODE[comp],
+                                                     out.ctypes.data), "pf_read_field")
+            self._cache[comp] = out
+        return self._cache[comp]
+
+    def read_box(self, comp: str, start, count, stride) -> np.ndarray:
+        box = N.PfBox()
+        box.start[:], box.count[:], box.stride[:] = start, count, stride
+        out = np.empty(tuple(count), dtype=self.npdt)
+        N.check(self.lib, self.lib.pf_read_box(self.h, N.COMP_CODE[comp], C.byref(box),
+                                               out.ctypes.data), "pf_read_box")
+        return out
+
+    def set(self, comp: str, values: np.ndarray) -> None:
+        arr = np.ascontiguousarray(values, dtype=self.npdt).reshape(self.shape)
+        self._cache.pop(comp, None)
+        N.check(self.lib, self.lib.pf_write_field(self.h, N.COMP_CODE[comp], arr.ctypes.data),
+                "pf_write_field")
+
+    def dft(self, name: str) -> Dict[str, np.ndarray]:
+        mon, comps, count, n_freq = self.dft_slots[name]
+        out = {}
+        for slot, c in enumerate(comps):
+            buf = np.empty((n_freq,) + count, dtype=np.complex128)
+            N.check(self.lib, self.lib.pf_dft_read(self.h, mon, slot,
+                                                   buf.view(np.float64).ctypes.data_as(
+                                                       C.POINTER(C.c_double))), "pf_dft_read")
+            out[c] = buf
+        return out
+
+    def flux(self) -> Dict[str, float]:
+        if not self.flux_names:
+            return {}
+        buf = np.zeros(len(self.flux_names), dtype=np.float64)
+        N.check(self.lib, self.lib.pf_flux_read(self.h, N.as_ptr(buf, C.c_double)),
+                "pf_flux_read")
+        return {n: float(v) for n, v in zip(self.flux_names, buf)}
+
+
+def device_info():
+    lib = N.load_library()
+    name = C.create_string_buffer(256)
+    free_mb, total_mb = C.c_int64(), C.c_int64()
+    N.check(lib, lib.pf_device_info(name, 256, C.byref(free_mb), C.byref(total_mb)),
+            "pf_device_info")
+    return name.value.decode(), int(free_mb.value), int(total_mb.value)
+
+
+def run_loop(sim, engine: HipEngine) -> Result:
+    """Chunked stepping, FieldMonitor recording and Result assembly."""
+    grid = sim.grid
+    dt = sim.dt
+    n_steps = sim.n_steps
+    mon_dtype = sim.dtypes["monitors"]
+    result = Result(times=np.arange(n_steps) * dt)
+
+    rec: Dict[str, dict] = {}
+    for m in sim.monitors:
+        if isinstance(m, FieldMonitor):
+            if m.times is not None:
+                steps = {int(round(tm / dt)) for tm in m.times}
+            else:
+                steps = set(range(0, n_steps, m.interval))
+            steps = {s for s in steps if 0 <= s < n_steps}
+            zplane = (None if m.plane_z is None else
+                      int(np.argmin(np.abs(np.asarray(grid.coords[2]) - m.plane_z))))
+            rec[m.name] = dict(mon=m, steps=steps, n=len(steps), rows={c: None for c in m.components},
+                               times=[], box=_snap_box(grid.shape, m.downsample, zplane=zplane))
+        elif isinstance(m, DFTMonitor):
+            result.dft_freqs[m.name] = np.asarray(m.freqs, dtype=np.float64)
+
+    def record(r, step):
+        row = len(r["times"])
+        start, count, stride = r["box"]
+        full = count == list(grid.shape) and stride == [1, 1, 1]
+        for c in r["mon"].components:
+            snap = engine.get(c) if full else engine.read_box(c, start, count, stride)
+            if r["rows"][c] is None:
+                r["rows"][c] = np.empty((r["n"],) + snap.shape, dtype=mon_dtype)
+            r["rows"][c][row] = snap
+        r["times"].append(step * dt)
+
+    stops = sorted(set().union(*(r["steps"] for r in rec.values()))) if rec else []
+    tick = max(n_steps // 100, 1)
+    pos = 0
+
+    def advance_to(end):
+        nonlocal pos
+        while pos < end:
+            n = min(end - pos, tick if sim.progress_callback else end - pos)
+            engine.advance(pos, n)
+            pos += n
+            if sim.progress_callback is not None:
+                sim.progress_callback(min(pos, n_steps), n_steps)
+
+    for s in stops:
+        advance_to(s + 1)
+        for r in rec.values():
+            if s in r["steps"]:
+                record(r, s)
+    advance_to(n_steps)
+    if sim.progress_callback is not None:
+        sim.progress_callback(n_steps, n_steps)
+
+    for m in sim.monitors:
+        if isinstance(m, FieldMonitor):
+            r = rec[m.name]
+            if r["times"]:
+                result.fields[m.name] = dict(r["rows"])
+                result.monitor_times[m.name] = np.array(r["times"])
+            else:
+                result.fields[m.name] = {c: np.zeros((0,) + grid.shape) for c in m.components}
+                result.monitor_times[m.name] = np.array([])
+        elif isinstance(m, DFTMonitor):
+            result.dft[m.name] = engine.dft(m.name)
+        elif isinstance(m, FluxMonitor):
+            pass
+    result.flux.update(engine.flux())
+    return result
+
+
+def run_hip(sim) -> Result:
+    """Run ``sim`` on the gfx950 stepper and return a ``Result``."""
+    npdt = validate(sim)
+    engine = HipEngine(sim, npdt, exact_mode(npdt))
+    if sim.verbose:
+        name, free_mb, total_mb = device_info()
+        print(f"[photonfdtd_amd] HIP backend, device: {name} ({free_mb}/{total_mb} MB free)",
+              file=sys.stderr, flush=True)
+    try:
+        return run_loop(sim, engine)
+    finally:
+        engine.close()

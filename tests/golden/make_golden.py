@@ -1,0 +1,63 @@
+"""Write the golden fixtures from the reference's own NumPy float64 path.
+
+Run in a container that holds the reference checkout (read-only, never
+copied): ``python tests/golden/make_golden.py [/path/to/reference]``.
+For every scene in ``scenes.py`` this imports ``photonfdtd`` from the
+reference's ``src/`` (no bytecode written), runs it with ``backend="numpy"``
+and stores inputs-derived tables and outputs as ``<scene>.npz`` (float64) and
+``<scene>_f32.npz`` (float32 scenes). Only data is written: monitor arrays,
+DFT accumulators, flux totals, dt / n_steps, CPML tables, the stamped
+permittivity and the expanded source list.
+"""
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+from scenes import F32_SCENES, SCENES  # noqa: E402
+
+
+def dump(sim, res):
+    out = {"dt": np.float64(sim.dt), "n_steps": np.int64(sim.n_steps),
+           "eps_r": np.asarray(sim.eps_r), "times_len": np.int64(res.times.size)}
+    for a in range(3):
+        out[f"cpml_b_e{a}"] = np.asarray(sim._b_e[a])
+        out[f"cpml_c_e{a}"] = np.asarray(sim._c_e[a])
+        out[f"cpml_b_h{a}"] = np.asarray(sim._b_h[a])
+        out[f"cpml_c_h{a}"] = np.asarray(sim._c_h[a])
+    out["src_pos"] = np.array([list(s.position) + [0.0] * (3 - len(s.position))
+                               for s in sim.sources], dtype=np.float64).reshape(-1, 3)
+    out["src_amp"] = np.array([s.amplitude for s in sim.sources], dtype=np.float64)
+    out["src_comp"] = np.array([s.component for s in sim.sources])
+    for name, comps in res.fields.items():
+        out[f"mt/{name}"] = np.asarray(res.monitor_times[name])
+        for c, v in comps.items():
+            out[f"field/{name}/{c}"] = np.asarray(v)
+    for name, comps in res.dft.items():
+        out[f"dftf/{name}"] = np.asarray(res.dft_freqs[name])
+        for c, v in comps.items():
+            out[f"dft/{name}/{c}"] = np.asarray(v)
+    for name, v in res.flux.items():
+        out[f"flux/{name}"] = np.float64(v)
+    return out
+
+
+def main(ref_root):
+    sys.dont_write_bytecode = True
+    sys.path.insert(0, str(Path(ref_root) / "src"))
+    import photonfdtd as ref
+    for key, builder in SCENES.items():
+        precs = ["float64"] + (["float32"] if key in F32_SCENES else [])
+        for prec in precs:
+            sim = builder(ref, "numpy", precision=prec)
+            res = sim.run()
+            name = key if prec == "float64" else key + "_f32"
+            np.savez_compressed(HERE / f"{name}.npz", **dump(sim, res))
+            print("wrote", name, sim.grid.shape, sim.n_steps, flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.environ.get("PHOTONFDTD_REF", "/root/reference"))
