@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -80,6 +81,64 @@ class DeviceArena {
  private:
   std::vector<void*> ptrs_;
 };
+
+// True when the nodes with map >= 0 are exactly [0, lo) U [hi, n) with the
+// compact index running in node order; stores (lo, hi).
+bool two_slabs(const std::vector<int>& map, int out[2]) {
+  const int n = (int)map.size();
+  int lo = 0;
+  while (lo < n && map[lo] >= 0) ++lo;
+  int hi = lo;
+  while (hi < n && map[hi] < 0) ++hi;
+  for (int i = hi; i < n; ++i)
+    if (map[i] < 0) return false;
+  for (int i = 0; i < n; ++i) {
+    const bool in = i < lo || i >= hi;
+    if (in != (map[i] >= 0)) return false;
+    if (in && map[i] != (i < lo ? i : lo + (i - hi))) return false;
+  }
+  out[0] = lo;
+  out[1] = hi;
+  return true;
+}
+
+// Split [0, n) into segments of at most `len` nodes that never straddle the
+// bulk range [b0, b1): (start, count, in_bulk).
+struct Seg {
+  int start, count;
+  bool bulk;
+};
+std::vector<Seg> split_axis(int n, int b0, int b1, int len, bool widen) {
+  std::vector<Seg> out;
+  auto emit = [&](int lo, int hi, bool bulk) {
+    for (int s = lo; s < hi; s += len) out.push_back({s, std::min(len, hi - s), bulk});
+  };
+  b0 = std::max(0, std::min(b0, n));
+  b1 = std::max(b0, std::min(b1, n));
+  if (widen) {
+    // edge segments rounded up to whole tiles, so CPML tiles use every lane/row
+    const int lo = b0 > 0 ? std::min(n, (b0 + len - 1) / len * len) : 0;
+    const int hi = b1 < n ? std::max(lo, n - (n - b1 + len - 1) / len * len) : n;
+    b0 = lo;
+    b1 = std::max(lo, hi);
+  }
+  emit(0, b0, false);
+  emit(b0, b1, true);
+  emit(b1, n, false);
+  return out;
+}
+
+// Order tiles so that the blocks the hardware sends to one XCD (block ids
+// congruent mod 8) are spatially contiguous; pad with empty tiles.
+std::vector<pfhip::MarchTile> xcd_order(const std::vector<pfhip::MarchTile>& t) {
+  const size_t g = (t.size() + 7) / 8;
+  std::vector<pfhip::MarchTile> out(8 * g, pfhip::MarchTile{0, 0, 0, 0, 0, 0, 0, 0});
+  for (size_t idx = 0; idx < t.size(); ++idx) {
+    const size_t xcd = idx / g, q = idx % g;
+    out[8 * q + xcd] = t[idx];
+  }
+  return out;
+}
 
 using pfhip::BoxDev;
 using pfhip::SourceGroupsDev;
@@ -189,8 +248,25 @@ class Stepper final : public pf_handle {
   hipStream_t stream_ = nullptr;
   bool exact_;
   int64_t n_steps_;
-  StepArgs<real> args_;
-  real** fields_dev_ = nullptr;  // device copy of the six field pointers
+  StepArgs<real> args_;         // generic in-place kernels (level 0)
+  real* f_[2][6] = {};           // field arrays per time level (level 1 only when fused)
+  real** fields_dev_[2] = {};    // device copies of f_[level]
+  int cur_ = 0;                  // level holding the current fields
+  bool fused_ = false;           // fused ping-pong step in use
+  pfhip::FusedArgs<real> fa_[2]; // fused-step arguments reading level p
+  bool slab_ok_ = true;          // CPML nodes form [0, lo) U [hi, n) on every axis
+  pfhip::MarchArgs<real> ma_[2]; // march-kernel arguments reading level p
+  real* lvl_[2] = {nullptr, nullptr};   // six components per time level
+  real* psi_side_[2] = {nullptr, nullptr};  // E-side psi, H-side psi (level 0)
+  real* psi_h_lvl1_ = nullptr;           // H-side psi, level 1
+  size_t psi_len_h_ = 0;
+  real* coef_ = nullptr;
+  int* tiles_[2] = {nullptr, nullptr};   // march tiles: [0] psi-free, [1] with CPML
+  int ntiles_[2] = {0, 0};
+  int march_nw_ = pfhip::MARCH_NW;
+  bool split_ = false;           // separate psi-free launch for tiles off the CPML slabs
+  void build_march_tiles(int xchunk);
+  int slab_e_[3][2], slab_h_[3][2];
   SourceGroupsDev src_h_, src_e_;
   std::vector<Ade> ade_;
   std::vector<Dft> dft_;
@@ -244,11 +320,16 @@ Stepper<real>::Stepper(const pf_desc& d) {
     args_.inv_d[a] = (real)(1.0 / d.cell[a]);
   }
   args_.ch = (real)d.ch;
-  for (int c = 0; c < 6; ++c) args_.f[c] = arena_.zeros<real>(ncell, stream_);
+  // one buffer per time level: the six components back to back
+  lvl_[0] = arena_.zeros<real>(6 * (size_t)ncell, stream_);
+  for (int c = 0; c < 6; ++c) args_.f[c] = lvl_[0] + (size_t)c * ncell;
   args_.ce = arena_.upload(static_cast<const real*>(d.ce), ncell, stream_);
 
-  // CPML tables: coefficients rounded to the field dtype, compact maps.
+  // CPML tables: coefficients rounded to the field dtype, in one table laid
+  // out [axis][b_e, c_e, b_h, c_h][node]; compact maps per axis.
   const int n_axis[3] = {nx, ny, nz};
+  coef_ = static_cast<real*>(arena_.alloc(4 * (size_t)(nx + ny + nz) * sizeof(real)));
+  size_t coef_off = 0;
   for (int a = 0; a < 3; ++a) {
     const int n = n_axis[a];
     std::vector<real> be(n), ce(n), bh(n), ch(n);
@@ -262,24 +343,44 @@ Stepper<real>::Stepper(const pf_desc& d) {
       if (d.c_e[a][i] != 0.0) me[i] = pe++;
       if (d.c_h[a][i] != 0.0) mh[i] = ph++;
     }
-    args_.b_e[a] = arena_.upload(be.data(), n, stream_);
-    args_.c_e[a] = arena_.upload(ce.data(), n, stream_);
-    args_.b_h[a] = arena_.upload(bh.data(), n, stream_);
-    args_.c_h[a] = arena_.upload(ch.data(), n, stream_);
+    real* tbl = coef_ + coef_off;
+    HIP_CHECK(hipMemcpy(tbl, be.data(), n * sizeof(real), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(tbl + n, ce.data(), n * sizeof(real), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(tbl + 2 * n, bh.data(), n * sizeof(real), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(tbl + 3 * n, ch.data(), n * sizeof(real), hipMemcpyHostToDevice));
+    args_.b_e[a] = tbl;
+    args_.c_e[a] = tbl + n;
+    args_.b_h[a] = tbl + 2 * n;
+    args_.c_h[a] = tbl + 3 * n;
+    coef_off += 4 * (size_t)n;
     args_.map_e[a] = arena_.upload(me.data(), n, stream_);
     args_.map_h[a] = arena_.upload(mh.data(), n, stream_);
     args_.pe[a] = pe;
     args_.ph[a] = ph;
+    slab_ok_ = slab_ok_ && two_slabs(me, slab_e_[a]) && two_slabs(mh, slab_h_[a]);
   }
-  // psi slot -> derivative axis
+  // psi: one buffer per side, its six slots back to back in slot order
   static const int slot_axis[12] = {1, 2, 2, 0, 0, 1, 1, 2, 2, 0, 0, 1};
+  size_t psi_len[2] = {0, 0};
   for (int s = 0; s < pfhip::PSI_COUNT; ++s) {
     const int ax = slot_axis[s];
     const int pc = s < 6 ? args_.pe[ax] : args_.ph[ax];
-    const int64_t n = ncell / n_axis[ax] * pc;
-    args_.psi[s] = arena_.zeros<real>(n, stream_);
+    psi_len[s / 6] += (size_t)(ncell / n_axis[ax]) * pc;
   }
-  fields_dev_ = arena_.upload(args_.f, 6, stream_);
+  psi_side_[0] = arena_.zeros<real>(psi_len[0], stream_);
+  psi_side_[1] = arena_.zeros<real>(psi_len[1], stream_);
+  {
+    size_t off[2] = {0, 0};
+    for (int s = 0; s < pfhip::PSI_COUNT; ++s) {
+      const int ax = slot_axis[s];
+      const int pc = s < 6 ? args_.pe[ax] : args_.ph[ax];
+      args_.psi[s] = psi_side_[s / 6] ? psi_side_[s / 6] + off[s / 6] : nullptr;
+      off[s / 6] += (size_t)(ncell / n_axis[ax]) * pc;
+    }
+  }
+  for (int c = 0; c < 6; ++c) f_[0][c] = args_.f[c];
+  psi_len_h_ = psi_len[1];
+  fields_dev_[0] = arena_.upload(f_[0], 6, stream_);
   const int32_t comp_index[6] = {0, 1, 2, 3, 4, 5};
   comp_index_dev_ = arena_.upload(comp_index, 6, stream_);
 
@@ -387,7 +488,126 @@ Stepper<real>::Stepper(const pf_desc& d) {
     flux_.push_back(st);
   }
 
+  // Fused ping-pong step: 3-D grids without H-side sources (an H source
+  // must land between the H and E halves of the step).
+  const char* env_fused = std::getenv("PHOTONFDTD_HIP_FUSED");
+  const bool want_fused = !(env_fused && env_fused[0] == '0');
+  if (want_fused && pfhip::fused_supported<real>(args_) && src_h_.n_groups == 0) {
+    fused_ = true;
+    lvl_[1] = arena_.zeros<real>(6 * (size_t)ncell, stream_);
+    for (int c = 0; c < 6; ++c) f_[1][c] = lvl_[1] + (size_t)c * ncell;
+    fields_dev_[1] = arena_.upload(f_[1], 6, stream_);
+    psi_h_lvl1_ = arena_.zeros<real>(psi_len_h_, stream_);
+    real* psi_h1[6];
+    for (int s = 0; s < 6; ++s)
+      psi_h1[s] = psi_h_lvl1_ ? psi_h_lvl1_ + (args_.psi[6 + s] - args_.psi[6]) : nullptr;
+    const char* env_xc = std::getenv("PHOTONFDTD_HIP_XCHUNK");
+    const int xchunk = env_xc ? std::max(1, std::atoi(env_xc)) : 128;
+    const char* env_var = std::getenv("PHOTONFDTD_HIP_VARIANT");
+    int variant = env_var ? std::atoi(env_var) : 2;
+    if (variant == 2 && !slab_ok_) variant = 1;
+    for (int p = 0; p < 2; ++p) {
+      pfhip::FusedArgs<real>& fa = fa_[p];
+      std::memset(&fa, 0, sizeof(fa));
+      for (int c = 0; c < 3; ++c) {
+        fa.Ein[c] = f_[p][c];
+        fa.Eout[c] = f_[p ^ 1][c];
+        fa.Hin[c] = f_[p][3 + c];
+        fa.Hout[c] = f_[p ^ 1][3 + c];
+      }
+      fa.ce = args_.ce;
+      for (int s = 0; s < 6; ++s) {
+        fa.psiE[s] = args_.psi[s];
+        real* lvl0 = args_.psi[6 + s];
+        fa.psiHin[s] = p == 0 ? lvl0 : psi_h1[s];
+        fa.psiHout[s] = p == 0 ? psi_h1[s] : lvl0;
+      }
+      for (int a = 0; a < 3; ++a) {
+        fa.b_e[a] = args_.b_e[a];
+        fa.c_e[a] = args_.c_e[a];
+        fa.b_h[a] = args_.b_h[a];
+        fa.c_h[a] = args_.c_h[a];
+        fa.map_e[a] = args_.map_e[a];
+        fa.map_h[a] = args_.map_h[a];
+        fa.pe[a] = args_.pe[a];
+        fa.ph[a] = args_.ph[a];
+        fa.d[a] = args_.d[a];
+        fa.inv_d[a] = args_.inv_d[a];
+      }
+      fa.nx = nx;
+      fa.ny = ny;
+      fa.nz = nz;
+      fa.nyz = args_.nyz;
+      fa.ch = args_.ch;
+      fa.xchunk = xchunk;
+      fa.variant = variant;
+      pfhip::MarchArgs<real>& ma = ma_[p];
+      std::memset(&ma, 0, sizeof(ma));
+      ma.Lin = lvl_[p];
+      ma.Lout = lvl_[p ^ 1];
+      ma.ce = args_.ce;
+      ma.psiE = psi_side_[0];
+      ma.psiHin = p == 0 ? psi_side_[1] : psi_h_lvl1_;
+      ma.psiHout = p == 0 ? psi_h_lvl1_ : psi_side_[1];
+      ma.coef = coef_;
+      ma.nx = nx;
+      ma.ny = ny;
+      ma.nz = nz;
+      for (int a = 0; a < 3; ++a) {
+        ma.pe[a] = args_.pe[a];
+        ma.ph[a] = args_.ph[a];
+        ma.d[a] = args_.d[a];
+        ma.inv_d[a] = args_.inv_d[a];
+        for (int q = 0; q < 2; ++q) {
+          ma.slab_e[a][q] = slab_e_[a][q];
+          ma.slab_h[a][q] = slab_h_[a][q];
+        }
+      }
+      ma.ch = args_.ch;
+      ma.xchunk = std::min(xchunk, pfhip::MARCH_MAX_XCHUNK);
+      for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 2; ++q) {
+          fa.slab_e[a][q] = slab_e_[a][q];
+          fa.slab_h[a][q] = slab_h_[a][q];
+        }
+    }
+  }
+
+  const char* env_nw = std::getenv("PHOTONFDTD_HIP_NW");
+  if (env_nw && std::atoi(env_nw) == 16) march_nw_ = 16;
+  const char* env_split = std::getenv("PHOTONFDTD_HIP_SPLIT");
+  split_ = env_split && env_split[0] == '1';
+  if (fused_ && fa_[0].variant == 2) build_march_tiles(std::min(fa_[0].xchunk, pfhip::MARCH_MAX_XCHUNK));
   HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// Tiles of the march kernel: tiles whose every node (including the H apron
+// row/lane and the plane before the chunk) lies outside the CPML slabs go to
+// the psi-free kernel, the rest to the full kernel.
+template <typename real>
+void Stepper<real>::build_march_tiles(int xchunk) {
+  const int n[3] = {args_.nx, args_.ny, args_.nz};
+  const int len[3] = {xchunk, march_nw_ - 2, 62};
+  std::vector<Seg> seg[3];
+  for (int a = 0; a < 3; ++a) {
+    const int b0 = std::max(slab_e_[a][0], slab_h_[a][0] + 1);
+    const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]);
+    seg[a] = split_ ? split_axis(n[a], b0, b1, len[a], a > 0) : split_axis(n[a], 0, 0, len[a], false);
+  }
+  std::vector<pfhip::MarchTile> bulk, pml;
+  for (const Seg& sx : seg[0])
+    for (const Seg& sy : seg[1])
+      for (const Seg& sz : seg[2]) {
+        pfhip::MarchTile t{sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0};
+        (split_ && sx.bulk && sy.bulk && sz.bulk ? bulk : pml).push_back(t);
+      }
+  const std::vector<pfhip::MarchTile>* lists[2] = {&bulk, &pml};
+  for (int q = 0; q < 2; ++q) {
+    std::vector<pfhip::MarchTile> ordered = xcd_order(*lists[q]);
+    ntiles_[q] = (int)ordered.size();
+    tiles_[q] = ordered.empty() ? nullptr
+                                : reinterpret_cast<int*>(arena_.upload(ordered.data(), ordered.size(), stream_));
+  }
 }
 
 template <typename real>
@@ -448,21 +668,48 @@ void Stepper<real>::timing_read(double* ms, int64_t* launches) {
 
 template <typename real>
 void Stepper<real>::step_once(int64_t step) {
-  timed(PF_K_YEE_H, [&] { pfhip::launch_generic_h<real>(args_, exact_, stream_); });
-  timed(PF_K_AUX, [&] {
-    pfhip::launch_inject<real>(fields_dev_, src_h_, n_steps_, step, stream_);
-    for (Ade& a : ade_) pfhip::launch_ade_capture<real>(fields_dev_, a.cells, a.n, a.En, stream_);
-  });
-  timed(PF_K_YEE_E, [&] { pfhip::launch_generic_e<real>(args_, exact_, stream_); });
-  timed(PF_K_AUX, [&] {
-    for (Ade& a : ade_)
-      pfhip::launch_ade_apply<real>(fields_dev_, a.cells, a.n, a.En, a.Pcur, a.Pprev, a.abc,
-                                    a.n_poles, a.inv_eps, stream_);
-    pfhip::launch_inject<real>(fields_dev_, src_e_, n_steps_, step, stream_);
-    for (Flux& f : flux_)
-      pfhip::launch_flux<real>(args_.f, f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA, f.dt,
-                               f.partial, f.n_part, f.total, stream_);
-  });
+  if (fused_) {
+    const int in = cur_, out = cur_ ^ 1;
+    timed(PF_K_YEE_FUSED, [&] {
+      if (fa_[in].variant == 2) {
+        for (int q = 0; q < 2; ++q) {
+          pfhip::MarchArgs<real> ma = ma_[in];
+          ma.tiles = tiles_[q];
+          ma.ntiles = ntiles_[q];
+          pfhip::launch_march<real>(ma, exact_, q == 1, march_nw_, stream_);
+        }
+      } else {
+        pfhip::launch_fused<real>(fa_[in], exact_, stream_);
+      }
+    });
+    timed(PF_K_AUX, [&] {
+      for (Ade& a : ade_)
+        pfhip::launch_ade_pp<real>(f_[in], f_[out], a.cells, a.n, a.Pcur, a.Pprev, a.abc,
+                                   a.n_poles, a.inv_eps, stream_);
+      cur_ = out;
+      pfhip::launch_inject<real>(fields_dev_[cur_], src_e_, n_steps_, step, stream_);
+      for (Flux& f : flux_)
+        pfhip::launch_flux<real>(f_[cur_], f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
+                                 f.dt, f.partial, f.n_part, f.total, stream_);
+    });
+  } else {
+    timed(PF_K_YEE_H, [&] { pfhip::launch_generic_h<real>(args_, exact_, stream_); });
+    timed(PF_K_AUX, [&] {
+      pfhip::launch_inject<real>(fields_dev_[0], src_h_, n_steps_, step, stream_);
+      for (Ade& a : ade_)
+        pfhip::launch_ade_capture<real>(fields_dev_[0], a.cells, a.n, a.En, stream_);
+    });
+    timed(PF_K_YEE_E, [&] { pfhip::launch_generic_e<real>(args_, exact_, stream_); });
+    timed(PF_K_AUX, [&] {
+      for (Ade& a : ade_)
+        pfhip::launch_ade_apply<real>(fields_dev_[0], a.cells, a.n, a.En, a.Pcur, a.Pprev, a.abc,
+                                      a.n_poles, a.inv_eps, stream_);
+      pfhip::launch_inject<real>(fields_dev_[0], src_e_, n_steps_, step, stream_);
+      for (Flux& f : flux_)
+        pfhip::launch_flux<real>(args_.f, f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
+                                 f.dt, f.partial, f.n_part, f.total, stream_);
+    });
+  }
   timed(PF_K_AUX, [&] { capture_dft(step); });
 }
 
@@ -474,7 +721,7 @@ void Stepper<real>::capture_dft(int64_t step) {
     if (sample >= m.n_samples) throw ArgError{"DFT sample beyond the phase table"};
     if (m.count == 0) m.sample0 = sample;
     else if (sample != m.sample0 + m.count) throw ArgError{"DFT samples must be consecutive"};
-    pfhip::launch_gather_box<real>(fields_dev_, m.comps_dev, m.n_comp, m.box, args_.nyz, args_.nz,
+    pfhip::launch_gather_box<real>(fields_dev_[cur_], m.comps_dev, m.n_comp, m.box, args_.nyz, args_.nz,
                                    m.ring + (int64_t)m.count * m.n_comp * m.nbox, stream_);
     if (++m.count == m.K) fold(m);
   }
@@ -519,7 +766,7 @@ void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
   const BoxDev b = check_box(box, shape);
   const int64_t nbox = b.count[0] * b.count[1] * b.count[2];
   real* out = staging(nbox);
-  pfhip::launch_gather_box<real>(fields_dev_, comp_index_dev_ + comp, 1, b, args_.nyz, args_.nz, out, stream_);
+  pfhip::launch_gather_box<real>(fields_dev_[cur_], comp_index_dev_ + comp, 1, b, args_.nyz, args_.nz, out, stream_);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(dst, out, nbox * sizeof(real), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -530,7 +777,7 @@ void Stepper<real>::write_field(int comp, const void* src) {
   if (comp < 0 || comp > 5) throw ArgError{"component out of range"};
   if (!src) throw ArgError{"null source"};
   const int64_t ncell = shape[0] * shape[1] * shape[2];
-  HIP_CHECK(hipMemcpyAsync(args_.f[comp], src, ncell * sizeof(real), hipMemcpyHostToDevice, stream_));
+  HIP_CHECK(hipMemcpyAsync(f_[cur_][comp], src, ncell * sizeof(real), hipMemcpyHostToDevice, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
 }
 

@@ -33,6 +33,62 @@ struct StepArgs {
   real ch;
 };
 
+// Kernel arguments of the fused ping-pong step (3-D grids).
+template <typename real>
+struct FusedArgs {
+  const real* Ein[3];
+  real* Eout[3];
+  const real* Hin[3];
+  real* Hout[3];
+  const real* ce;
+  real* psiE[6];          // E-side psi, updated in place
+  const real* psiHin[6];  // H-side psi, ping-ponged
+  real* psiHout[6];
+  const real* b_e[3];
+  const real* c_e[3];
+  const real* b_h[3];
+  const real* c_h[3];
+  const int* map_e[3];
+  const int* map_h[3];
+  int pe[3], ph[3];
+  int nx, ny, nz;
+  int64_t nyz;
+  real d[3];
+  real inv_d[3];
+  real ch;
+  int xchunk;             // x planes per workgroup
+  int variant;            // 0: register march, 1: LDS-staged march, 2: march2
+  int slab_e[3][2];       // per axis: E-side CPML nodes are n < lo or n >= hi
+  int slab_h[3][2];
+  int tiles_z, tiles_y, chunks_x, grid_blocks;   // filled by the launcher
+};
+
+// Arguments of the march kernel: a few base pointers, everything else derived.
+constexpr int MARCH_MAX_XCHUNK = 128;
+template <typename real>
+struct MarchArgs {
+  const real* Lin;        // level read: Ex Ey Ez Hx Hy Hz, each nx*ny*nz
+  real* Lout;             // level written
+  const real* ce;
+  real* psiE;             // E-side psi slots ExY ExZ EyZ EyX EzX EzY, back to back
+  const real* psiHin;     // H-side psi slots HxY HxZ HyZ HyX HzX HzY (read)
+  real* psiHout;          // H-side psi (written)
+  const real* coef;       // [axis][b_e, c_e, b_h, c_h][node]
+  int nx, ny, nz;
+  int pe[3], ph[3];
+  int slab_e[3][2], slab_h[3][2];
+  real d[3], inv_d[3], ch;
+  int xchunk;
+  const int* tiles;       // [ntiles][8]: i0, i1, j0, jn, k0, kn, -, - (host-built)
+  int ntiles;
+};
+
+// Tile of the march kernel: planes [i0, i1), owned rows j0..j0+jn-1 and owned
+// z nodes k0..k0+kn-1 (jn <= NW-2, kn <= 62). i1 <= i0 marks padding.
+struct MarchTile {
+  int i0, i1, j0, jn, k0, kn, pad0, pad1;
+};
+
 struct BoxDev {
   int64_t start[3];
   int64_t count[3];
@@ -71,5 +127,14 @@ void launch_dft_fold(const real* ring, int64_t nbox, int n_comp, const int32_t* 
 template <typename real>
 void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int ny, int nz,
                  double dA, double dt, double* partial, int n_part, double* total, hipStream_t s);
+
+constexpr int MARCH_NW = 10;   // wavefronts per march workgroup (rows j0-1 .. j0+NW-2)
+template <typename real>
+void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, hipStream_t s);
+template <typename real> bool fused_supported(const StepArgs<real>& a);
+template <typename real> void launch_fused(FusedArgs<real>& a, bool exact, hipStream_t s);
+template <typename real>
+void launch_ade_pp(real* const* Ein, real* const* Eout, const int64_t* cells, int64_t n, real* Pcur,
+                   real* Pprev, const real* abc, int n_poles, real inv_eps, hipStream_t s);
 
 }  // namespace pfhip
