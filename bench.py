@@ -10,7 +10,7 @@ A step is one full Yee time step of the whole grid (H and E updates, CPML,
 ADE poles, sources and DFT accumulation). ``value`` is Gcell-updates/s over
 the timed steps, fields resident in HBM. Printed as one JSON line, together
 with the roofline of the stencil kernels (HIP events on the stepper's stream),
-the CPU baseline (the NumPy float64 oracle on a bounded 96^3 sample) and the
+the CPU baseline (the NumPy float64 oracle on a bounded 48^3 x 600-step sample) and the
 float64/float32 error of the GPU stepper against that oracle run.
 """
 from __future__ import annotations
@@ -72,9 +72,11 @@ def _rel(a, b):
                  max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))
 
 
-def parity_and_cpu(n=80, steps=100):
+def parity_and_cpu(n=48, steps=600):
     """Oracle run on an n^3 sample (timed: the CPU baseline) and the GPU
-    float64 / float32 errors against it (DFT planes and final fields)."""
+    float64 / float32 errors against it (DFT planes and final fields). The
+    reference operation order in float32 (the oracle run at float32) is
+    reported beside the GPU float32 error as the intrinsic float32 level."""
     import photonfdtd_amd as pf
     from photonfdtd_amd.hip_backend import HipEngine
     from oracle.fdtd_numpy import run_oracle
@@ -99,10 +101,101 @@ def parity_and_cpu(n=80, steps=100):
             worst = max(worst, _rel(eng.get(c), state[c]))
         eng.close()
         errs[prec] = worst
+    ref32, state32 = run_oracle(dispersive_slab(pf, n, steps, "float32"), return_state=True)
+    intrinsic = 0.0
+    for nm in ("in", "out"):
+        for c, v in ref32["dft"][nm].items():
+            intrinsic = max(intrinsic, _rel(v, ref["dft"][nm][c]))
+    for c in state:
+        intrinsic = max(intrinsic, _rel(state32[c], state[c]))
+    errs["ref_float32"] = intrinsic
     return dict(cpu_seconds=cpu_s, cpu_value=cells * steps / cpu_s / 1e9,
                 sample=f"NumPy float64 oracle, {n}^3 cells x {steps} steps of the same scene "
                        f"({cells * steps:.3g} cell-updates), 1 thread",
-                fp64_rel_l2=errs["float64"], fp32_rel_l2=errs["float32"])
+                fp64_rel_l2=errs["float64"], fp32_rel_l2=errs["float32"],
+                ref_fp32_rel_l2=errs["ref_float32"])
+
+
+PROFILES = ROOT / "profiles"
+
+
+def measured_traffic(n, precision):
+    """HBM bytes per launch of the step kernel from the committed counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    the counter reads for the full-field gather copy), or None."""
+    f = PROFILES / f"dispersive_slab_{n}_{precision}_traffic.json"
+    if not f.exists():
+        return None
+    return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+
+
+def summarize(prof_dir, kind, out=None):
+    """Summarize rocprofv3 CSV output (kernel stats or one PMC counter)."""
+    import csv
+    import glob
+    prof_dir = Path(prof_dir)
+    lines = []
+    if kind == "stats":
+        path = sorted(glob.glob(str(prof_dir / "**" / "*kernel_stats.csv"), recursive=True))[0]
+        rows = list(csv.DictReader(open(path)))
+        lines.append("kernel | calls | total_ms | avg_us | min_us | max_us")
+        for r in rows:
+            lines.append(f"{r['Name'][:90]} | {r['Calls']} | {float(r['TotalDurationNs'])/1e6:.3f} | "
+                         f"{float(r['AverageNs'])/1e3:.2f} | {float(r['MinNs'])/1e3:.2f} | "
+                         f"{float(r['MaxNs'])/1e3:.2f}")
+    else:
+        path = sorted(glob.glob(str(prof_dir / "**" / "*counter_collection.csv"), recursive=True))[0]
+        agg = {}
+        for r in csv.DictReader(open(path)):
+            key = (r["Kernel_Name"], r["Counter_Name"])
+            agg.setdefault(key, []).append(float(r["Counter_Value"]))
+        lines.append("kernel | counter | dispatches | mean value (KiB per dispatch)")
+        for (k, c), v in sorted(agg.items()):
+            lines.append(f"{k[:90]} | {c} | {len(v)} | {sum(v) / len(v):.1f}")
+    text = "\n".join(lines) + "\n"
+    if out:
+        Path(out).write_text(text)
+    print(text)
+    return text
+
+
+def traffic_from_counters(fetch_dir, write_dir, n=512, precision="float32"):
+    """HBM bytes per launch of the step kernel from one FETCH_SIZE and one
+    WRITE_SIZE rocprofv3 run of this benchmark. Each counter is scaled by its
+    reading for the full-field gather (a copy of n^3 values each way) of the
+    same run. Writes profiles/dispersive_slab_<n>_<precision>_traffic.json."""
+    import csv
+    import glob
+
+    def dispatches(d):
+        path = sorted(glob.glob(str(Path(d) / "**" / "*counter_collection.csv"), recursive=True))[0]
+        return [(r["Kernel_Name"], float(r["Counter_Value"]), int(r["Grid_Size"]))
+                for r in csv.DictReader(open(path))]
+    cells = n ** 3
+    item = 4 if precision == "float32" else 8
+    true_kib = cells * item / 1024
+    fe, wr = dispatches(fetch_dir), dispatches(write_dir)
+    copy_f = [v for k, v, g in fe if "k_gather_box" in k and g == cells][0]
+    copy_w = [v for k, v, g in wr if "k_gather_box" in k and g == cells][0]
+    step_f = [v for k, v, g in fe if "k_yee_march" in k]
+    step_w = [v for k, v, g in wr if "k_yee_march" in k]
+    fetch = sum(step_f) / len(step_f) * (true_kib / copy_f) * 1024
+    write = sum(step_w) / len(step_w) * (true_kib / copy_w) * 1024
+    alg = cells * 13 * item
+    out = {"kernel": "k_yee_march", "workload": f"3-D dispersive slab {n}^3 {precision}",
+           "fetch_kib_raw_mean": sum(step_f) / len(step_f),
+           "write_kib_raw_mean": sum(step_w) / len(step_w),
+           "copy_calibration": {"kernel": f"k_gather_box full field ({cells} values each way)",
+                                "fetch_kib_raw": copy_f, "write_kib_raw": copy_w,
+                                "true_kib": true_kib, "fetch_factor": true_kib / copy_f,
+                                "write_factor": true_kib / copy_w},
+           "hbm_bytes_per_launch": round(fetch + write), "fetch_bytes": round(fetch),
+           "write_bytes": round(write), "algorithmic_bytes_per_launch": alg,
+           "traffic_over_algorithmic": (fetch + write) / alg}
+    PROFILES.mkdir(exist_ok=True)
+    (PROFILES / f"dispersive_slab_{n}_{precision}_traffic.json").write_text(
+        json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out, indent=1))
 
 
 def main():
@@ -113,7 +206,17 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--precision", default="float32", choices=("float32", "float64"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg")
+    ap.add_argument("--summarize", nargs=2, metavar=("DIR", "KIND"),
+                    help="print a summary of rocprofv3 CSV output (KIND: stats | pmc)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="derive HBM bytes per step launch from two rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.summarize:
+        summarize(*args.summarize)
+        return
+    if args.traffic:
+        traffic_from_counters(*args.traffic, n=args.n, precision=args.precision)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this stepper runs on one GPU; multi-GPU runs are not "
                                    "part of this build (see the sharding design notes)",
@@ -150,6 +253,9 @@ def main():
             "pf_timing_read")
     # sanity: the DFT accumulators must be finite
     finite = all(np.isfinite(v).all() for v in eng.dft("out").values())
+    # one full-field gather: a plain copy kernel of known size (n cells read,
+    # n cells written) that calibrates the FETCH_SIZE / WRITE_SIZE counters
+    field_max = float(np.abs(eng.get("Ey")).max())
     eng.close()
 
     cells = sim.grid.n_cells
@@ -183,13 +289,15 @@ def main():
                    "cells": cells, "exact_order": exact, "device": device_info()[0]},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.n, args.precision),
                      "bytes_per_cell_update": bytes_per_update,
                      "kernel_ms_per_step": round(step_ms, 4)},
         "kernel_ms": {name: {"total_ms": round(t, 3), "launch_groups": int(c)}
                       for name, (t, c) in k.items()},
         "hbm_fraction_of_value": round(value * 1e9 * bytes_per_update / 1e9 / HBM_PEAK_GBS, 4),
         "dft_finite": bool(finite),
+        "ey_max_final": field_max,
         "setup_s": round(setup_s, 1),
     }
     if check is not None:
@@ -197,6 +305,7 @@ def main():
                                "cores": 1, "kind": "port", "sample": check["sample"]}
         out["fp64_rel_l2_vs_oracle"] = check["fp64_rel_l2"]
         out["fp32_rel_l2_vs_oracle"] = check["fp32_rel_l2"]
+        out["reference_order_fp32_rel_l2_vs_oracle"] = check["ref_fp32_rel_l2"]
     print(json.dumps(out), flush=True)
 
 
