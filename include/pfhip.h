@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 1
+#define PFHIP_ABI_VERSION 2
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -109,7 +109,20 @@ typedef struct pf_desc {
   const pf_dft_desc* dft;
   int32_t n_flux;
   const pf_flux_desc* flux;
+  int32_t flags;              /* PF_FLAG_* */
 } pf_desc;
+
+/* pf_desc.flags: use the in-place two-pass kernels only (no fused ping-pong
+ * step); required for pf_checkpoint_enable / pf_adjoint. */
+#define PF_FLAG_GENERIC 1
+
+/* Cotangent of one DFT accumulator: g = dJ/d(Re F) + i dJ/d(Im F) for monitor
+ * `mon`, component slot `slot`, complex128 interleaved [n_freq][box cells]. */
+typedef struct pf_cotangent {
+  int32_t mon;
+  int32_t slot;
+  const double* g;
+} pf_cotangent;
 
 typedef struct pf_handle pf_handle;
 
@@ -143,6 +156,16 @@ int pf_write_field(pf_handle* h, int32_t comp, const void* src);
 int pf_dft_read(pf_handle* h, int32_t mon, int32_t slot, double* dst);
 /* Accumulated flux of every flux monitor, [n_flux] doubles. */
 int pf_flux_read(pf_handle* h, double* dst);
+/* Discrete adjoint. The reference differentiates its stepper through JAX
+ * (jaxbackend.py:666-722, value_and_grad_eps); these two calls are the native
+ * replacement. pf_checkpoint_enable(h, C) must precede the forward run from
+ * step 0: pf_advance then stores the full state every C steps on the device.
+ * After the forward run reaches the end of the waveform table, pf_adjoint
+ * sweeps backwards (recomputing each C-step segment from its checkpoint) with
+ * the DFT cotangents as time-reversed sources and writes dJ/dce for every
+ * node (float64, nx*ny*nz) to grad_ce. Dispersive media are rejected. */
+int pf_checkpoint_enable(pf_handle* h, int32_t interval);
+int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* grad_ce);
 /* Event timing (no reference counterpart; used by the benchmark): when on,
  * each kernel-class launch group of pf_advance is bracketed by HIP events on
  * the handle's stream. Enabling resets the totals. pf_timing_read returns,

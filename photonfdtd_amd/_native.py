@@ -11,7 +11,8 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
+PF_FLAG_GENERIC = 1
 
 PF_F32, PF_F64 = 0, 1
 KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux")
@@ -48,7 +49,12 @@ class PfDesc(C.Structure):
                 ("n_steps", _i64), ("n_src", _i32), ("src_comp", _P(_i32)), ("src_cell", _P(_i64)),
                 ("src_amp", _P(_f64)), ("src_wave", _P(_i32)), ("n_wave", _i32),
                 ("wave", _P(_f64)), ("n_ade", _i32), ("ade", _P(PfAdeDesc)), ("n_dft", _i32),
-                ("dft", _P(PfDftDesc)), ("n_flux", _i32), ("flux", _P(PfFluxDesc))]
+                ("dft", _P(PfDftDesc)), ("n_flux", _i32), ("flux", _P(PfFluxDesc)),
+                ("flags", _i32)]
+
+
+class PfCotangent(C.Structure):
+    _fields_ = [("mon", _i32), ("slot", _i32), ("g", _P(_f64))]
 
 
 # Exported symbols and their signatures; tests check every one is present.
@@ -61,6 +67,8 @@ SIGNATURES = {
     "pf_write_field": (_i32, [_vp, _i32, _vp]),
     "pf_dft_read": (_i32, [_vp, _i32, _i32, _P(_f64)]),
     "pf_flux_read": (_i32, [_vp, _P(_f64)]),
+    "pf_checkpoint_enable": (_i32, [_vp, _i32]),
+    "pf_adjoint": (_i32, [_vp, _i32, _P(PfCotangent), _P(_f64)]),
     "pf_timing_enable": (_i32, [_vp, _i32]),
     "pf_timing_read": (_i32, [_vp, _P(_f64), _P(_i64)]),
     "pf_device_info": (_i32, [C.c_char_p, _i32, _P(_i64), _P(_i64)]),

@@ -167,6 +167,8 @@ struct pf_handle {
   virtual void dft_read(int mon, int slot, double* dst) = 0;
   virtual void flux_read(double* dst) = 0;
   virtual void timing_enable(bool on) = 0;
+  virtual void checkpoint_enable(int interval) = 0;
+  virtual void adjoint(int n_cot, const pf_cotangent* cot, double* grad) = 0;
   virtual void timing_read(double* ms, int64_t* launches) = 0;
   int64_t shape[3];
 };
@@ -195,6 +197,8 @@ class Stepper final : public pf_handle {
   void dft_read(int mon, int slot, double* dst) override;
   void flux_read(double* dst) override;
   void timing_enable(bool on) override;
+  void checkpoint_enable(int interval) override;
+  void adjoint(int n_cot, const pf_cotangent* cot, double* grad) override;
   void timing_read(double* ms, int64_t* launches) override;
 
  private:
@@ -236,7 +240,9 @@ class Stepper final : public pf_handle {
     double* total;
   };
 
-  void step_once(int64_t step);
+  void step_once(int64_t step, bool monitors = true);
+  void save_state(real* dst);
+  void load_state(const real* src);
   void capture_dft(int64_t step);
   void fold(Dft& m);
   real* staging(size_t elems);
@@ -265,6 +271,11 @@ class Stepper final : public pf_handle {
   int ntiles_[2] = {0, 0};
   int march_nw_ = pfhip::MARCH_NW;
   bool split_ = false;           // separate psi-free launch for tiles off the CPML slabs
+  // checkpoints for the adjoint (generic layout: level 0 fields + both psi sides)
+  int ckpt_interval_ = 0;
+  std::vector<real*> ckpts_;
+  size_t psi_len_e_ = 0;
+  int64_t steps_done_ = 0;
   void build_march_tiles(int xchunk);
   int slab_e_[3][2], slab_h_[3][2];
   SourceGroupsDev src_h_, src_e_;
@@ -380,6 +391,7 @@ Stepper<real>::Stepper(const pf_desc& d) {
   }
   for (int c = 0; c < 6; ++c) f_[0][c] = args_.f[c];
   psi_len_h_ = psi_len[1];
+  psi_len_e_ = psi_len[0];
   fields_dev_[0] = arena_.upload(f_[0], 6, stream_);
   const int32_t comp_index[6] = {0, 1, 2, 3, 4, 5};
   comp_index_dev_ = arena_.upload(comp_index, 6, stream_);
@@ -491,7 +503,7 @@ Stepper<real>::Stepper(const pf_desc& d) {
   // Fused ping-pong step: 3-D grids without H-side sources (an H source
   // must land between the H and E halves of the step).
   const char* env_fused = std::getenv("PHOTONFDTD_HIP_FUSED");
-  const bool want_fused = !(env_fused && env_fused[0] == '0');
+  const bool want_fused = !(env_fused && env_fused[0] == '0') && !(d.flags & PF_FLAG_GENERIC);
   if (want_fused && pfhip::fused_supported<real>(args_) && src_h_.n_groups == 0) {
     fused_ = true;
     lvl_[1] = arena_.zeros<real>(6 * (size_t)ncell, stream_);
@@ -667,7 +679,7 @@ void Stepper<real>::timing_read(double* ms, int64_t* launches) {
 }
 
 template <typename real>
-void Stepper<real>::step_once(int64_t step) {
+void Stepper<real>::step_once(int64_t step, bool monitors) {
   if (fused_) {
     const int in = cur_, out = cur_ ^ 1;
     timed(PF_K_YEE_FUSED, [&] {
@@ -706,11 +718,143 @@ void Stepper<real>::step_once(int64_t step) {
                                       a.n_poles, a.inv_eps, stream_);
       pfhip::launch_inject<real>(fields_dev_[0], src_e_, n_steps_, step, stream_);
       for (Flux& f : flux_)
-        pfhip::launch_flux<real>(args_.f, f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
-                                 f.dt, f.partial, f.n_part, f.total, stream_);
+        if (monitors)
+          pfhip::launch_flux<real>(args_.f, f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
+                                   f.dt, f.partial, f.n_part, f.total, stream_);
     });
   }
-  timed(PF_K_AUX, [&] { capture_dft(step); });
+  if (monitors) timed(PF_K_AUX, [&] { capture_dft(step); });
+}
+
+template <typename real>
+void Stepper<real>::save_state(real* dst) {
+  const size_t nf = 6 * (size_t)(shape[0] * shape[1] * shape[2]);
+  HIP_CHECK(hipMemcpyAsync(dst, lvl_[0], nf * sizeof(real), hipMemcpyDeviceToDevice, stream_));
+  if (psi_len_e_)
+    HIP_CHECK(hipMemcpyAsync(dst + nf, psi_side_[0], psi_len_e_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+  if (psi_len_h_)
+    HIP_CHECK(hipMemcpyAsync(dst + nf + psi_len_e_, psi_side_[1], psi_len_h_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+}
+
+template <typename real>
+void Stepper<real>::load_state(const real* src) {
+  const size_t nf = 6 * (size_t)(shape[0] * shape[1] * shape[2]);
+  HIP_CHECK(hipMemcpyAsync(lvl_[0], src, nf * sizeof(real), hipMemcpyDeviceToDevice, stream_));
+  if (psi_len_e_)
+    HIP_CHECK(hipMemcpyAsync(psi_side_[0], src + nf, psi_len_e_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+  if (psi_len_h_)
+    HIP_CHECK(hipMemcpyAsync(psi_side_[1], src + nf + psi_len_e_, psi_len_h_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+}
+
+template <typename real>
+void Stepper<real>::checkpoint_enable(int interval) {
+  if (fused_) throw ArgError{"checkpoints need the generic kernels: create with PF_FLAG_GENERIC"};
+  if (interval < 1) throw ArgError{"checkpoint interval must be positive"};
+  if (!ade_.empty()) throw ArgError{"the adjoint does not support dispersive media"};
+  ckpt_interval_ = interval;
+  const size_t n = (size_t)((n_steps_ + interval - 1) / interval);
+  const size_t elems = 6 * (size_t)(shape[0] * shape[1] * shape[2]) + psi_len_e_ + psi_len_h_;
+  ckpts_.clear();
+  for (size_t c = 0; c < n; ++c) ckpts_.push_back(static_cast<real*>(arena_.alloc(elems * sizeof(real))));
+}
+
+template <typename real>
+void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_host) {
+  if (ckpt_interval_ <= 0) throw ArgError{"pf_checkpoint_enable was not called before the forward run"};
+  if (steps_done_ != n_steps_) throw ArgError{"run the forward pass to the last step first"};
+  if (!grad_host) throw ArgError{"null gradient destination"};
+  const int64_t ncell = shape[0] * shape[1] * shape[2];
+  const size_t nf = 6 * (size_t)ncell;
+  const size_t elems = nf + psi_len_e_ + psi_len_h_;
+  const int C = ckpt_interval_;
+  DeviceArena scratch;   // freed when the sweep ends
+  // cotangents: per monitor [n_comp][n_freq][nbox][2], zero where not given
+  std::vector<double*> cot_dev(dft_.size(), nullptr);
+  for (size_t m = 0; m < dft_.size(); ++m)
+    cot_dev[m] = scratch.zeros<double>((size_t)dft_[m].n_comp * dft_[m].n_freq * dft_[m].nbox * 2, stream_);
+  for (int c = 0; c < n_cot; ++c) {
+    const pf_cotangent& g = cot[c];
+    if (g.mon < 0 || g.mon >= (int)dft_.size() || g.slot < 0 || g.slot >= dft_[g.mon].n_comp || !g.g)
+      throw ArgError{"bad cotangent"};
+    const Dft& m = dft_[g.mon];
+    const size_t n = (size_t)m.n_freq * m.nbox * 2;
+    HIP_CHECK(hipMemcpy(cot_dev[g.mon] + (size_t)g.slot * n, g.g, n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  real* lam = scratch.zeros<real>(elems, stream_);
+  real* lam_d = scratch.zeros<real>(6 * (size_t)ncell, stream_);
+  double* grad = scratch.zeros<double>(ncell, stream_);
+  real* traj = static_cast<real*>(scratch.alloc((size_t)C * elems * sizeof(real)));
+  real* lam_tab[6];
+  for (int c = 0; c < 6; ++c) lam_tab[c] = lam + (size_t)c * ncell;
+  real** lam_dev = scratch.upload(lam_tab, 6, stream_);
+
+  pfhip::AdjArgs<real> a;
+  std::memset(&a, 0, sizeof(a));
+  for (int c = 0; c < 6; ++c) {
+    a.f[c] = f_[0][c];
+    a.lam_f[c] = lam_tab[c];
+    a.lam_d[c] = lam_d + (size_t)c * ncell;
+  }
+  a.ce = args_.ce;
+  for (int sl = 0; sl < 12; ++sl) {
+    const size_t off = sl < 6 ? (size_t)(args_.psi[sl] - args_.psi[0])
+                              : psi_len_e_ + (size_t)(args_.psi[sl] - args_.psi[6]);
+    a.lam_psi[sl] = lam + nf + off;
+    a.psi_fwd[sl] = args_.psi[sl];   // E side: the live state; H side set per step
+  }
+  a.grad = grad;
+  for (int ax = 0; ax < 3; ++ax) {
+    a.b_e[ax] = args_.b_e[ax];
+    a.c_e[ax] = args_.c_e[ax];
+    a.b_h[ax] = args_.b_h[ax];
+    a.c_h[ax] = args_.c_h[ax];
+    a.map_e[ax] = args_.map_e[ax];
+    a.map_h[ax] = args_.map_h[ax];
+    a.pe[ax] = args_.pe[ax];
+    a.ph[ax] = args_.ph[ax];
+    a.inv_d[ax] = args_.inv_d[ax];
+  }
+  a.nx = args_.nx;
+  a.ny = args_.ny;
+  a.nz = args_.nz;
+  a.nyz = args_.nyz;
+  a.ch = args_.ch;
+
+  const int64_t nseg = (n_steps_ + C - 1) / C;
+  for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+    const int64_t s0 = sg * C, s1 = std::min<int64_t>(s0 + C, n_steps_);
+    load_state(ckpts_[sg]);
+    for (int64_t st = s0; st < s1; ++st) {
+      save_state(traj + (size_t)(st - s0) * elems);
+      step_once(st, false);
+    }
+    for (int64_t st = s1 - 1; st >= s0; --st) {
+      const real* T = traj + (size_t)(st - s0) * elems;
+      // adjoint sources of the samples taken after this step
+      for (size_t m = 0; m < dft_.size(); ++m) {
+        const Dft& d = dft_[m];
+        if (st % d.interval != 0) continue;
+        pfhip::launch_adj_dft_source<real>(lam_dev, d.comps_dev, d.is_h_dev, d.n_comp, d.box,
+                                           args_.nyz, args_.nz, cot_dev[m], d.n_freq, d.phase_e,
+                                           d.phase_h, st / d.interval, d.dt, stream_);
+      }
+      // rebuild H^{n+1/2} from the stored pre-step state
+      load_state(T);
+      pfhip::launch_generic_h<real>(args_, exact_, stream_);
+      pfhip::launch_inject<real>(fields_dev_[0], src_h_, n_steps_, st, stream_);
+      for (int sl = 0; sl < 6; ++sl)
+        a.psi_fwd[6 + sl] = T + nf + psi_len_e_ + (size_t)(args_.psi[6 + sl] - args_.psi[6]);
+      for (int phase = 0; phase < 4; ++phase) pfhip::launch_adjoint_step<real>(a, stream_, phase);
+    }
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(grad_host, grad, (size_t)ncell * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  scratch.release();
 }
 
 template <typename real>
@@ -742,12 +886,17 @@ void Stepper<real>::advance(int64_t step0, int64_t n) {
   if ((src_h_.n_groups || src_e_.n_groups) && step0 + n > n_steps_)
     throw ArgError{"step range exceeds the source waveform table"};
   for (int64_t s = step0; s < step0 + n; ++s) {
+    if (ckpt_interval_ > 0 && s % ckpt_interval_ == 0) {
+      if (s / ckpt_interval_ >= (int64_t)ckpts_.size()) throw ArgError{"step beyond the checkpoint table"};
+      save_state(ckpts_[s / ckpt_interval_]);
+    }
     step_once(s);
     if (timing_ && pending_.size() > 4096) collect_timing();
   }
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(stream_));
   collect_timing();
+  steps_done_ = step0 + n;
 }
 
 template <typename real>
@@ -882,6 +1031,17 @@ int pf_timing_enable(pf_handle* h, int32_t on) {
 int pf_timing_read(pf_handle* h, double* ms, int64_t* launches) {
   if (!h) return fail(PF_E_ARG, "null handle");
   return guarded([&] { h->timing_read(ms, launches); });
+}
+
+int pf_checkpoint_enable(pf_handle* h, int32_t interval) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->checkpoint_enable(interval); });
+}
+
+int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* grad_ce) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  if (n_cot < 0 || (n_cot > 0 && !cot)) return fail(PF_E_ARG, "bad cotangent list");
+  return guarded([&] { h->adjoint(n_cot, cot, grad_ce); });
 }
 
 int pf_device_info(char* name, int32_t name_len, int64_t* free_mb, int64_t* total_mb) {

@@ -89,6 +89,29 @@ struct MarchTile {
   int i0, i1, j0, jn, k0, kn, pad0, pad1;
 };
 
+// Arguments of the adjoint kernels (generic in-place layout).
+template <typename real>
+struct AdjArgs {
+  const real* f[6];          // forward fields: E^n and H^{n+1/2}
+  const real* ce;
+  const real* psi_fwd[12];   // forward psi before the respective update
+  real* lam_f[6];            // adjoint fields (updated in place)
+  real* lam_psi[12];         // adjoint psi (updated in place)
+  real* lam_d[6];            // per-term scratch, one nx*ny*nz array each
+  double* grad;              // dJ/dce accumulator
+  const real* b_e[3];
+  const real* c_e[3];
+  const real* b_h[3];
+  const real* c_h[3];
+  const int* map_e[3];
+  const int* map_h[3];
+  int pe[3], ph[3];
+  int nx, ny, nz;
+  int64_t nyz;
+  real inv_d[3];
+  real ch;
+};
+
 struct BoxDev {
   int64_t start[3];
   int64_t count[3];
@@ -136,5 +159,12 @@ template <typename real> void launch_fused(FusedArgs<real>& a, bool exact, hipSt
 template <typename real>
 void launch_ade_pp(real* const* Ein, real* const* Eout, const int64_t* cells, int64_t n, real* Pcur,
                    real* Pprev, const real* abc, int n_poles, real inv_eps, hipStream_t s);
+
+template <typename real> void launch_adjoint_step(const AdjArgs<real>& a, hipStream_t s, int phase);
+template <typename real>
+void launch_adj_dft_source(real* const* lam_dev, const int32_t* comps, const int32_t* is_h,
+                           int n_comp, const BoxDev& box, int64_t nyz, int nz, const double* cot,
+                           int n_freq, const double* phase_e, const double* phase_h,
+                           int64_t sample, double dt, hipStream_t s);
 
 }  // namespace pfhip

@@ -139,7 +139,7 @@ def build_tables(sim, npdt) -> Dict:
 class HipEngine:
     """One device-resident simulation behind the C ABI."""
 
-    def __init__(self, sim, npdt, exact: bool):
+    def __init__(self, sim, npdt, exact: bool, flags: int = 0):
         self.lib = N.load_library()
         self.npdt = np.dtype(npdt).type
         self.shape = sim.grid.shape
@@ -147,6 +147,7 @@ class HipEngine:
         keep: List = []                       # host arrays alive until pf_create returns
         d = N.PfDesc()
         d.dtype = N.PF_F64 if self.npdt == np.float64 else N.PF_F32
+        d.flags = int(flags)
         d.exact = 1 if exact else 0
         d.shape[:] = self.shape
         d.cell[:] = t["cell"]
@@ -270,6 +271,30 @@ class HipEngine:
                                                        C.POINTER(C.c_double))), "pf_dft_read")
             out[c] = buf
         return out
+
+    def enable_checkpoints(self, interval: int) -> None:
+        N.check(self.lib, self.lib.pf_checkpoint_enable(self.h, int(interval)),
+                "pf_checkpoint_enable")
+
+    def adjoint(self, cotangents: Dict[str, Dict[str, np.ndarray]]) -> np.ndarray:
+        """dJ/dce per node given dJ/d(DFT) as complex arrays (Re/Im parts of
+        the derivative w.r.t. the real and imaginary accumulator parts)."""
+        keep, items = [], []
+        for name, comps in cotangents.items():
+            mon, names, count, n_freq = self.dft_slots[name]
+            for c, g in comps.items():
+                arr = np.ascontiguousarray(np.asarray(g, dtype=np.complex128)
+                                           .reshape((n_freq,) + count))
+                keep.append(arr)
+                items.append((mon, names.index(c), arr))
+        cot = (N.PfCotangent * max(len(items), 1))()
+        for n_, (mon, slot, arr) in enumerate(items):
+            cot[n_].mon, cot[n_].slot = mon, slot
+            cot[n_].g = arr.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+        grad = np.zeros(self.shape, dtype=np.float64)
+        N.check(self.lib, self.lib.pf_adjoint(self.h, len(items), cot, N.as_ptr(grad, C.c_double)),
+                "pf_adjoint")
+        return grad
 
     def flux(self) -> Dict[str, float]:
         if not self.flux_names:
