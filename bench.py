@@ -67,6 +67,60 @@ def dispersive_slab(pf, n=512, steps=2000, precision="float32", backend="hip"):
                          backend=backend)
 
 
+def batched_2d_scenes(pf, n_sims=64, lo=256, hi=1024, steps=4000, precision="float32", seed=0,
+                      backend="hip", component="Ex"):
+    """Independent 2-D scenes of uneven size for batched stepping.
+
+    Sides uniform in [lo, hi] cells (square), dx = 20 nm, 12-cell CPML on x
+    and y, the 2-D default Courant number. Each scene holds 1-4 PolySlab
+    scatterers (3-8 vertices at sorted random angles around a random centre,
+    radius 0.3-1.5 um, clipped to the region between the DFT planes), index
+    uniform in [1.44, 3.48], 20% of them a single-pole Lorentz medium; a
+    point dipole on ``component`` (Gaussian pulse at 1.55 um, 10 fs FWHM);
+    two x-plane DFT monitors (Ey, Hz for TE, Ez, Hy for TM) at
+    x = +-(L/2 - 1 um), 16 frequencies in [c/1.7 um, c/1.4 um].
+    """
+    rng = np.random.default_rng(seed)
+    te = component in ("Ex", "Ey", "Hz")
+    comps = ("Ey", "Hz") if te else ("Ez", "Hy")
+    freqs = np.linspace(pf.C_0 / 1.7e-6, pf.C_0 / 1.4e-6, 16)
+    sims = []
+    for _ in range(n_sims):
+        n = int(rng.integers(lo, hi + 1))
+        L = n * DX
+        sc = min(1.0, L / (256 * DX))     # margins shrink only below 256 cells
+        xp = L / 2 - 1e-6 * sc
+        ylim = L / 2 - 14 * DX
+        structures = []
+        for _p in range(int(rng.integers(1, 5))):
+            nv = int(rng.integers(3, 9))
+            c = rng.uniform(-(xp - 0.5e-6 * sc), xp - 0.5e-6 * sc, size=2)
+            r = rng.uniform(0.3e-6, 1.5e-6) * sc
+            ang = np.sort(rng.uniform(0, 2 * np.pi, size=nv))
+            vx = np.clip(c[0] + r * np.cos(ang), -(xp - 2 * DX), xp - 2 * DX)
+            vy = np.clip(c[1] + r * np.sin(ang), -ylim, ylim)
+            idx = rng.uniform(1.44, 3.48)
+            if rng.uniform() < 0.2:
+                med = pf.DispersiveMedium.lorentz(
+                    idx ** 2, [(rng.uniform(0.1, 1.0), rng.uniform(100e12, 400e12),
+                                rng.uniform(0.0, 20e12))])
+            else:
+                med = pf.Medium.from_index(idx)
+            structures.append(pf.PolySlab(vertices=tuple(zip(vx, vy)), z_bounds=(-1.0, 1.0),
+                                          medium=med))
+        src = pf.PointDipole(position=(-(L / 2 - 0.6e-6 * sc), 0.0), component=component,
+                             waveform=pf.GaussianPulse(freq0=pf.C_0 / 1.55e-6, fwhm=10e-15))
+        mons = [pf.DFTMonitor(name=nm, components=comps, freqs=freqs, plane_axis="x",
+                              plane_position=x) for nm, x in (("in", -xp), ("out", xp))]
+        grid = pf.Grid(size=(L, L), cell_size=DX, pml_layers=(12, 12, 0))
+        probe = pf.Simulation(grid, structures=structures, sources=[src], monitors=mons,
+                              run_time=1e-15, precision=precision, backend=backend)
+        sims.append(pf.Simulation(grid, structures=structures, sources=[src], monitors=mons,
+                                  run_time=(steps - 0.5) * probe.dt, precision=precision,
+                                  backend=backend))
+    return sims
+
+
 def _rel(a, b):
     return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) /
                  max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 2
+#define PFHIP_ABI_VERSION 3
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -166,6 +166,25 @@ int pf_flux_read(pf_handle* h, double* dst);
  * node (float64, nx*ny*nz) to grad_ce. Dispersive media are rejected. */
 int pf_checkpoint_enable(pf_handle* h, int32_t interval);
 int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* grad_ce);
+/* Batched 2-D stepping (no reference counterpart: the reference runs one
+ * simulation per Simulation.run). A batch holds n independent xy-plane
+ * simulations (nz == 1) of one dtype, each described by a pf_desc, with
+ * pols[i] = 0 (TE: Ex, Ey, Hz) or 1 (TM: Ez, Hx, Hy) the group its sources
+ * excite. Supported per simulation: CPML, soft sources, ADE media and DFT
+ * monitors (no flux). pf_batch_advance(b, n) advances every simulation n
+ * steps (a simulation stops at its own step count). Modes: 0 one launch per
+ * phase, 1 one persistent cooperative launch, 2 per-phase launches replayed
+ * from a captured hipGraph. pf_batch_tune times the candidate (mode,
+ * workgroup size, cells per work chunk) settings and resets the batch. */
+typedef struct pf_batch pf_batch;
+int pf_batch_create(const pf_desc* descs, const int32_t* pols, int32_t n, pf_batch** out);
+int pf_batch_destroy(pf_batch* b);
+int pf_batch_configure(pf_batch* b, int32_t mode, int32_t block, int32_t chunk);
+int pf_batch_tune(pf_batch* b, int32_t trial_steps, int32_t* mode, int32_t* block, int32_t* chunk,
+                  double* ms_per_step);
+int pf_batch_advance(pf_batch* b, int64_t n);
+int pf_batch_dft_read(pf_batch* b, int32_t sim, int32_t mon, int32_t slot, double* dst);
+int pf_batch_read_field(pf_batch* b, int32_t sim, int32_t comp, void* dst);
 /* Event timing (no reference counterpart; used by the benchmark): when on,
  * each kernel-class launch group of pf_advance is bracketed by HIP events on
  * the handle's stream. Enabling resets the totals. pf_timing_read returns,

@@ -11,7 +11,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 PF_FLAG_GENERIC = 1
 
 PF_F32, PF_F64 = 0, 1
@@ -74,6 +74,13 @@ SIGNATURES = {
     "pf_device_info": (_i32, [C.c_char_p, _i32, _P(_i64), _P(_i64)]),
     "pf_last_error": (C.c_char_p, []),
     "pf_abi_version": (_i32, []),
+    "pf_batch_create": (_i32, [_P(PfDesc), _P(_i32), _i32, _P(_vp)]),
+    "pf_batch_destroy": (_i32, [_vp]),
+    "pf_batch_configure": (_i32, [_vp, _i32, _i32, _i32]),
+    "pf_batch_tune": (_i32, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32), _P(_f64)]),
+    "pf_batch_advance": (_i32, [_vp, _i64]),
+    "pf_batch_dft_read": (_i32, [_vp, _i32, _i32, _i32, _P(_f64)]),
+    "pf_batch_read_field": (_i32, [_vp, _i32, _i32, _vp]),
 }
 
 _LIB = None

@@ -1,0 +1,187 @@
+"""Batched 2-D stepping: many independent xy-plane simulations in one launch.
+
+The reference runs one simulation per ``Simulation.run()``
+(``simulation.py:896-904``); a batch of design candidates is a Python loop of
+runs. ``SimBatch`` holds n simulations of one precision on the device and
+advances them together: the per-phase kernels (or one persistent cooperative
+kernel) walk a list of fixed-size work chunks that spans every grid, so
+uneven sizes share the launch without padding. Each simulation keeps the
+single-run semantics of ``backend="hip"``: the same tables
+(``hip_backend.build_desc``) and the same per-node arithmetic, so a batched
+simulation reproduces its single run bit for bit.
+
+Only the polarisation group that the sources excite is stepped (TE: Ex, Ey,
+Hz; TM: Ez, Hx, Hy); the other group stays exactly zero in 2-D, as it does in
+the reference's six-component update. Supported per simulation: CPML, soft
+sources, dispersive (ADE) media and DFT monitors. FieldMonitor and
+FluxMonitor need per-simulation host stops and are rejected.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .hip_backend import build_desc, exact_mode, validate
+from .monitors import DFTMonitor, FieldMonitor, FluxMonitor
+from .simulation import Result
+
+TE_COMPONENTS = ("Ex", "Ey", "Hz")
+TM_COMPONENTS = ("Ez", "Hx", "Hy")
+MODES = {"phases": 0, "persistent": 1, "graph": 2}
+
+
+def polarisation(sim) -> int:
+    """0 for TE (sources on Ex/Ey/Hz), 1 for TM (Ez/Hx/Hy)."""
+    comps = {s.component for s in sim.sources}
+    if not comps:
+        raise ValueError("a batched simulation needs at least one source")
+    if comps <= set(TE_COMPONENTS):
+        return 0
+    if comps <= set(TM_COMPONENTS):
+        return 1
+    raise ValueError(f"sources mix the TE and TM groups: {sorted(comps)}")
+
+
+def _check(sim) -> np.dtype:
+    npdt = validate(sim)
+    nx, ny, nz = sim.grid.shape
+    if nz != 1 or nx < 2 or ny < 2:
+        raise ValueError(f"batched stepping needs an xy-plane 2-D grid, got shape {sim.grid.shape}")
+    for m in sim.monitors:
+        if isinstance(m, (FieldMonitor, FluxMonitor)):
+            raise ValueError(f"monitor {m.name!r}: a batch supports DFTMonitor only")
+    polarisation(sim)
+    return npdt
+
+
+class SimBatch:
+    """n 2-D simulations of one precision advanced together on the device."""
+
+    def __init__(self, sims: Sequence, mode: str = "persistent", block: int = 256,
+                 chunk: int = 4096):
+        if not sims:
+            raise ValueError("empty batch")
+        dts = {_check(s) for s in sims}
+        if len(dts) != 1:
+            raise ValueError("a batch needs one compute precision")
+        self.npdt = dts.pop()
+        self.sims = list(sims)
+        self.lib = N.load_library()
+        n = len(sims)
+        descs = (N.PfDesc * n)()
+        pols = (C.c_int32 * n)()
+        keep: List = []
+        self.slots: List[Dict[str, tuple]] = []
+        for i, sim in enumerate(sims):
+            d, k, t, slots, _ = build_desc(sim, self.npdt, exact_mode(self.npdt), N.PF_FLAG_GENERIC)
+            keep += [d, k, t]
+            descs[i] = d
+            pols[i] = polarisation(sim)
+            self.slots.append(slots)
+        h = C.c_void_p()
+        N.check(self.lib, self.lib.pf_batch_create(descs, pols, n, C.byref(h)), "pf_batch_create")
+        self.h = h
+        self.step = 0
+        self.n_steps = max(s.n_steps for s in sims)
+        del keep
+        self.configure(mode, block, chunk)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.pf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, mode: str, block: int = 256, chunk: int = 4096) -> None:
+        if mode not in MODES:
+            raise ValueError(f"mode must be one of {sorted(MODES)}")
+        N.check(self.lib, self.lib.pf_batch_configure(self.h, MODES[mode], int(block), int(chunk)),
+                "pf_batch_configure")
+        self.mode, self.block, self.chunk = mode, int(block), int(chunk)
+
+    def tune(self, trial_steps: int = 100) -> Dict:
+        """Time the candidate launch settings, keep the fastest and reset
+        every simulation to step 0."""
+        m, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        ms = C.c_double()
+        N.check(self.lib, self.lib.pf_batch_tune(self.h, int(trial_steps), C.byref(m), C.byref(b),
+                                                 C.byref(c), C.byref(ms)), "pf_batch_tune")
+        self.mode = {v: k for k, v in MODES.items()}[m.value]
+        self.block, self.chunk = b.value, c.value
+        self.step = 0
+        return dict(mode=self.mode, block=self.block, chunk=self.chunk, ms_per_step=ms.value)
+
+    def advance(self, n: Optional[int] = None) -> None:
+        n = self.n_steps - self.step if n is None else int(n)
+        N.check(self.lib, self.lib.pf_batch_advance(self.h, n), "pf_batch_advance")
+        self.step += n
+
+    def field(self, i: int, comp: str) -> np.ndarray:
+        out = np.empty(self.sims[i].grid.shape, dtype=self.npdt)
+        N.check(self.lib, self.lib.pf_batch_read_field(self.h, int(i), N.COMP_CODE[comp],
+                                                       out.ctypes.data), "pf_batch_read_field")
+        return out
+
+    def dft(self, i: int, name: str) -> Dict[str, np.ndarray]:
+        mon, comps, count, n_freq = self.slots[i][name]
+        out = {}
+        for slot, c in enumerate(comps):
+            buf = np.empty((n_freq,) + count, dtype=np.complex128)
+            N.check(self.lib, self.lib.pf_batch_dft_read(
+                self.h, int(i), mon, slot, buf.view(np.float64).ctypes.data_as(
+                    C.POINTER(C.c_double))), "pf_batch_dft_read")
+            out[c] = buf
+        return out
+
+    def results(self) -> List[Result]:
+        out = []
+        for i, sim in enumerate(self.sims):
+            r = Result(times=np.arange(sim.n_steps) * sim.dt)
+            for m in sim.monitors:
+                if isinstance(m, DFTMonitor):
+                    r.dft_freqs[m.name] = np.asarray(m.freqs, dtype=np.float64)
+                    r.dft[m.name] = self.dft(i, m.name)
+            out.append(r)
+        return out
+
+
+def run_batch(sims: Sequence, mode: str = "persistent", tune: bool = False,
+              trial_steps: int = 100) -> List[Result]:
+    """Run every simulation to its own step count; one ``Result`` each."""
+    b = SimBatch(sims, mode=mode)
+    try:
+        if tune:
+            b.tune(trial_steps)
+        b.advance()
+        return b.results()
+    finally:
+        b.close()
+
+
+def dft_flux(fields: Dict[str, np.ndarray], axis: str, dA: float) -> np.ndarray:
+    """Spectral flux Re(E x H*) . n summed over a DFT plane, per frequency.
+
+    ``fields`` maps component names to (n_freq, ...) complex arrays of one
+    DFTMonitor; missing components count as zero. The reference has no
+    spectral flux (its FluxMonitor is time-domain, simulation.py:1440-1459);
+    the Poynting components follow the same S . n of that function.
+    """
+    z = None
+    pairs = {"x": (("Ey", "Hz", 1), ("Ez", "Hy", -1)),
+             "y": (("Ez", "Hx", 1), ("Ex", "Hz", -1)),
+             "z": (("Ex", "Hy", 1), ("Ey", "Hx", -1))}[axis]
+    for e, h, sign in pairs:
+        if e in fields and h in fields:
+            term = sign * np.real(fields[e] * np.conj(fields[h]))
+            z = term if z is None else z + term
+    if z is None:
+        raise ValueError(f"no Poynting pair for axis {axis!r} in {sorted(fields)}")
+    return z.reshape(z.shape[0], -1).sum(axis=1) * dA

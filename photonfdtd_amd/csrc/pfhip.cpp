@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -199,6 +200,11 @@ class Stepper final : public pf_handle {
   void timing_enable(bool on) override;
   void checkpoint_enable(int interval) override;
   void adjoint(int n_cot, const pf_cotangent* cot, double* grad) override;
+  // batched 2-D stepping support
+  void describe(pfhip::SimDev<real>& s, int pol);
+  void reset(hipStream_t s);
+  hipStream_t stream() const { return stream_; }
+  int64_t ncell() const { return shape[0] * shape[1] * shape[2]; }
   void timing_read(double* ms, int64_t* launches) override;
 
  private:
@@ -751,6 +757,107 @@ void Stepper<real>::load_state(const real* src) {
 }
 
 template <typename real>
+void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol) {
+  if (fused_) throw ArgError{"batched stepping needs the generic layout (PF_FLAG_GENERIC)"};
+  if (shape[2] != 1 || shape[0] < 2 || shape[1] < 2)
+    throw ArgError{"batched stepping supports xy-plane 2-D grids (nx, ny >= 2, nz == 1)"};
+  if (shape[0] * shape[1] >= (int64_t(1) << 31)) throw ArgError{"grid too large for a batch"};
+  if (pol != 0 && pol != 1) throw ArgError{"polarisation must be 0 (TE) or 1 (TM)"};
+  if (dft_.size() > (size_t)pfhip::BATCH_MAX_DFT) throw ArgError{"too many DFT monitors for a batch"};
+  if (ade_.size() > (size_t)pfhip::BATCH_MAX_ADE) throw ArgError{"too many dispersive media for a batch"};
+  if (!flux_.empty()) throw ArgError{"flux monitors are not supported in a batch"};
+  std::memset(&s, 0, sizeof(s));
+  for (int c = 0; c < 6; ++c) s.f[c] = f_[0][c];
+  s.ce = args_.ce;
+  for (int q = 0; q < pfhip::PSI_COUNT; ++q) s.psi[q] = args_.psi[q];
+  for (int a = 0; a < 3; ++a) {
+    s.b_e[a] = args_.b_e[a];
+    s.c_e[a] = args_.c_e[a];
+    s.b_h[a] = args_.b_h[a];
+    s.c_h[a] = args_.c_h[a];
+    s.map_e[a] = args_.map_e[a];
+    s.map_h[a] = args_.map_h[a];
+    s.pe[a] = args_.pe[a];
+    s.ph[a] = args_.ph[a];
+    s.d[a] = args_.d[a];
+    s.inv_d[a] = args_.inv_d[a];
+  }
+  s.nx = args_.nx;
+  s.ny = args_.ny;
+  s.ch = args_.ch;
+  s.pol = pol;
+  s.n_steps = n_steps_;
+  s.src_e = src_e_;
+  s.src_h = src_h_;
+  for (int side = 0; side < 2; ++side) {
+    const pfhip::SourceGroupsDev& g = side ? src_h_ : src_e_;
+    s.src_lo[side] = INT64_MAX;
+    s.src_hi[side] = -1;
+    if (g.n_groups > 0) {
+      std::vector<int64_t> cells(g.n_groups);
+      HIP_CHECK(hipMemcpy(cells.data(), g.cell, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+      s.src_lo[side] = *std::min_element(cells.begin(), cells.end());
+      s.src_hi[side] = *std::max_element(cells.begin(), cells.end());
+    }
+  }
+  if (!ade_.empty()) {
+    std::vector<int> code((size_t)ncell(), -1);
+    for (size_t m = 0; m < ade_.size(); ++m) {
+      if (ade_[m].n >= (int64_t(1) << 26)) throw ArgError{"dispersive region too large for a batch"};
+      std::vector<int64_t> cells(ade_[m].n);
+      HIP_CHECK(hipMemcpy(cells.data(), ade_[m].cells, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+      for (int64_t t = 0; t < ade_[m].n; ++t) code[cells[t]] = (int)(m << 26) | (int)t;
+      pfhip::AdeDev<real>& A = s.ade[m];
+      A.n = ade_[m].n;
+      A.n_poles = ade_[m].n_poles;
+      A.abc = ade_[m].abc;
+      A.inv_eps = ade_[m].inv_eps;
+      A.Pcur = ade_[m].Pcur;
+      A.Pprev = ade_[m].Pprev;
+    }
+    s.ade_code = arena_.upload(code.data(), code.size(), stream_);
+    s.n_ade = (int)ade_.size();
+  }
+  s.n_dft = (int)dft_.size();
+  for (size_t m = 0; m < dft_.size(); ++m) {
+    pfhip::DftDev& d = s.dft[m];
+    d.n_comp = dft_[m].n_comp;
+    for (int c = 0; c < 6; ++c) d.comps[c] = c < d.n_comp ? dft_[m].comps[c] : -1;
+    d.box = dft_[m].box;
+    for (int a = 0; a < 2; ++a) {
+      d.lo[a] = (int)d.box.start[a];
+      d.hi[a] = (int)(d.box.start[a] + (d.box.count[a] - 1) * d.box.stride[a]);
+    }
+    d.n_freq = dft_[m].n_freq;
+    d.interval = dft_[m].interval;
+    d.phase_e = dft_[m].phase_e;
+    d.phase_h = dft_[m].phase_h;
+    d.dt = dft_[m].dt;
+    d.acc = dft_[m].acc;
+  }
+}
+
+template <typename real>
+void Stepper<real>::reset(hipStream_t s) {
+  const size_t nf = 6 * (size_t)ncell();
+  HIP_CHECK(hipMemsetAsync(lvl_[0], 0, nf * sizeof(real), s));
+  if (psi_len_e_) HIP_CHECK(hipMemsetAsync(psi_side_[0], 0, psi_len_e_ * sizeof(real), s));
+  if (psi_len_h_) HIP_CHECK(hipMemsetAsync(psi_side_[1], 0, psi_len_h_ * sizeof(real), s));
+  for (Ade& a : ade_) {
+    const size_t n = 3 * (size_t)a.n_poles * a.n;
+    if (n) {
+      HIP_CHECK(hipMemsetAsync(a.Pcur, 0, n * sizeof(real), s));
+      HIP_CHECK(hipMemsetAsync(a.Pprev, 0, n * sizeof(real), s));
+    }
+  }
+  for (Dft& m : dft_) {
+    HIP_CHECK(hipMemsetAsync(m.acc, 0, (size_t)m.n_comp * m.n_freq * m.nbox * 2 * sizeof(double), s));
+    m.count = 0;
+    m.sample0 = 0;
+  }
+}
+
+template <typename real>
 void Stepper<real>::checkpoint_enable(int interval) {
   if (fused_) throw ArgError{"checkpoints need the generic kernels: create with PF_FLAG_GENERIC"};
   if (interval < 1) throw ArgError{"checkpoint interval must be positive"};
@@ -953,6 +1060,188 @@ void Stepper<real>::flux_read(double* dst) {
   HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
+// ---------------------------------------------------------------------------
+// Batch of 2-D simulations advanced together.
+// ---------------------------------------------------------------------------
+struct BatchBase {
+  virtual ~BatchBase() = default;
+  virtual void configure(int mode, int block, int chunk) = 0;
+  virtual void tune(int trial_steps, int* mode, int* block, int* chunk, double* ms) = 0;
+  virtual void advance(int64_t n) = 0;
+  virtual pf_handle* sim(int i) = 0;
+  virtual int size() const = 0;
+};
+
+template <typename real>
+class Batch final : public BatchBase {
+ public:
+  Batch(const pf_desc* descs, int n, const int* pols) : exact_(descs[0].exact != 0) {
+    if (n < 1) throw ArgError{"empty batch"};
+    for (int i = 1; i < n; ++i)
+      if ((descs[i].exact != 0) != exact_) throw ArgError{"a batch needs one arithmetic mode"};
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    std::vector<pfhip::SimDev<real>> host(n);
+    for (int i = 0; i < n; ++i) {
+      pf_desc d = descs[i];
+      d.flags |= PF_FLAG_GENERIC;
+      sims_.emplace_back(new Stepper<real>(d));
+      sims_.back()->describe(host[i], pols[i]);
+      n_steps_max_ = std::max<int64_t>(n_steps_max_, host[i].n_steps);
+    }
+    sims_dev_ = arena_.upload(host.data(), host.size(), stream_);
+    step_dev_ = arena_.zeros<int64_t>(1, stream_);
+    configure(1, 256, 4096);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  ~Batch() override {
+    drop_graph();
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      arena_.release();
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+  pf_handle* sim(int i) override {
+    if (i < 0 || i >= (int)sims_.size()) throw ArgError{"simulation index out of range"};
+    return sims_[i].get();
+  }
+  int size() const override { return (int)sims_.size(); }
+
+  // mode 0: one launch per phase; 1: persistent cooperative kernel;
+  // 2: per-phase launches captured as a graph of graph_steps_ steps.
+  void configure(int mode, int block, int chunk) override {
+    if (mode < 0 || mode > 2 || (block != 256 && block != 512 && block != 1024) || chunk < 256)
+      throw ArgError{"bad batch configuration"};
+    drop_graph();
+    mode_ = mode;
+    block_ = block;
+    if (chunk != chunk_cells_) {
+      chunk_cells_ = chunk;
+      std::vector<int> cs;
+      std::vector<int> cf;
+      for (size_t i = 0; i < sims_.size(); ++i)
+        for (int64_t c = 0; c < sims_[i]->ncell(); c += chunk) {
+          cs.push_back((int)i);
+          cf.push_back((int)c);
+        }
+      chunk_sim_ = arena_.upload(cs.data(), cs.size(), stream_);
+      chunk_first_ = arena_.upload(cf.data(), cf.size(), stream_);
+      n_chunks_ = (int)cs.size();
+    }
+    args_.sims = sims_dev_;
+    args_.chunk_sim = chunk_sim_;
+    args_.chunk_first = chunk_first_;
+    args_.n_chunks = n_chunks_;
+    args_.chunk_cells = chunk_cells_;
+    args_.step = step_dev_;
+  }
+
+  void advance(int64_t n) override {
+    if (n < 0 || step_ + n > n_steps_max_) throw ArgError{"step range exceeds the batch"};
+    HIP_CHECK(hipMemcpyAsync(step_dev_, &step_, sizeof(int64_t), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    const int grid = std::min(n_chunks_, 4096);
+    if (mode_ == 1) {
+      int g = 0;
+      HIP_CHECK(pfhip::launch_batch_persistent<real>(args_, exact_, block_, step_, step_ + n, stream_, &g));
+    } else {
+      int64_t done = 0;
+      if (mode_ == 2) {
+        if (!graph_exec_) build_graph(grid);
+        for (; done + graph_steps_ <= n; done += graph_steps_)
+          HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+      }
+      for (; done < n; ++done) {
+        pfhip::launch_batch_phase<real>(args_, 0, exact_, grid, block_, stream_);
+        pfhip::launch_batch_phase<real>(args_, 1, exact_, grid, block_, stream_);
+        pfhip::launch_batch_tick(step_dev_, stream_);
+      }
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    step_ += n;
+  }
+
+  // Times each candidate (mode, block, chunk) over trial_steps steps from a
+  // zero state, keeps the fastest and resets every simulation to step 0.
+  void tune(int trial_steps, int* mode, int* block, int* chunk, double* ms) override {
+    const int modes[2] = {1, 2};
+    const int blocks[2] = {256, 512};
+    const int chunks[3] = {1024, 4096, 16384};
+    int64_t steps = std::min<int64_t>(trial_steps, n_steps_max_);
+    double best = 1e300;
+    int bm = 1, bb = 256, bc = 4096;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    for (int m : modes)
+      for (int b : blocks)
+        for (int c : chunks) {
+          configure(m, b, c);
+          reset_all();
+          HIP_CHECK(hipEventRecord(e0, stream_));
+          advance(steps);
+          HIP_CHECK(hipEventRecord(e1, stream_));
+          HIP_CHECK(hipEventSynchronize(e1));
+          float t = 0.f;
+          HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+          if (t < best) {
+            best = t;
+            bm = m;
+            bb = b;
+            bc = c;
+          }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    configure(bm, bb, bc);
+    reset_all();
+    if (mode) *mode = bm;
+    if (block) *block = bb;
+    if (chunk) *chunk = bc;
+    if (ms) *ms = best / std::max<int64_t>(steps, 1);
+  }
+
+ private:
+  void reset_all() {
+    for (auto& s : sims_) s->reset(stream_);
+    step_ = 0;
+    HIP_CHECK(hipMemsetAsync(step_dev_, 0, sizeof(int64_t), stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  void build_graph(int grid) {
+    hipGraph_t g;
+    HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < graph_steps_; ++s) {
+      pfhip::launch_batch_phase<real>(args_, 0, exact_, grid, block_, stream_);
+      pfhip::launch_batch_phase<real>(args_, 1, exact_, grid, block_, stream_);
+      pfhip::launch_batch_tick(step_dev_, stream_);
+    }
+    HIP_CHECK(hipStreamEndCapture(stream_, &g));
+    HIP_CHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(g));
+  }
+  void drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+
+  std::vector<std::unique_ptr<Stepper<real>>> sims_;
+  DeviceArena arena_;
+  hipStream_t stream_ = nullptr;
+  pfhip::SimDev<real>* sims_dev_ = nullptr;
+  int* chunk_sim_ = nullptr;
+  int* chunk_first_ = nullptr;
+  int n_chunks_ = 0, chunk_cells_ = 0;
+  int64_t* step_dev_ = nullptr;
+  int64_t step_ = 0, n_steps_max_ = 0;
+  int mode_ = 1, block_ = 256;
+  bool exact_;
+  int graph_steps_ = 50;
+  hipGraphExec_t graph_exec_ = nullptr;
+  pfhip::BatchArgs<real> args_{};
+};
+
 template <typename F>
 int guarded(F&& f) {
   try {
@@ -1042,6 +1331,69 @@ int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* gra
   if (!h) return fail(PF_E_ARG, "null handle");
   if (n_cot < 0 || (n_cot > 0 && !cot)) return fail(PF_E_ARG, "bad cotangent list");
   return guarded([&] { h->adjoint(n_cot, cot, grad_ce); });
+}
+
+struct pf_batch {
+  std::unique_ptr<BatchBase> impl;
+};
+
+int pf_batch_create(const pf_desc* descs, const int32_t* pols, int32_t n, pf_batch** out) {
+  if (!descs || !pols || !out || n < 1) return fail(PF_E_ARG, "bad batch arguments");
+  *out = nullptr;
+  return guarded([&] {
+    for (int i = 1; i < n; ++i)
+      if (descs[i].dtype != descs[0].dtype) throw ArgError{"a batch needs one dtype"};
+    std::unique_ptr<pf_batch> b(new pf_batch);
+    if (descs[0].dtype == PF_F32) b->impl.reset(new Batch<float>(descs, n, pols));
+    else if (descs[0].dtype == PF_F64) b->impl.reset(new Batch<double>(descs, n, pols));
+    else throw ArgError{"dtype must be PF_F32 or PF_F64"};
+    *out = b.release();
+  });
+}
+
+int pf_batch_destroy(pf_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+int pf_batch_configure(pf_batch* b, int32_t mode, int32_t block, int32_t chunk) {
+  if (!b) return fail(PF_E_ARG, "null batch");
+  return guarded([&] { b->impl->configure(mode, block, chunk); });
+}
+
+int pf_batch_tune(pf_batch* b, int32_t trial_steps, int32_t* mode, int32_t* block, int32_t* chunk,
+                  double* ms_per_step) {
+  if (!b) return fail(PF_E_ARG, "null batch");
+  return guarded([&] {
+    int m = 0, bl = 0, c = 0;
+    b->impl->tune(trial_steps, &m, &bl, &c, ms_per_step);
+    if (mode) *mode = m;
+    if (block) *block = bl;
+    if (chunk) *chunk = c;
+  });
+}
+
+int pf_batch_advance(pf_batch* b, int64_t n) {
+  if (!b) return fail(PF_E_ARG, "null batch");
+  return guarded([&] { b->impl->advance(n); });
+}
+
+int pf_batch_dft_read(pf_batch* b, int32_t sim, int32_t mon, int32_t slot, double* dst) {
+  if (!b) return fail(PF_E_ARG, "null batch");
+  return guarded([&] { b->impl->sim(sim)->dft_read(mon, slot, dst); });
+}
+
+int pf_batch_read_field(pf_batch* b, int32_t sim, int32_t comp, void* dst) {
+  if (!b) return fail(PF_E_ARG, "null batch");
+  return guarded([&] {
+    pf_handle* h = b->impl->sim(sim);
+    pf_box box;
+    for (int a = 0; a < 3; ++a) {
+      box.start[a] = 0;
+      box.count[a] = h->shape[a];
+      box.stride[a] = 1;
+    }
+    h->read_box(comp, box, dst);
+  });
 }
 
 int pf_device_info(char* name, int32_t name_len, int64_t* free_mb, int64_t* total_mb) {

@@ -128,6 +128,76 @@ struct SourceGroupsDev {
   double* wave = nullptr;        // [n_wave][n_steps]
 };
 
+// ---- batched 2-D stepping -------------------------------------------------
+struct DftDev {
+  int n_comp;
+  int comps[6];
+  BoxDev box;
+  int lo[2], hi[2];       // first and last (i, j) node of the box
+  int n_freq, interval;
+  const double* phase_e;
+  const double* phase_h;
+  double dt;
+  double* acc;            // [slot][freq][box][2]
+};
+
+template <typename real>
+struct AdeDev {
+  int64_t n;              // cells of this medium
+  int n_poles;
+  const real* abc;
+  real inv_eps;
+  real* Pcur;
+  real* Pprev;
+};
+
+constexpr int BATCH_MAX_DFT = 8;
+constexpr int BATCH_MAX_ADE = 8;
+
+// One simulation of a batch: an xy-plane grid (nz = 1), in-place fields.
+template <typename real>
+struct SimDev {
+  real* f[6];
+  const real* ce;
+  real* psi[PSI_COUNT];
+  const real* b_e[3];
+  const real* c_e[3];
+  const real* b_h[3];
+  const real* c_h[3];
+  const int* map_e[3];
+  const int* map_h[3];
+  int pe[3], ph[3];
+  int nx, ny;
+  real d[3], inv_d[3], ch;
+  int pol;                // 0: TE (Ex, Ey, Hz), 1: TM (Ez, Hx, Hy)
+  int64_t n_steps;
+  SourceGroupsDev src_e, src_h;
+  int64_t src_lo[2], src_hi[2];   // range of source cells per side (0: E, 1: H)
+  const int* ade_code;    // per node: (medium << 26) | index, or -1
+  int n_ade;
+  AdeDev<real> ade[BATCH_MAX_ADE];
+  int n_dft;
+  DftDev dft[BATCH_MAX_DFT];
+};
+
+template <typename real>
+struct BatchArgs {
+  const SimDev<real>* sims;
+  const int* chunk_sim;
+  const int* chunk_first;
+  int n_chunks;
+  int chunk_cells;
+  const int64_t* step;    // device step counter (per-phase kernels)
+};
+
+template <typename real>
+void launch_batch_phase(const BatchArgs<real>& a, int phase, bool exact, int grid, int block,
+                        hipStream_t s);
+void launch_batch_tick(int64_t* step, hipStream_t s);
+template <typename real>
+hipError_t launch_batch_persistent(BatchArgs<real>& a, bool exact, int block, int64_t step0,
+                                   int64_t step1, hipStream_t s, int* grid_out);
+
 template <typename real> void launch_generic_h(const StepArgs<real>& a, bool exact, hipStream_t s);
 template <typename real> void launch_generic_e(const StepArgs<real>& a, bool exact, hipStream_t s);
 template <typename real>

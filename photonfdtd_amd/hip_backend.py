@@ -136,6 +136,87 @@ def build_tables(sim, npdt) -> Dict:
     return t
 
 
+def build_desc(sim, npdt, exact: bool, flags: int = 0):
+    """Fill a ``pf_desc`` for ``sim``. Returns (desc, host arrays that must
+    stay alive until pf_create returns, tables, DFT slots, flux names)."""
+    npdt = np.dtype(npdt).type
+    t = build_tables(sim, npdt)
+    keep: List = []                       # host arrays alive until pf_create returns
+    d = N.PfDesc()
+    d.dtype = N.PF_F64 if npdt == np.float64 else N.PF_F32
+    d.flags = int(flags)
+    d.exact = 1 if exact else 0
+    d.shape[:] = sim.grid.shape
+    d.cell[:] = t["cell"]
+    d.ch = sim.dt / MU_0
+    d.ce = t["ce"].ctypes.data
+    cp = t["cpml"]
+    for a in range(3):
+        d.b_e[a] = N.as_ptr(cp[a], C.c_double)
+        d.c_e[a] = N.as_ptr(cp[3 + a], C.c_double)
+        d.b_h[a] = N.as_ptr(cp[6 + a], C.c_double)
+        d.c_h[a] = N.as_ptr(cp[9 + a], C.c_double)
+    d.n_steps = t["wave"].shape[1]
+    d.n_src = t["src_comp"].size
+    d.src_comp = N.as_ptr(t["src_comp"], C.c_int32)
+    d.src_cell = N.as_ptr(t["src_cell"], C.c_int64)
+    d.src_amp = N.as_ptr(t["src_amp"], C.c_double)
+    d.src_wave = N.as_ptr(t["src_wave"], C.c_int32)
+    d.n_wave = t["wave"].shape[0]
+    d.wave = N.as_ptr(t["wave"], C.c_double)
+
+    ade = (N.PfAdeDesc * max(len(t["ade"]), 1))()
+    for n_, med in enumerate(t["ade"]):
+        cells = np.ascontiguousarray(med.cells, dtype=np.int64)
+        abc = np.ascontiguousarray(med.abc, dtype=np.float64)
+        keep += [cells, abc]
+        ade[n_].n_cells = cells.size
+        ade[n_].cells = N.as_ptr(cells, C.c_int64)
+        ade[n_].n_poles = abc.shape[0]
+        ade[n_].abc = N.as_ptr(abc, C.c_double)
+        ade[n_].inv_eps = med.inv_eps
+    d.n_ade = len(t["ade"])
+    d.ade = ade
+
+    dft = (N.PfDftDesc * max(len(t["dft"]), 1))()
+    dft_slots: Dict[str, tuple] = {}
+    for n_, spec in enumerate(t["dft"]):
+        m = spec["mon"]
+        comps = [N.COMP_CODE[c] for c in m.components]
+        if len(set(comps)) != len(comps) or not 1 <= len(comps) <= 6:
+            raise ValueError(f"DFTMonitor {m.name!r}: components must be distinct field names")
+        ph_e = np.ascontiguousarray(spec["phase_e"]).view(np.float64)
+        ph_h = np.ascontiguousarray(spec["phase_h"]).view(np.float64)
+        keep += [ph_e, ph_h]
+        dd = dft[n_]
+        dd.n_comp = len(comps)
+        for s, c in enumerate(comps):
+            dd.comps[s] = c
+        start, count, stride = spec["box"]
+        dd.box.start[:], dd.box.count[:], dd.box.stride[:] = start, count, stride
+        dd.n_freq = len(m.freqs)
+        dd.interval = m.interval
+        dd.n_samples = spec["n_samples"]
+        dd.phase_e = N.as_ptr(ph_e, C.c_double)
+        dd.phase_h = N.as_ptr(ph_h, C.c_double)
+        dd.dt = sim.dt
+        dft_slots[m.name] = (n_, list(m.components), tuple(count), len(m.freqs))
+    d.n_dft = len(t["dft"])
+    d.dft = dft
+
+    flux = (N.PfFluxDesc * max(len(t["flux"]), 1))()
+    flux_names: List[str] = []
+    for n_, spec in enumerate(t["flux"]):
+        flux[n_].axis = spec["axis"]
+        flux[n_].index = spec["index"]
+        flux[n_].dA = spec["dA"]
+        flux[n_].dt = sim.dt
+        flux_names.append(spec["mon"].name)
+    d.n_flux = len(t["flux"])
+    d.flux = flux
+    return d, keep, t, dft_slots, flux_names
+
+
 class HipEngine:
     """One device-resident simulation behind the C ABI."""
 
@@ -143,80 +224,7 @@ class HipEngine:
         self.lib = N.load_library()
         self.npdt = np.dtype(npdt).type
         self.shape = sim.grid.shape
-        self.t = t = build_tables(sim, self.npdt)
-        keep: List = []                       # host arrays alive until pf_create returns
-        d = N.PfDesc()
-        d.dtype = N.PF_F64 if self.npdt == np.float64 else N.PF_F32
-        d.flags = int(flags)
-        d.exact = 1 if exact else 0
-        d.shape[:] = self.shape
-        d.cell[:] = t["cell"]
-        d.ch = sim.dt / MU_0
-        d.ce = t["ce"].ctypes.data
-        cp = t["cpml"]
-        for a in range(3):
-            d.b_e[a] = N.as_ptr(cp[a], C.c_double)
-            d.c_e[a] = N.as_ptr(cp[3 + a], C.c_double)
-            d.b_h[a] = N.as_ptr(cp[6 + a], C.c_double)
-            d.c_h[a] = N.as_ptr(cp[9 + a], C.c_double)
-        d.n_steps = t["wave"].shape[1]
-        d.n_src = t["src_comp"].size
-        d.src_comp = N.as_ptr(t["src_comp"], C.c_int32)
-        d.src_cell = N.as_ptr(t["src_cell"], C.c_int64)
-        d.src_amp = N.as_ptr(t["src_amp"], C.c_double)
-        d.src_wave = N.as_ptr(t["src_wave"], C.c_int32)
-        d.n_wave = t["wave"].shape[0]
-        d.wave = N.as_ptr(t["wave"], C.c_double)
-
-        ade = (N.PfAdeDesc * max(len(t["ade"]), 1))()
-        for n_, med in enumerate(t["ade"]):
-            cells = np.ascontiguousarray(med.cells, dtype=np.int64)
-            abc = np.ascontiguousarray(med.abc, dtype=np.float64)
-            keep += [cells, abc]
-            ade[n_].n_cells = cells.size
-            ade[n_].cells = N.as_ptr(cells, C.c_int64)
-            ade[n_].n_poles = abc.shape[0]
-            ade[n_].abc = N.as_ptr(abc, C.c_double)
-            ade[n_].inv_eps = med.inv_eps
-        d.n_ade = len(t["ade"])
-        d.ade = ade
-
-        dft = (N.PfDftDesc * max(len(t["dft"]), 1))()
-        self.dft_slots = {}
-        for n_, spec in enumerate(t["dft"]):
-            m = spec["mon"]
-            comps = [N.COMP_CODE[c] for c in m.components]
-            if len(set(comps)) != len(comps) or not 1 <= len(comps) <= 6:
-                raise ValueError(f"DFTMonitor {m.name!r}: components must be distinct field names")
-            ph_e = np.ascontiguousarray(spec["phase_e"]).view(np.float64)
-            ph_h = np.ascontiguousarray(spec["phase_h"]).view(np.float64)
-            keep += [ph_e, ph_h]
-            dd = dft[n_]
-            dd.n_comp = len(comps)
-            for s, c in enumerate(comps):
-                dd.comps[s] = c
-            start, count, stride = spec["box"]
-            dd.box.start[:], dd.box.count[:], dd.box.stride[:] = start, count, stride
-            dd.n_freq = len(m.freqs)
-            dd.interval = m.interval
-            dd.n_samples = spec["n_samples"]
-            dd.phase_e = N.as_ptr(ph_e, C.c_double)
-            dd.phase_h = N.as_ptr(ph_h, C.c_double)
-            dd.dt = sim.dt
-            self.dft_slots[m.name] = (n_, list(m.components), tuple(count), len(m.freqs))
-        d.n_dft = len(t["dft"])
-        d.dft = dft
-
-        flux = (N.PfFluxDesc * max(len(t["flux"]), 1))()
-        self.flux_names = []
-        for n_, spec in enumerate(t["flux"]):
-            flux[n_].axis = spec["axis"]
-            flux[n_].index = spec["index"]
-            flux[n_].dA = spec["dA"]
-            flux[n_].dt = sim.dt
-            self.flux_names.append(spec["mon"].name)
-        d.n_flux = len(t["flux"])
-        d.flux = flux
+        d, keep, self.t, self.dft_slots, self.flux_names = build_desc(sim, self.npdt, exact, flags)
 
         h = C.c_void_p()
         N.check(self.lib, self.lib.pf_create(C.byref(d), C.byref(h)), "pf_create")
