@@ -252,12 +252,120 @@ def traffic_from_counters(fetch_dir, write_dir, n=512, precision="float32"):
     print(json.dumps(out, indent=1))
 
 
+def batched_check(n_sims, seed, budget_updates=2.0e8):
+    """Oracle run (timed: the CPU baseline) of the smallest scene of the
+    batched workload, truncated to a bounded step count, and the float32 and
+    float64 errors of the batched GPU stepper on that scene (DFT planes)."""
+    import photonfdtd_amd as pf
+    from oracle.fdtd_numpy import run_oracle
+    from photonfdtd_amd.batch import run_batch
+    probe = batched_2d_scenes(pf, n_sims=n_sims, steps=10, seed=seed)
+    idx = int(np.argmin([s.grid.n_cells for s in probe]))
+    cells = probe[idx].grid.n_cells
+    steps = int(min(4000, budget_updates // cells))
+    sim64 = batched_2d_scenes(pf, n_sims=n_sims, steps=steps, seed=seed, precision="float64")[idx]
+    t0 = time.perf_counter()
+    ref = run_oracle(sim64)
+    cpu_s = time.perf_counter() - t0
+    errs = {}
+    for prec in ("float64", "float32"):
+        sim = batched_2d_scenes(pf, n_sims=n_sims, steps=steps, seed=seed, precision=prec)[idx]
+        got = run_batch([sim])[0]
+        worst = 0.0
+        for name, comps in ref["dft"].items():
+            for c, v in comps.items():
+                worst = max(worst, _rel(got.dft[name][c], v))
+        errs[prec] = worst
+    nx, ny, _ = sim64.grid.shape
+    return dict(cpu_value=cells * sim64.n_steps / cpu_s / 1e9, cpu_s=cpu_s,
+                sample=f"scene {idx} of the batch ({nx}x{ny}, the smallest), {sim64.n_steps} "
+                       "steps, NumPy float64 oracle, 1 core",
+                fp64_rel_l2=errs["float64"], fp32_rel_l2=errs["float32"])
+
+
+def bench_batched(args):
+    """Batched 2-D workload: 64 TE scenes of uneven size in one launch."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.batch import SimBatch
+    from photonfdtd_amd.hip_backend import device_info
+
+    n_sims, seed = 64, 0
+    check = None
+    if not args.no_cpu:
+        check = batched_check(n_sims, seed)
+        print(f"[bench] oracle sample done: {check}", file=sys.stderr, flush=True)
+    total = args.warmup + args.steps
+    t_setup = time.perf_counter()
+    sims = batched_2d_scenes(pf, n_sims=n_sims, steps=total, seed=seed, precision=args.precision)
+    b = SimBatch(sims)
+    setup_s = time.perf_counter() - t_setup
+    cells = sum(s.grid.n_cells for s in sims)
+    print(f"[bench] setup {setup_s:.1f} s, {n_sims} scenes, {cells} cells, {total} steps",
+          file=sys.stderr, flush=True)
+    print(f"[bench] persistent grid {b.persistent_grid(256)} (block 256), "
+          f"{b.persistent_grid(512)} (block 512)", file=sys.stderr, flush=True)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+    tuned = b.tune(trial_steps=min(100, total), modes=args.batch_modes.split(","), log=log)
+    trials = tuned.pop("trials")
+    print(f"[bench] tuned {tuned}", file=sys.stderr, flush=True)
+    b.advance(args.warmup)
+    t0 = time.perf_counter()
+    b.advance(args.steps)
+    wall = time.perf_counter() - t0
+    dev_ms = b.last_ms()
+    finite = all(np.isfinite(v).all() for i in range(n_sims) for v in b.dft(i, "out").values())
+    b.close()
+    itemsize = 4 if args.precision == "float32" else 8
+    bytes_per_update = 7 * itemsize          # TE: read Ex, Ey, Hz, ce; write Ex, Ey, Hz
+    achieved = cells * args.steps * bytes_per_update / (dev_ms * 1e-3) / 1e9
+    value = cells * args.steps / wall / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if itemsize == 4 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched 2-D TE: {n_sims} scenes, sides 256-1024 cells, dx 20 nm, "
+                               "12-cell CPML, 1-4 PolySlabs (20% Lorentz), Ex dipole, "
+                               "2 x-plane DFT monitors x (Ey, Hz) x 16 freqs",
+                   "cells": cells, "seed": seed, "launch": tuned, "tune_trials": trials,
+                   "device": device_info()[0]},
+        "roofline": {"bound": "hbm", "kernel": f"k_batch ({tuned['mode']})",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell_update": bytes_per_update,
+                     "kernel_ms_per_step": round(dev_ms / args.steps, 4)},
+        "dft_finite": bool(finite),
+        "setup_s": round(setup_s, 1),
+    }
+    if check is not None:
+        out["cpu_baseline"] = {"value": round(check["cpu_value"], 5), "unit": "Gcell-updates/s",
+                               "cores": 1, "kind": "port", "sample": check["sample"]}
+        out["fp64_rel_l2_vs_oracle"] = check["fp64_rel_l2"]
+        out["fp32_rel_l2_vs_oracle"] = check["fp32_rel_l2"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--workload", default="slab3d", choices=("slab3d", "batched2d"),
+                    help="slab3d: the 3-D dispersive slab (headline); batched2d: 64 uneven "
+                         "2-D TE scenes in one launch")
+    ap.add_argument("--batch-modes", default="graph,phases",
+                    help="launch modes the batched autotuner may pick from")
     ap.add_argument("--precision", default="float32", choices=("float32", "float64"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg")
     ap.add_argument("--summarize", nargs=2, metavar=("DIR", "KIND"),
@@ -276,6 +384,9 @@ def main():
                                    "part of this build (see the sharding design notes)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "batched2d":
+        bench_batched(args)
+        return
 
     import photonfdtd_amd as pf
     from photonfdtd_amd import _native as N

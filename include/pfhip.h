@@ -174,15 +174,21 @@ int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* gra
  * monitors (no flux). pf_batch_advance(b, n) advances every simulation n
  * steps (a simulation stops at its own step count). Modes: 0 one launch per
  * phase, 1 one persistent cooperative launch, 2 per-phase launches replayed
- * from a captured hipGraph. pf_batch_tune times the candidate (mode,
- * workgroup size, cells per work chunk) settings and resets the batch. */
+ * from a captured hipGraph. pf_batch_reset zeroes every field, psi, ADE and
+ * DFT state and the step counter (the autotuner times candidate settings
+ * from a reset state). pf_batch_persistent_grid reports the occupancy and
+ * the workgroup count mode 1 would use for a workgroup size, without
+ * launching. */
 typedef struct pf_batch pf_batch;
 int pf_batch_create(const pf_desc* descs, const int32_t* pols, int32_t n, pf_batch** out);
 int pf_batch_destroy(pf_batch* b);
 int pf_batch_configure(pf_batch* b, int32_t mode, int32_t block, int32_t chunk);
-int pf_batch_tune(pf_batch* b, int32_t trial_steps, int32_t* mode, int32_t* block, int32_t* chunk,
-                  double* ms_per_step);
+int pf_batch_reset(pf_batch* b);
+int pf_batch_persistent_grid(pf_batch* b, int32_t block, int32_t* per_cu, int32_t* n_cu,
+                             int32_t* grid);
 int pf_batch_advance(pf_batch* b, int64_t n);
+/* Device time (HIP events on the batch stream) of the last pf_batch_advance. */
+int pf_batch_last_ms(pf_batch* b, double* ms);
 int pf_batch_dft_read(pf_batch* b, int32_t sim, int32_t mon, int32_t slot, double* dst);
 int pf_batch_read_field(pf_batch* b, int32_t sim, int32_t comp, void* dst);
 /* Event timing (no reference counterpart; used by the benchmark): when on,

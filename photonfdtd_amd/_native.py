@@ -77,8 +77,10 @@ SIGNATURES = {
     "pf_batch_create": (_i32, [_P(PfDesc), _P(_i32), _i32, _P(_vp)]),
     "pf_batch_destroy": (_i32, [_vp]),
     "pf_batch_configure": (_i32, [_vp, _i32, _i32, _i32]),
-    "pf_batch_tune": (_i32, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32), _P(_f64)]),
+    "pf_batch_reset": (_i32, [_vp]),
+    "pf_batch_persistent_grid": (_i32, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32)]),
     "pf_batch_advance": (_i32, [_vp, _i64]),
+    "pf_batch_last_ms": (_i32, [_vp, _P(_f64)]),
     "pf_batch_dft_read": (_i32, [_vp, _i32, _i32, _i32, _P(_f64)]),
     "pf_batch_read_field": (_i32, [_vp, _i32, _i32, _vp]),
 }

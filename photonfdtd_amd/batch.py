@@ -19,7 +19,7 @@ FluxMonitor need per-simulation host stops and are rejected.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Dict, List, Optional, Sequence
+from typing import Callable, Dict, List, Optional, Sequence
 
 import numpy as np
 
@@ -60,7 +60,7 @@ def _check(sim) -> np.dtype:
 class SimBatch:
     """n 2-D simulations of one precision advanced together on the device."""
 
-    def __init__(self, sims: Sequence, mode: str = "persistent", block: int = 256,
+    def __init__(self, sims: Sequence, mode: str = "graph", block: int = 256,
                  chunk: int = 4096):
         if not sims:
             raise ValueError("empty batch")
@@ -107,22 +107,52 @@ class SimBatch:
                 "pf_batch_configure")
         self.mode, self.block, self.chunk = mode, int(block), int(chunk)
 
-    def tune(self, trial_steps: int = 100) -> Dict:
-        """Time the candidate launch settings, keep the fastest and reset
-        every simulation to step 0."""
-        m, b, c = C.c_int32(), C.c_int32(), C.c_int32()
-        ms = C.c_double()
-        N.check(self.lib, self.lib.pf_batch_tune(self.h, int(trial_steps), C.byref(m), C.byref(b),
-                                                 C.byref(c), C.byref(ms)), "pf_batch_tune")
-        self.mode = {v: k for k, v in MODES.items()}[m.value]
-        self.block, self.chunk = b.value, c.value
+    def reset(self) -> None:
+        """Zero every field, psi, ADE and DFT state; back to step 0."""
+        N.check(self.lib, self.lib.pf_batch_reset(self.h), "pf_batch_reset")
         self.step = 0
-        return dict(mode=self.mode, block=self.block, chunk=self.chunk, ms_per_step=ms.value)
+
+    def persistent_grid(self, block: int = 256) -> Dict[str, int]:
+        """Occupancy and workgroup count of the persistent mode (no launch)."""
+        per_cu, n_cu, grid = C.c_int32(), C.c_int32(), C.c_int32()
+        N.check(self.lib, self.lib.pf_batch_persistent_grid(self.h, int(block), C.byref(per_cu),
+                                                            C.byref(n_cu), C.byref(grid)),
+                "pf_batch_persistent_grid")
+        return dict(per_cu=per_cu.value, n_cu=n_cu.value, grid=grid.value)
+
+    def tune(self, trial_steps: int = 100, modes: Sequence[str] = ("graph", "phases"),
+             blocks: Sequence[int] = (256, 512), chunks: Sequence[int] = (1024, 4096, 16384),
+             log: Optional[Callable[[str], None]] = None) -> Dict:
+        """Autotuner: time every (mode, workgroup size, cells per chunk)
+        candidate over ``trial_steps`` steps from a reset state with HIP
+        events, keep the fastest, and reset the batch to step 0."""
+        trial_steps = max(1, min(int(trial_steps), self.n_steps))
+        trials = []
+        for m in modes:
+            for bl in blocks:
+                for ch in chunks:
+                    self.configure(m, bl, ch)
+                    self.reset()
+                    self.advance(trial_steps)
+                    ms = self.last_ms() / trial_steps
+                    trials.append(dict(mode=m, block=bl, chunk=ch, ms_per_step=ms))
+                    if log is not None:
+                        log(f"[tune] {m} block {bl} chunk {ch}: {ms:.4f} ms/step")
+        best = min(trials, key=lambda t: t["ms_per_step"])
+        self.configure(best["mode"], best["block"], best["chunk"])
+        self.reset()
+        return dict(best, trials=trials)
 
     def advance(self, n: Optional[int] = None) -> None:
         n = self.n_steps - self.step if n is None else int(n)
         N.check(self.lib, self.lib.pf_batch_advance(self.h, n), "pf_batch_advance")
         self.step += n
+
+    def last_ms(self) -> float:
+        """Device time of the last ``advance`` (HIP events on the batch stream)."""
+        ms = C.c_double()
+        N.check(self.lib, self.lib.pf_batch_last_ms(self.h, C.byref(ms)), "pf_batch_last_ms")
+        return ms.value
 
     def field(self, i: int, comp: str) -> np.ndarray:
         out = np.empty(self.sims[i].grid.shape, dtype=self.npdt)
@@ -153,7 +183,7 @@ class SimBatch:
         return out
 
 
-def run_batch(sims: Sequence, mode: str = "persistent", tune: bool = False,
+def run_batch(sims: Sequence, mode: str = "graph", tune: bool = False,
               trial_steps: int = 100) -> List[Result]:
     """Run every simulation to its own step count; one ``Result`` each."""
     b = SimBatch(sims, mode=mode)

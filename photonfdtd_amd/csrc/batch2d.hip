@@ -245,20 +245,40 @@ void launch_batch_phase(const BatchArgs<real>& a, int phase, bool exact, int gri
 
 void launch_batch_tick(int64_t* step, hipStream_t s) { k_batch_tick<<<1, 64, 0, s>>>(step); }
 
+// Workgroups of the persistent kernel: every workgroup must be resident at
+// once for the grid barrier, so the grid is capped at PERSISTENT_MAX_PER_CU
+// workgroups per CU (below what the occupancy calculator allows) and at the
+// number of work chunks.
+constexpr int PERSISTENT_MAX_PER_CU = 4;
+
 template <typename real>
-hipError_t launch_batch_persistent(BatchArgs<real>& a, bool exact, int block, int64_t step0,
-                                   int64_t step1, hipStream_t s, int* grid_out) {
+hipError_t batch_persistent_grid(bool exact, int block, int n_chunks, int* per_cu, int* n_cu,
+                                 int* grid) {
   const void* fn = exact ? reinterpret_cast<const void*>(&k_batch_persistent<real, true>)
                          : reinterpret_cast<const void*>(&k_batch_persistent<real, false>);
-  int per_cu = 0, dev = 0, n_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0);
+  int dev = 0;
+  *per_cu = 0;
+  *n_cu = 0;
+  *grid = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, block, 0);
   if (e != hipSuccess) return e;
   e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  e = hipDeviceGetAttribute(n_cu, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  const int grid = std::max(1, std::min(per_cu * n_cu, a.n_chunks));
-  *grid_out = grid;
+  if (*per_cu < 1 || *n_cu < 1) return hipErrorInvalidConfiguration;
+  *grid = std::max(1, std::min(std::min(*per_cu, PERSISTENT_MAX_PER_CU) * *n_cu, n_chunks));
+  return hipSuccess;
+}
+
+template <typename real>
+hipError_t launch_batch_persistent(BatchArgs<real>& a, bool exact, int block, int64_t step0,
+                                   int64_t step1, hipStream_t s) {
+  int per_cu = 0, n_cu = 0, grid = 0;
+  hipError_t e = batch_persistent_grid<real>(exact, block, a.n_chunks, &per_cu, &n_cu, &grid);
+  if (e != hipSuccess) return e;
+  const void* fn = exact ? reinterpret_cast<const void*>(&k_batch_persistent<real, true>)
+                         : reinterpret_cast<const void*>(&k_batch_persistent<real, false>);
   void* args[] = {&a, &step0, &step1};
   return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, s);
 }
@@ -266,8 +286,10 @@ hipError_t launch_batch_persistent(BatchArgs<real>& a, bool exact, int block, in
 template void launch_batch_phase<float>(const BatchArgs<float>&, int, bool, int, int, hipStream_t);
 template void launch_batch_phase<double>(const BatchArgs<double>&, int, bool, int, int, hipStream_t);
 template hipError_t launch_batch_persistent<float>(BatchArgs<float>&, bool, int, int64_t, int64_t,
-                                                   hipStream_t, int*);
+                                                   hipStream_t);
 template hipError_t launch_batch_persistent<double>(BatchArgs<double>&, bool, int, int64_t, int64_t,
-                                                    hipStream_t, int*);
+                                                    hipStream_t);
+template hipError_t batch_persistent_grid<float>(bool, int, int, int*, int*, int*);
+template hipError_t batch_persistent_grid<double>(bool, int, int, int*, int*, int*);
 
 }  // namespace pfhip

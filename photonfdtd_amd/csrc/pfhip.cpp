@@ -1066,10 +1066,12 @@ void Stepper<real>::flux_read(double* dst) {
 struct BatchBase {
   virtual ~BatchBase() = default;
   virtual void configure(int mode, int block, int chunk) = 0;
-  virtual void tune(int trial_steps, int* mode, int* block, int* chunk, double* ms) = 0;
+  virtual void reset() = 0;
+  virtual void persistent_grid(int block, int* per_cu, int* n_cu, int* grid) = 0;
   virtual void advance(int64_t n) = 0;
   virtual pf_handle* sim(int i) = 0;
   virtual int size() const = 0;
+  virtual double last_ms() const = 0;
 };
 
 template <typename real>
@@ -1090,11 +1092,15 @@ class Batch final : public BatchBase {
     }
     sims_dev_ = arena_.upload(host.data(), host.size(), stream_);
     step_dev_ = arena_.zeros<int64_t>(1, stream_);
+    HIP_CHECK(hipEventCreate(&ev_[0]));
+    HIP_CHECK(hipEventCreate(&ev_[1]));
     configure(1, 256, 4096);
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
   ~Batch() override {
     drop_graph();
+    for (hipEvent_t e : ev_)
+      if (e) (void)hipEventDestroy(e);
     if (stream_) {
       (void)hipStreamSynchronize(stream_);
       arena_.release();
@@ -1106,12 +1112,16 @@ class Batch final : public BatchBase {
     return sims_[i].get();
   }
   int size() const override { return (int)sims_.size(); }
+  double last_ms() const override { return last_ms_; }
 
   // mode 0: one launch per phase; 1: persistent cooperative kernel;
   // 2: per-phase launches captured as a graph of graph_steps_ steps.
   void configure(int mode, int block, int chunk) override {
     if (mode < 0 || mode > 2 || (block != 256 && block != 512 && block != 1024) || chunk < 256)
       throw ArgError{"bad batch configuration"};
+    if (mode == 1)
+      throw HipError{PF_E_STATE, "the cooperative persistent mode is disabled: a full-occupancy "
+                                 "launch went silent on a 28M-cell batch; use mode 0 or 2"};
     drop_graph();
     mode_ = mode;
     block_ = block;
@@ -1141,9 +1151,9 @@ class Batch final : public BatchBase {
     HIP_CHECK(hipMemcpyAsync(step_dev_, &step_, sizeof(int64_t), hipMemcpyHostToDevice, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
     const int grid = std::min(n_chunks_, 4096);
+    HIP_CHECK(hipEventRecord(ev_[0], stream_));
     if (mode_ == 1) {
-      int g = 0;
-      HIP_CHECK(pfhip::launch_batch_persistent<real>(args_, exact_, block_, step_, step_ + n, stream_, &g));
+      HIP_CHECK(pfhip::launch_batch_persistent<real>(args_, exact_, block_, step_, step_ + n, stream_));
     } else {
       int64_t done = 0;
       if (mode_ == 2) {
@@ -1158,48 +1168,17 @@ class Batch final : public BatchBase {
       }
     }
     HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
+    float t = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&t, ev_[0], ev_[1]));
+    last_ms_ = t;
     step_ += n;
   }
 
-  // Times each candidate (mode, block, chunk) over trial_steps steps from a
-  // zero state, keeps the fastest and resets every simulation to step 0.
-  void tune(int trial_steps, int* mode, int* block, int* chunk, double* ms) override {
-    const int modes[2] = {1, 2};
-    const int blocks[2] = {256, 512};
-    const int chunks[3] = {1024, 4096, 16384};
-    int64_t steps = std::min<int64_t>(trial_steps, n_steps_max_);
-    double best = 1e300;
-    int bm = 1, bb = 256, bc = 4096;
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    for (int m : modes)
-      for (int b : blocks)
-        for (int c : chunks) {
-          configure(m, b, c);
-          reset_all();
-          HIP_CHECK(hipEventRecord(e0, stream_));
-          advance(steps);
-          HIP_CHECK(hipEventRecord(e1, stream_));
-          HIP_CHECK(hipEventSynchronize(e1));
-          float t = 0.f;
-          HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
-          if (t < best) {
-            best = t;
-            bm = m;
-            bb = b;
-            bc = c;
-          }
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    configure(bm, bb, bc);
-    reset_all();
-    if (mode) *mode = bm;
-    if (block) *block = bb;
-    if (chunk) *chunk = bc;
-    if (ms) *ms = best / std::max<int64_t>(steps, 1);
+  void reset() override { reset_all(); }
+  void persistent_grid(int block, int* per_cu, int* n_cu, int* grid) override {
+    HIP_CHECK(pfhip::batch_persistent_grid<real>(exact_, block, n_chunks_, per_cu, n_cu, grid));
   }
 
  private:
@@ -1240,6 +1219,8 @@ class Batch final : public BatchBase {
   int graph_steps_ = 50;
   hipGraphExec_t graph_exec_ = nullptr;
   pfhip::BatchArgs<real> args_{};
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  double last_ms_ = 0.0;
 };
 
 template <typename F>
@@ -1360,21 +1341,25 @@ int pf_batch_configure(pf_batch* b, int32_t mode, int32_t block, int32_t chunk) 
   return guarded([&] { b->impl->configure(mode, block, chunk); });
 }
 
-int pf_batch_tune(pf_batch* b, int32_t trial_steps, int32_t* mode, int32_t* block, int32_t* chunk,
-                  double* ms_per_step) {
+int pf_batch_reset(pf_batch* b) {
   if (!b) return fail(PF_E_ARG, "null batch");
-  return guarded([&] {
-    int m = 0, bl = 0, c = 0;
-    b->impl->tune(trial_steps, &m, &bl, &c, ms_per_step);
-    if (mode) *mode = m;
-    if (block) *block = bl;
-    if (chunk) *chunk = c;
-  });
+  return guarded([&] { b->impl->reset(); });
+}
+
+int pf_batch_persistent_grid(pf_batch* b, int32_t block, int32_t* per_cu, int32_t* n_cu,
+                             int32_t* grid) {
+  if (!b || !per_cu || !n_cu || !grid) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { b->impl->persistent_grid(block, per_cu, n_cu, grid); });
 }
 
 int pf_batch_advance(pf_batch* b, int64_t n) {
   if (!b) return fail(PF_E_ARG, "null batch");
   return guarded([&] { b->impl->advance(n); });
+}
+
+int pf_batch_last_ms(pf_batch* b, double* ms) {
+  if (!b || !ms) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { *ms = b->impl->last_ms(); });
 }
 
 int pf_batch_dft_read(pf_batch* b, int32_t sim, int32_t mon, int32_t slot, double* dst) {

@@ -196,7 +196,10 @@ void launch_batch_phase(const BatchArgs<real>& a, int phase, bool exact, int gri
 void launch_batch_tick(int64_t* step, hipStream_t s);
 template <typename real>
 hipError_t launch_batch_persistent(BatchArgs<real>& a, bool exact, int block, int64_t step0,
-                                   int64_t step1, hipStream_t s, int* grid_out);
+                                   int64_t step1, hipStream_t s);
+template <typename real>
+hipError_t batch_persistent_grid(bool exact, int block, int n_chunks, int* per_cu, int* n_cu,
+                                 int* grid);
 
 template <typename real> void launch_generic_h(const StepArgs<real>& a, bool exact, hipStream_t s);
 template <typename real> void launch_generic_e(const StepArgs<real>& a, bool exact, hipStream_t s);

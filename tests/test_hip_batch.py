@@ -50,8 +50,13 @@ def _has_ade(sim):
     return any(type(s.medium).__name__ == "DispersiveMedium" for s in sim.structures)
 
 
+PERSISTENT_OFF = pytest.mark.skip(reason="cooperative persistent mode disabled: a full-occupancy "
+                                          "launch went silent on a large batch")
+
+
 @pytest.mark.parametrize("precision", ["float32", "float64"])
-@pytest.mark.parametrize("mode", ["phases", "persistent", "graph"])
+@pytest.mark.parametrize("mode", ["phases", pytest.param("persistent", marks=PERSISTENT_OFF),
+                                  "graph"])
 def test_te_batch_matches_single_runs(precision, mode):
     from photonfdtd_amd.batch import SimBatch
     sims = _scenes(precision, "Ex", n=4, seed=1)
@@ -68,7 +73,7 @@ def test_te_batch_matches_single_runs(precision, mode):
         b.close()
 
 
-@pytest.mark.parametrize("mode", ["persistent", "graph"])
+@pytest.mark.parametrize("mode", [pytest.param("persistent", marks=PERSISTENT_OFF), "graph"])
 def test_tm_batch_matches_single_runs(mode):
     from photonfdtd_amd.batch import SimBatch
     sims = _scenes("float32", "Ez", n=4, seed=5)
@@ -88,7 +93,7 @@ def test_mixed_polarisations_in_one_batch():
     from photonfdtd_amd.batch import SimBatch
     sims = _scenes("float32", "Ex", n=2, seed=7) + _scenes("float32", "Ez", n=2, seed=8)
     print("\n[case] mixed TE/TM batch", flush=True)
-    b = SimBatch(sims, mode="persistent")
+    b = SimBatch(sims, mode="phases")
     try:
         b.advance()
         for i, s in enumerate(sims):
@@ -103,9 +108,9 @@ def test_tune_resets_state():
     print("\n[case] tune then run", flush=True)
     b = SimBatch(sims)
     try:
-        best = b.tune(trial_steps=40)
-        print(f"[tune] {best}", flush=True)
-        assert best["mode"] in ("persistent", "graph") and best["ms_per_step"] > 0
+        best = b.tune(trial_steps=60, log=lambda m: print(m, flush=True))
+        assert best["mode"] in ("graph", "phases") and best["ms_per_step"] > 0
+        assert len(best["trials"]) == 12
         b.advance()
         for i, s in enumerate(sims):
             _assert_same(b, i, s, "tuned")
