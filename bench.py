@@ -302,8 +302,8 @@ def bench_batched(args):
     cells = sum(s.grid.n_cells for s in sims)
     print(f"[bench] setup {setup_s:.1f} s, {n_sims} scenes, {cells} cells, {total} steps",
           file=sys.stderr, flush=True)
-    print(f"[bench] persistent grid {b.persistent_grid(256)} (block 256), "
-          f"{b.persistent_grid(512)} (block 512)", file=sys.stderr, flush=True)
+    print(f"[bench] persistent grid {b.persistent_grid(4)} (4 waves per workgroup)",
+          file=sys.stderr, flush=True)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
@@ -339,7 +339,7 @@ def bench_batched(args):
                                "2 x-plane DFT monitors x (Ey, Hz) x 16 freqs",
                    "cells": cells, "seed": seed, "launch": tuned, "tune_trials": trials,
                    "device": device_info()[0]},
-        "roofline": {"bound": "hbm", "kernel": f"k_batch ({tuned['mode']})",
+        "roofline": {"bound": "hbm", "kernel": f"k_batch_march ({tuned['mode']})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_cell_update": bytes_per_update,
@@ -364,7 +364,7 @@ def main():
     ap.add_argument("--workload", default="slab3d", choices=("slab3d", "batched2d"),
                     help="slab3d: the 3-D dispersive slab (headline); batched2d: 64 uneven "
                          "2-D TE scenes in one launch")
-    ap.add_argument("--batch-modes", default="graph,phases",
+    ap.add_argument("--batch-modes", default="graph,stream,persistent",
                     help="launch modes the batched autotuner may pick from")
     ap.add_argument("--precision", default="float32", choices=("float32", "float64"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg")

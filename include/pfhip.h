@@ -172,20 +172,23 @@ int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* gra
  * pols[i] = 0 (TE: Ex, Ey, Hz) or 1 (TM: Ez, Hx, Hy) the group its sources
  * excite. Supported per simulation: CPML, soft sources, ADE media and DFT
  * monitors (no flux). pf_batch_advance(b, n) advances every simulation n
- * steps (a simulation stops at its own step count). Modes: 0 one launch per
- * phase, 1 one persistent cooperative launch, 2 per-phase launches replayed
- * from a captured hipGraph. pf_batch_reset zeroes every field, psi, ADE and
- * DFT state and the step counter (the autotuner times candidate settings
- * from a reset state). pf_batch_persistent_grid reports the occupancy and
- * the workgroup count mode 1 would use for a workgroup size, without
+ * steps (a simulation stops at its own step count). One march kernel does
+ * a whole time step; each wavefront owns a 62-column strip over `rows` rows.
+ * Modes: 0 one march launch per step, 1 one persistent launch for the whole
+ * advance (grid barrier with a bounded wait: PF_E_STATE if the workgroups
+ * were not all resident), 2 march launches replayed from a captured
+ * hipGraph. waves_per_block is 1, 2, 4 or 8. pf_batch_reset zeroes every
+ * field, psi, ADE and DFT state and the step counter (the autotuner times
+ * candidate settings from a reset state). pf_batch_persistent_grid reports
+ * the occupancy and the workgroup count mode 1 would use, without
  * launching. */
 typedef struct pf_batch pf_batch;
 int pf_batch_create(const pf_desc* descs, const int32_t* pols, int32_t n, pf_batch** out);
 int pf_batch_destroy(pf_batch* b);
-int pf_batch_configure(pf_batch* b, int32_t mode, int32_t block, int32_t chunk);
+int pf_batch_configure(pf_batch* b, int32_t mode, int32_t waves_per_block, int32_t rows);
 int pf_batch_reset(pf_batch* b);
-int pf_batch_persistent_grid(pf_batch* b, int32_t block, int32_t* per_cu, int32_t* n_cu,
-                             int32_t* grid);
+int pf_batch_persistent_grid(pf_batch* b, int32_t waves_per_block, int32_t* per_cu,
+                             int32_t* n_cu, int32_t* grid);
 int pf_batch_advance(pf_batch* b, int64_t n);
 /* Device time (HIP events on the batch stream) of the last pf_batch_advance. */
 int pf_batch_last_ms(pf_batch* b, double* ms);

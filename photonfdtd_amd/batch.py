@@ -3,9 +3,10 @@
 The reference runs one simulation per ``Simulation.run()``
 (``simulation.py:896-904``); a batch of design candidates is a Python loop of
 runs. ``SimBatch`` holds n simulations of one precision on the device and
-advances them together: the per-phase kernels (or one persistent cooperative
-kernel) walk a list of fixed-size work chunks that spans every grid, so
-uneven sizes share the launch without padding. Each simulation keeps the
+advances them together: one march kernel per time step (launched per step,
+replayed from a captured hipGraph, or looped inside one persistent launch)
+walks a list of wavefront chunks (a 62-column strip over a run of rows) that
+spans every grid, so uneven sizes share the launch without padding. Each simulation keeps the
 single-run semantics of ``backend="hip"``: the same tables
 (``hip_backend.build_desc``) and the same per-node arithmetic, so a batched
 simulation reproduces its single run bit for bit.
@@ -30,7 +31,7 @@ from .simulation import Result
 
 TE_COMPONENTS = ("Ex", "Ey", "Hz")
 TM_COMPONENTS = ("Ez", "Hx", "Hy")
-MODES = {"phases": 0, "persistent": 1, "graph": 2}
+MODES = {"stream": 0, "persistent": 1, "graph": 2}
 
 
 def polarisation(sim) -> int:
@@ -60,8 +61,7 @@ def _check(sim) -> np.dtype:
 class SimBatch:
     """n 2-D simulations of one precision advanced together on the device."""
 
-    def __init__(self, sims: Sequence, mode: str = "graph", block: int = 256,
-                 chunk: int = 4096):
+    def __init__(self, sims: Sequence, mode: str = "graph", waves: int = 4, rows: int = 64):
         if not sims:
             raise ValueError("empty batch")
         dts = {_check(s) for s in sims}
@@ -87,7 +87,7 @@ class SimBatch:
         self.step = 0
         self.n_steps = max(s.n_steps for s in sims)
         del keep
-        self.configure(mode, block, chunk)
+        self.configure(mode, waves, rows)
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -100,46 +100,49 @@ class SimBatch:
         except Exception:
             pass
 
-    def configure(self, mode: str, block: int = 256, chunk: int = 4096) -> None:
+    def configure(self, mode: str, waves: int = 4, rows: int = 64) -> None:
+        """Launch mode, wavefronts per workgroup (1, 2, 4, 8) and owned rows
+        per wavefront chunk."""
         if mode not in MODES:
             raise ValueError(f"mode must be one of {sorted(MODES)}")
-        N.check(self.lib, self.lib.pf_batch_configure(self.h, MODES[mode], int(block), int(chunk)),
+        N.check(self.lib, self.lib.pf_batch_configure(self.h, MODES[mode], int(waves), int(rows)),
                 "pf_batch_configure")
-        self.mode, self.block, self.chunk = mode, int(block), int(chunk)
+        self.mode, self.waves, self.rows = mode, int(waves), int(rows)
 
     def reset(self) -> None:
         """Zero every field, psi, ADE and DFT state; back to step 0."""
         N.check(self.lib, self.lib.pf_batch_reset(self.h), "pf_batch_reset")
         self.step = 0
 
-    def persistent_grid(self, block: int = 256) -> Dict[str, int]:
+    def persistent_grid(self, waves: int = 4) -> Dict[str, int]:
         """Occupancy and workgroup count of the persistent mode (no launch)."""
         per_cu, n_cu, grid = C.c_int32(), C.c_int32(), C.c_int32()
-        N.check(self.lib, self.lib.pf_batch_persistent_grid(self.h, int(block), C.byref(per_cu),
+        N.check(self.lib, self.lib.pf_batch_persistent_grid(self.h, int(waves), C.byref(per_cu),
                                                             C.byref(n_cu), C.byref(grid)),
                 "pf_batch_persistent_grid")
         return dict(per_cu=per_cu.value, n_cu=n_cu.value, grid=grid.value)
 
-    def tune(self, trial_steps: int = 100, modes: Sequence[str] = ("graph", "phases"),
-             blocks: Sequence[int] = (256, 512), chunks: Sequence[int] = (1024, 4096, 16384),
+    def tune(self, trial_steps: int = 100,
+             modes: Sequence[str] = ("graph", "stream", "persistent"),
+             waves: Sequence[int] = (2, 4, 8), rows: Sequence[int] = (16, 32, 64, 128),
              log: Optional[Callable[[str], None]] = None) -> Dict:
-        """Autotuner: time every (mode, workgroup size, cells per chunk)
-        candidate over ``trial_steps`` steps from a reset state with HIP
-        events, keep the fastest, and reset the batch to step 0."""
+        """Autotuner: time every (mode, wavefronts per workgroup, rows per
+        chunk) candidate over ``trial_steps`` steps from a reset state with
+        HIP events, keep the fastest, and reset the batch to step 0."""
         trial_steps = max(1, min(int(trial_steps), self.n_steps))
         trials = []
         for m in modes:
-            for bl in blocks:
-                for ch in chunks:
-                    self.configure(m, bl, ch)
+            for w in waves:
+                for r in rows:
+                    self.configure(m, w, r)
                     self.reset()
                     self.advance(trial_steps)
                     ms = self.last_ms() / trial_steps
-                    trials.append(dict(mode=m, block=bl, chunk=ch, ms_per_step=ms))
+                    trials.append(dict(mode=m, waves=w, rows=r, ms_per_step=ms))
                     if log is not None:
-                        log(f"[tune] {m} block {bl} chunk {ch}: {ms:.4f} ms/step")
+                        log(f"[tune] {m} waves {w} rows {r}: {ms:.4f} ms/step")
         best = min(trials, key=lambda t: t["ms_per_step"])
-        self.configure(best["mode"], best["block"], best["chunk"])
+        self.configure(best["mode"], best["waves"], best["rows"])
         self.reset()
         return dict(best, trials=trials)
 

@@ -1,17 +1,27 @@
 // Batched 2-D stepping: many independent xy-plane simulations of different
-// sizes advanced together by one kernel per phase, or by one persistent
-// cooperative kernel that loops over all time steps with a grid barrier
-// between the H and E phases.
+// sizes advanced by one march kernel per time step, or by one persistent
+// kernel that loops over the steps with a grid barrier between them.
 //
-// Only the active polarisation group is stepped (TE: Ex, Ey, Hz; TM: Ez, Hx,
-// Hy): in 2-D the two groups decouple, so with sources in one group the other
-// stays exactly zero, as in the reference's six-component update. The owning
-// thread of a node also applies the node's soft sources, its ADE pole update
-// (E phase) and its DFT accumulation, in the reference's order (curl update,
-// ADE, sources, then the DFT sample; simulation.py:1319-1406), with the same
-// per-node arithmetic as the generic kernels.
-#include <hip/hip_cooperative_groups.h>
-
+// Each wavefront owns a strip of 62 columns (lanes 1..62 of a 64-lane
+// window) over a run of rows and marches down the rows. At row i it computes
+// H^{n+1/2}(i) from E^n(i) (registers) and E^n(i+1) (loaded one row ahead),
+// then E^{n+1}(i) from H^{n+1/2}(i) and H^{n+1/2}(i-1) (registers). The
+// neighbour columns come from the adjacent lanes (lane 63 supplies E(j+1),
+// lane 0 recomputes H(j-1)); the first row recomputes H(i0-1). Fields and
+// H-side psi are double-buffered (read level p = step & 1, write p ^ 1), so
+// the recomputed halo reads only old values; every node is written by one
+// lane. Per node and step this reads E (2 components), H (1) and ce once and
+// writes E and H once: 28 bytes in float32, the Yee minimum.
+//
+// Only the polarisation group the sources excite is stepped (TE: Ex, Ey, Hz;
+// TM: Ez, Hx, Hy): in 2-D the two groups decouple and the other one stays
+// exactly zero, as in the reference's six-component update. The march does
+// the curl updates with the CPML slab terms and the H-side soft sources; a
+// post pass over the few nodes that need more then applies, per node and in
+// the reference's order (simulation.py:1319-1406), the ADE correction, the
+// E-side soft sources and the DFT sample. Every node does the arithmetic of
+// the generic kernels, so a batched simulation reproduces its single run bit
+// for bit.
 #include "pfhip_internal.h"
 #include "yee_common.h"
 
@@ -19,68 +29,37 @@ namespace pfhip {
 
 namespace {
 
-template <typename real>
-__device__ __forceinline__ void psi_term(Term<real>& t, const SimDev<real>& s, bool e_side, int ax,
-                                         int slot, int i, int j) {
-  const int node = ax == 0 ? i : j;
-  const int m = (e_side ? s.map_e[ax] : s.map_h[ax])[node];
-  if (m < 0) {
-    t.psi = nullptr;
-    return;
-  }
-  const int pc = e_side ? s.pe[ax] : s.ph[ax];
-  t.psi = s.psi[slot];
-  t.pid = ax == 0 ? (int64_t)m * s.ny + j : (int64_t)i * pc + m;
-  t.b = (e_side ? s.b_e[ax] : s.b_h[ax])[node];
-  t.c = (e_side ? s.c_e[ax] : s.c_h[ax])[node];
-}
+// CPML membership of an axis: node n is in the slab set iff n < lo or
+// n >= hi; m(n) is its compact psi index.
+struct Slab {
+  int lo, hi;
+  __device__ __forceinline__ bool in(int n) const { return n < lo || n >= hi; }
+  __device__ __forceinline__ int m(int n) const { return n < lo ? n : lo + (n - hi); }
+};
 
-// Soft sources of one side landing on (comp, cell), in source order.
+// Soft sources of one group (one component and cell), in source order
+// (simulation.py:1366-1390): one rounding to the field type per add.
 template <typename real>
-__device__ __forceinline__ real add_sources(const SimDev<real>& s, int side, int comp, int64_t cell,
-                                            int64_t step, real v) {
-  if (cell < s.src_lo[side] || cell > s.src_hi[side]) return v;
-  const SourceGroupsDev& g = side ? s.src_h : s.src_e;
-  const int ng = g.n_groups;
-  for (int q = 0; q < ng; ++q) {
-    if (g.comp[q] != comp || g.cell[q] != cell) continue;
-    for (int t = g.begin[q]; t < g.begin[q + 1]; ++t)
-      v = (real)((double)v + g.amp[t] * g.wave[(int64_t)g.wave_row[t] * s.n_steps + step]);
-  }
+__device__ __forceinline__ real add_group(const SourceGroupsDev& g, int q, int64_t n_steps,
+                                          int64_t step, real v) {
+  for (int t = g.begin[q]; t < g.begin[q + 1]; ++t)
+    v = (real)((double)v + g.amp[t] * g.wave[(int64_t)g.wave_row[t] * n_steps + step]);
   return v;
 }
 
-// DFT sample of one component value at a node, if the node is in a monitor box.
 template <typename real>
-__device__ __forceinline__ void dft_sample(const SimDev<real>& s, int comp, int i, int j,
-                                           int64_t step, real v) {
-  for (int m = 0; m < s.n_dft; ++m) {
-    const DftDev& d = s.dft[m];
-    const int di = i - d.lo[0], dj = j - d.lo[1];
-    if (di < 0 || dj < 0 || i > d.hi[0] || j > d.hi[1]) continue;
-    if (d.interval > 1 && step % d.interval != 0) continue;
-    int slot = -1;
-    for (int c = 0; c < d.n_comp; ++c)
-      if (d.comps[c] == comp) slot = c;
-    if (slot < 0) continue;
-    const int sx = (int)d.box.stride[0], sy = (int)d.box.stride[1];
-    if ((sx > 1 && di % sx) || (sy > 1 && dj % sy)) continue;
-    const int64_t bi = sx > 1 ? di / sx : di, bj = sy > 1 ? dj / sy : dj;
-    const int64_t r = bi * d.box.count[1] + bj;
-    const int64_t nbox = d.box.count[0] * d.box.count[1];
-    const int64_t q = step / d.interval;
-    const double* ph = (comp >= 3 ? d.phase_h : d.phase_e) + 2 * q * d.n_freq;
-    double* A = d.acc + 2 * ((int64_t)slot * d.n_freq * nbox + r);
-    const double x = (double)v;
-    for (int f = 0; f < d.n_freq; ++f) {
-      double* a = A + 2 * (int64_t)f * nbox;
-      a[0] = a[0] + (ph[2 * f] * x) * d.dt;
-      a[1] = a[1] + (ph[2 * f + 1] * x) * d.dt;
-    }
-  }
+__device__ __forceinline__ real add_h_sources(const SimDev<real>& s, int comp, int64_t cell,
+                                              int64_t step, real v) {
+  if (cell < s.src_lo || cell > s.src_hi) return v;
+  const SourceGroupsDev& g = s.src_h;
+  for (int q = 0; q < g.n_groups; ++q)
+    if (g.comp[q] == comp && g.cell[q] == cell) v = add_group(g, q, s.n_steps, step, v);
+  return v;
 }
 
-// ADE for an E component at a node (simulation.py:645-669); en = E^n.
+// ADE of one E component at a node (simulation.py:645-669): en = E^n,
+// e = curl-updated E^{n+1}; P^{n+1} = a P^n + b P^{n-1} + c E^n and
+// E -= inv_eps * sum_p (P^{n+1} - P^n).
 template <typename real>
 __device__ __forceinline__ real ade_update(const SimDev<real>& s, int code, int comp, real en,
                                            real e) {
@@ -100,196 +79,503 @@ __device__ __forceinline__ real ade_update(const SimDev<real>& s, int code, int 
   return e - M.inv_eps * delta;
 }
 
-template <typename real, bool EXACT>
-__device__ __forceinline__ void node_h(const SimDev<real>& s, int id, int64_t step) {
+// ---- TE: Hz -= ch (d_x Ey - d_y Ex); Ex += ce d_y Hz; Ey += ce (-d_x Hz) ----
+template <typename real>
+struct RowTE {
+  real ex, ey;       // E^n(i + 1)
+  real hz, ce;       // H^{n-1/2}(i), ce(i)
+  real phx, phy;     // Hz psi: x term (HZX), y term (HZY), level p
+  real pey, pex;     // Ex psi y term (EXY), Ey psi x term (EYX)
+};
+
+template <typename real, bool EXACT, bool HSRC>
+__device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk& c, int p,
+                                         int64_t step) {
+  const int lane = threadIdx.x & 63;
   const int nx = s.nx, ny = s.ny;
-  const int i = id / ny, j = id - i * ny;
-  const real K = -s.ch;
-  Term<real> A, B;
-  if (s.pol == 0) {   // TE: Hz -= ch (d_x Ey - d_y Ex)
-    if (i < nx - 1 && j < ny - 1) {
-      A.on = true;
-      B.on = true;
-      A.d = diff<real, EXACT>(s.f[1][id + ny], s.f[1][id], s.d[0], s.inv_d[0]);
-      psi_term(A, s, false, 0, PSI_HZX, i, j);
-      B.d = diff<real, EXACT>(s.f[0][id + 1], s.f[0][id], s.d[1], s.inv_d[1]);
-      psi_term(B, s, false, 1, PSI_HZY, i, j);
-      apply_terms(s.f[5], id, K, A, B);
+  const int j = c.j0 + lane;
+  const bool col = j >= 0 && j < ny;
+  const bool own = col && lane >= 1 && lane <= MARCH_OWNED;
+  const int jc = min(max(j, 0), ny - 1);
+  const real* Ex = s.f[p][0];
+  const real* Ey = s.f[p][1];
+  const real* Hz = s.f[p][5];
+  real* Exo = s.f[p ^ 1][0];
+  real* Eyo = s.f[p ^ 1][1];
+  real* Hzo = s.f[p ^ 1][5];
+  const real* hin_x = s.psiH[p][PSI_HZX - 6];
+  const real* hin_y = s.psiH[p][PSI_HZY - 6];
+  real* hout_x = s.psiH[p ^ 1][PSI_HZX - 6];
+  real* hout_y = s.psiH[p ^ 1][PSI_HZY - 6];
+  real* pe_y = s.psiE[PSI_EXY];
+  real* pe_x = s.psiE[PSI_EYX];
+  const Slab xe{s.slab_e[0][0], s.slab_e[0][1]}, xh{s.slab_h[0][0], s.slab_h[0][1]};
+  const Slab ye{s.slab_e[1][0], s.slab_e[1][1]}, yh{s.slab_h[1][0], s.slab_h[1][1]};
+  const bool pyh = col && yh.in(jc), pye = own && ye.in(jc);
+  const int myh = yh.m(jc), mye = ye.m(jc);
+  const real byh = pyh ? s.b_h[1][jc] : real(0), cyh = pyh ? s.c_h[1][jc] : real(0);
+  const real bye = pye ? s.b_e[1][jc] : real(0), cye = pye ? s.c_e[1][jc] : real(0);
+  const int ph1 = s.ph[1], pe1 = s.pe[1];
+  const real d0 = s.d[0], d1 = s.d[1], r0 = s.inv_d[0], r1 = s.inv_d[1];
+  const real Kh = -s.ch;
+  const int i0 = c.i0, i1 = c.i1, ib = i0 > 0 ? i0 - 1 : 0;
+
+  auto fetch = [&](RowTE<real>& R, int i) {
+    const int o = i * ny + jc;
+    R.ex = R.ey = real(0);
+    if (col && i + 1 < nx) {
+      R.ex = Ex[o + ny];
+      R.ey = Ey[o + ny];
     }
-    real v = add_sources(s, 1, 5, id, step, s.f[5][id]);
-    s.f[5][id] = v;
-    dft_sample(s, 5, i, j, step, v);
-  } else {            // TM: Hx -= ch d_y Ez ; Hy -= ch (- d_x Ez)
-    B.on = false;
-    if (j < ny - 1) {
-      A.on = true;
-      A.d = diff<real, EXACT>(s.f[2][id + 1], s.f[2][id], s.d[1], s.inv_d[1]);
-      psi_term(A, s, false, 1, PSI_HXY, i, j);
-      apply_terms(s.f[3], id, K, A, B);
+    R.hz = col ? Hz[o] : real(0);
+    R.ce = own ? s.ce[o] : real(0);
+    R.phx = (col && xh.in(i)) ? hin_x[xh.m(i) * ny + jc] : real(0);
+    R.phy = pyh ? hin_y[i * ph1 + myh] : real(0);
+    R.pey = pye ? pe_y[i * pe1 + mye] : real(0);
+    R.pex = (own && xe.in(i)) ? pe_x[xe.m(i) * ny + jc] : real(0);
+  };
+
+  real ex_c = real(0), ey_c = real(0);   // E^n(i)
+  if (col) {
+    ex_c = Ex[ib * ny + jc];
+    ey_c = Ey[ib * ny + jc];
+  }
+  real hz_prev = real(0);                // H^{n+1/2}(i - 1)
+  auto row = [&](int i, const RowTE<real>& R, RowTE<real>& N) {
+    if (i + 1 < i1) fetch(N, i + 1);
+    const bool live = i >= i0;
+    const real ex_r = __shfl_down(ex_c, 1);
+    // H^{n+1/2}(i)
+    const bool pxh = xh.in(i);
+    const bool hupd = col && i < nx - 1 && j < ny - 1;
+    real hz = R.hz, nhx = real(0), nhy = real(0);
+    if (hupd) {
+      const real bxh = pxh ? s.b_h[0][i] : real(0), cxh = pxh ? s.c_h[0][i] : real(0);
+      const real dA = diff<real, EXACT>(R.ey, ey_c, d0, r0);
+      const real dB = diff<real, EXACT>(ex_r, ex_c, d1, r1);
+      hz = curl_update(hz, Kh, true, dA, pxh, R.phx, bxh, cxh, &nhx, true, dB, pyh, R.phy, byh,
+                       cyh, &nhy);
     }
-    A.on = false;
-    if (i < nx - 1) {
-      B.on = true;
-      B.d = diff<real, EXACT>(s.f[2][id + ny], s.f[2][id], s.d[0], s.inv_d[0]);
-      psi_term(B, s, false, 0, PSI_HYX, i, j);
-      apply_terms(s.f[4], id, K, A, B);
+    if (HSRC && col) hz = add_h_sources(s, 5, (int64_t)i * ny + j, step, hz);
+    if (own && live) {
+      const int o = i * ny + j;
+      Hzo[o] = hz;
+      if (hupd && pxh) hout_x[xh.m(i) * ny + j] = nhx;
+      if (hupd && pyh) hout_y[i * ph1 + myh] = nhy;
     }
-    for (int c = 3; c <= 4; ++c) {
-      real v = add_sources(s, 1, c, id, step, s.f[c][id]);
-      s.f[c][id] = v;
-      dft_sample(s, c, i, j, step, v);
+    const real hz_l = __shfl_up(hz, 1);
+    // E^{n+1}(i)
+    if (own && live) {
+      const int o = i * ny + j;
+      const real K = R.ce;
+      const bool pxe = xe.in(i);
+      real ex = ex_c, ey = ey_c, nex = real(0), ney = real(0), unused = real(0);
+      if (j >= 1) {
+        const real dA = diff<real, EXACT>(hz, hz_l, d1, r1);
+        ex = curl_update(ex, K, true, dA, pye, R.pey, bye, cye, &nex, false, real(0), false,
+                         real(0), real(0), real(0), &unused);
+      }
+      if (i >= 1) {
+        const real bxe = pxe ? s.b_e[0][i] : real(0), cxe = pxe ? s.c_e[0][i] : real(0);
+        const real dB = diff<real, EXACT>(hz, hz_prev, d0, r0);
+        ey = curl_update(ey, K, false, real(0), false, real(0), real(0), real(0), &unused, true,
+                         dB, pxe, R.pex, bxe, cxe, &ney);
+      }
+      Exo[o] = ex;
+      Eyo[o] = ey;
+      if (j >= 1 && pye) pe_y[i * pe1 + mye] = nex;
+      if (i >= 1 && pxe) pe_x[xe.m(i) * ny + j] = ney;
+    }
+    hz_prev = hz;
+  };
+  // Two row buffers alternate so no register still being loaded is copied.
+  RowTE<real> A, B;
+  fetch(A, ib);
+  for (int i = ib; i < i1; i += 2) {
+    if (i > ib) {
+      ex_c = B.ex;
+      ey_c = B.ey;
+    }
+    row(i, A, B);
+    if (i + 1 >= i1) break;
+    ex_c = A.ex;
+    ey_c = A.ey;
+    row(i + 1, B, A);
+  }
+}
+
+// ---- TM: Hx -= ch d_y Ez; Hy -= ch (-d_x Ez); Ez += ce (d_x Hy - d_y Hx) ----
+template <typename real>
+struct RowTM {
+  real ez;           // E^n(i + 1)
+  real hx, hy, ce;   // H^{n-1/2}(i), ce(i)
+  real phy, phx;     // Hx psi y term (HXY), Hy psi x term (HYX), level p
+  real pex, pey;     // Ez psi x term (EZX), y term (EZY)
+};
+
+template <typename real, bool EXACT, bool HSRC>
+__device__ __forceinline__ void march_tm(const SimDev<real>& s, const MarchChunk& c, int p,
+                                         int64_t step) {
+  const int lane = threadIdx.x & 63;
+  const int nx = s.nx, ny = s.ny;
+  const int j = c.j0 + lane;
+  const bool col = j >= 0 && j < ny;
+  const bool own = col && lane >= 1 && lane <= MARCH_OWNED;
+  const int jc = min(max(j, 0), ny - 1);
+  const real* Ez = s.f[p][2];
+  const real* Hx = s.f[p][3];
+  const real* Hy = s.f[p][4];
+  real* Ezo = s.f[p ^ 1][2];
+  real* Hxo = s.f[p ^ 1][3];
+  real* Hyo = s.f[p ^ 1][4];
+  const real* hin_y = s.psiH[p][PSI_HXY - 6];
+  const real* hin_x = s.psiH[p][PSI_HYX - 6];
+  real* hout_y = s.psiH[p ^ 1][PSI_HXY - 6];
+  real* hout_x = s.psiH[p ^ 1][PSI_HYX - 6];
+  real* pe_x = s.psiE[PSI_EZX];
+  real* pe_y = s.psiE[PSI_EZY];
+  const Slab xe{s.slab_e[0][0], s.slab_e[0][1]}, xh{s.slab_h[0][0], s.slab_h[0][1]};
+  const Slab ye{s.slab_e[1][0], s.slab_e[1][1]}, yh{s.slab_h[1][0], s.slab_h[1][1]};
+  const bool pyh = col && yh.in(jc), pye = own && ye.in(jc);
+  const int myh = yh.m(jc), mye = ye.m(jc);
+  const real byh = pyh ? s.b_h[1][jc] : real(0), cyh = pyh ? s.c_h[1][jc] : real(0);
+  const real bye = pye ? s.b_e[1][jc] : real(0), cye = pye ? s.c_e[1][jc] : real(0);
+  const int ph1 = s.ph[1], pe1 = s.pe[1];
+  const real d0 = s.d[0], d1 = s.d[1], r0 = s.inv_d[0], r1 = s.inv_d[1];
+  const real Kh = -s.ch;
+  const int i0 = c.i0, i1 = c.i1, ib = i0 > 0 ? i0 - 1 : 0;
+
+  auto fetch = [&](RowTM<real>& R, int i) {
+    const int o = i * ny + jc;
+    R.ez = (col && i + 1 < nx) ? Ez[o + ny] : real(0);
+    R.hx = col ? Hx[o] : real(0);
+    R.hy = col ? Hy[o] : real(0);
+    R.ce = own ? s.ce[o] : real(0);
+    R.phy = pyh ? hin_y[i * ph1 + myh] : real(0);
+    R.phx = (col && xh.in(i)) ? hin_x[xh.m(i) * ny + jc] : real(0);
+    R.pex = (own && xe.in(i)) ? pe_x[xe.m(i) * ny + jc] : real(0);
+    R.pey = pye ? pe_y[i * pe1 + mye] : real(0);
+  };
+
+  real ez_c = col ? Ez[ib * ny + jc] : real(0);   // E^n(i)
+  real hy_prev = real(0);                          // Hy^{n+1/2}(i - 1)
+  auto row = [&](int i, const RowTM<real>& R, RowTM<real>& N) {
+    if (i + 1 < i1) fetch(N, i + 1);
+    const bool live = i >= i0;
+    const real ez_r = __shfl_down(ez_c, 1);
+    const bool pxh = xh.in(i);
+    real hx = R.hx, hy = R.hy, nhy = real(0), nhx = real(0), unused = real(0);
+    const bool xupd = col && j < ny - 1, yupd = col && i < nx - 1;
+    if (xupd) {
+      const real dA = diff<real, EXACT>(ez_r, ez_c, d1, r1);
+      hx = curl_update(hx, Kh, true, dA, pyh, R.phy, byh, cyh, &nhy, false, real(0), false,
+                       real(0), real(0), real(0), &unused);
+    }
+    if (yupd) {
+      const real bxh = pxh ? s.b_h[0][i] : real(0), cxh = pxh ? s.c_h[0][i] : real(0);
+      const real dB = diff<real, EXACT>(R.ez, ez_c, d0, r0);
+      hy = curl_update(hy, Kh, false, real(0), false, real(0), real(0), real(0), &unused, true, dB,
+                       pxh, R.phx, bxh, cxh, &nhx);
+    }
+    if (HSRC && col) {
+      const int64_t cell = (int64_t)i * ny + j;
+      hx = add_h_sources(s, 3, cell, step, hx);
+      hy = add_h_sources(s, 4, cell, step, hy);
+    }
+    if (own && live) {
+      const int o = i * ny + j;
+      Hxo[o] = hx;
+      Hyo[o] = hy;
+      if (xupd && pyh) hout_y[i * ph1 + myh] = nhy;
+      if (yupd && pxh) hout_x[xh.m(i) * ny + j] = nhx;
+    }
+    const real hx_l = __shfl_up(hx, 1);
+    if (own && live) {
+      const int o = i * ny + j;
+      const real K = R.ce;
+      const bool pxe = xe.in(i);
+      real ez = ez_c, nzx = real(0), nzy = real(0);
+      const bool eupd = i >= 1 && j >= 1;
+      if (eupd) {
+        const real bxe = pxe ? s.b_e[0][i] : real(0), cxe = pxe ? s.c_e[0][i] : real(0);
+        const real dA = diff<real, EXACT>(hy, hy_prev, d0, r0);
+        const real dB = diff<real, EXACT>(hx, hx_l, d1, r1);
+        ez = curl_update(ez, K, true, dA, pxe, R.pex, bxe, cxe, &nzx, true, dB, pye, R.pey, bye,
+                         cye, &nzy);
+      }
+      Ezo[o] = ez;
+      if (eupd && pxe) pe_x[xe.m(i) * ny + j] = nzx;
+      if (eupd && pye) pe_y[i * pe1 + mye] = nzy;
+    }
+    hy_prev = hy;
+  };
+  RowTM<real> A, B;
+  fetch(A, ib);
+  for (int i = ib; i < i1; i += 2) {
+    if (i > ib) ez_c = B.ez;
+    row(i, A, B);
+    if (i + 1 >= i1) break;
+    ez_c = A.ez;
+    row(i + 1, B, A);
+  }
+}
+
+template <typename real, bool EXACT, int POL, bool HSRC>
+__device__ __forceinline__ void march_chunk(const BatchArgs<real>& a, const MarchChunk& c,
+                                            int64_t step) {
+  const SimDev<real>& s = a.sims[c.sim];
+  if (step >= s.n_steps) return;
+  const int p = (int)(step & 1);
+  if (POL == 0) march_te<real, EXACT, HSRC>(s, c, p, step);
+  else march_tm<real, EXACT, HSRC>(s, c, p, step);
+}
+
+// Fold DFT samples q0..q1 of entry e from the ring into its accumulators:
+// acc += (phase * v) * dt per sample in order (simulation.py:1116-1133).
+template <typename real>
+__device__ void dft_fold(const BatchArgs<real>& a, const SimDev<real>& s, int e, int64_t q0,
+                         int64_t q1) {
+  const DftEntry D = a.dft[e];
+  const DftDev& d = s.dft[D.mon];
+  const double* ph = D.comp >= 3 ? d.phase_h : d.phase_e;
+  double* A = d.acc + 2 * ((int64_t)D.slot * d.n_freq * d.nbox + D.r);
+  for (int f = 0; f < d.n_freq; ++f) {
+    double* acc = A + 2 * (int64_t)f * d.nbox;
+    double re = acc[0], im = acc[1];
+    for (int64_t q = q0; q <= q1; ++q) {
+      const double v = (double)a.ring[(q % BATCH_DFT_RING) * a.n_dft + e];
+      const double* pq = ph + 2 * (q * d.n_freq + f);
+      re = re + (pq[0] * v) * d.dt;
+      im = im + (pq[1] * v) * d.dt;
+    }
+    acc[0] = re;
+    acc[1] = im;
+  }
+}
+
+// Post pass of one node: ADE correction of the curl-updated E (E^n from
+// level p), E-side soft sources, then the DFT samples of the node.
+template <typename real>
+__device__ void post_node(const BatchArgs<real>& a, int idx, int64_t step) {
+  const PostEntry P = a.post[idx];
+  const SimDev<real>& s = a.sims[P.sim];
+  if (step >= s.n_steps) return;
+  const int p = (int)(step & 1), po = p ^ 1;
+  const int cell = P.cell;
+  if (P.ade_code >= 0) {
+    const int c0 = s.pol == 0 ? 0 : 2, c1 = s.pol == 0 ? 1 : 2;   // idle poles stay zero
+    for (int c = c0; c <= c1; ++c) {
+      real* E = s.f[po][c];
+      E[cell] = ade_update(s, P.ade_code, c, s.f[p][c][cell], E[cell]);
+    }
+  }
+  for (int k = P.src0; k < P.src1; ++k) {
+    const int q = a.src_list[k];
+    real* F = s.f[po][s.src_e.comp[q]];
+    F[cell] = add_group(s.src_e, q, s.n_steps, step, F[cell]);
+  }
+  for (int e = P.dft0; e < P.dft1; ++e) {
+    const DftEntry D = a.dft[e];
+    const int interval = s.dft[D.mon].interval;
+    if (interval > 1 && step % interval != 0) continue;
+    const int64_t q = step / interval;
+    a.ring[(q % BATCH_DFT_RING) * a.n_dft + e] = s.f[po][D.comp][cell];
+    if (q + 1 - a.folded[e] == BATCH_DFT_RING) {
+      dft_fold(a, s, e, a.folded[e], q);
+      a.folded[e] = q + 1;
     }
   }
 }
 
-template <typename real, bool EXACT>
-__device__ __forceinline__ void node_e(const SimDev<real>& s, int id, int64_t step) {
-  const int ny = s.ny;
-  const int i = id / ny, j = id - i * ny;
-  const real K = s.ce[id];
-  const int code = s.ade_code ? s.ade_code[id] : -1;
-  Term<real> A, B;
-  if (s.pol == 0) {   // TE: Ex += ce d_y Hz ; Ey += ce (- d_x Hz)
-    const real ex0 = s.f[0][id], ey0 = s.f[1][id];
-    B.on = false;
-    if (j >= 1) {
-      A.on = true;
-      A.d = diff<real, EXACT>(s.f[5][id], s.f[5][id - 1], s.d[1], s.inv_d[1]);
-      psi_term(A, s, true, 1, PSI_EXY, i, j);
-      apply_terms(s.f[0], id, K, A, B);
-    }
-    A.on = false;
-    if (i >= 1) {
-      B.on = true;
-      B.d = diff<real, EXACT>(s.f[5][id], s.f[5][id - ny], s.d[0], s.inv_d[0]);
-      psi_term(B, s, true, 0, PSI_EYX, i, j);
-      apply_terms(s.f[1], id, K, A, B);
-    }
-    real ex = s.f[0][id], ey = s.f[1][id];
-    if (code >= 0) {
-      ex = ade_update(s, code, 0, ex0, ex);
-      ey = ade_update(s, code, 1, ey0, ey);   // the idle Ez poles stay zero
-    }
-    ex = add_sources(s, 0, 0, id, step, ex);
-    ey = add_sources(s, 0, 1, id, step, ey);
-    s.f[0][id] = ex;
-    s.f[1][id] = ey;
-    dft_sample(s, 0, i, j, step, ex);
-    dft_sample(s, 1, i, j, step, ey);
-  } else {            // TM: Ez += ce (d_x Hy - d_y Hx)
-    const real ez0 = s.f[2][id];
-    if (i >= 1 && j >= 1) {
-      A.on = true;
-      B.on = true;
-      A.d = diff<real, EXACT>(s.f[4][id], s.f[4][id - ny], s.d[0], s.inv_d[0]);
-      psi_term(A, s, true, 0, PSI_EZX, i, j);
-      B.d = diff<real, EXACT>(s.f[3][id], s.f[3][id - 1], s.d[1], s.inv_d[1]);
-      psi_term(B, s, true, 1, PSI_EZY, i, j);
-      apply_terms(s.f[2], id, K, A, B);
-    }
-    real ez = s.f[2][id];
-    if (code >= 0) ez = ade_update(s, code, 2, ez0, ez);   // idle Ex/Ey poles stay zero
-    ez = add_sources(s, 0, 2, id, step, ez);
-    s.f[2][id] = ez;
-    dft_sample(s, 2, i, j, step, ez);
-  }
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
 }
 
-// One phase (0 = H, 1 = E) over the work chunks owned by this block.
-template <typename real, bool EXACT>
-__device__ __forceinline__ void run_phase(const BatchArgs<real>& a, int phase, int64_t step) {
-  for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
-    const int sim = a.chunk_sim[ch];
-    const SimDev<real>& s = a.sims[sim];
-    if (step >= s.n_steps) continue;
-    const int c0 = a.chunk_first[ch];
-    const int c1 = min(c0 + a.chunk_cells, s.nx * s.ny);
-    for (int id = c0 + (int)threadIdx.x; id < c1; id += (int)blockDim.x) {
-      if (phase == 0) node_h<real, EXACT>(s, id, step);
-      else node_e<real, EXACT>(s, id, step);
+// Grid barrier with a bounded wait. The last workgroup to arrive resets the
+// arrival count and advances the generation; the others wait for the
+// generation to change. A wait that outlasts wait_ticks of the wall clock
+// (workgroups that are not all resident at once) sets *fail, and every
+// workgroup then leaves its step loop, so the kernel always finishes.
+__device__ bool grid_barrier(unsigned* bar, int* fail, uint64_t wait_ticks) {
+  __shared__ int ok_shared;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned n = gridDim.x;
+    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned arrived =
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (arrived == n) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const uint64_t t0 = wall_clock64();
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ok = 0;
+          break;
+        }
+        if (wall_clock64() - t0 > wait_ticks) {
+          __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    ok_shared = ok;
   }
+  __syncthreads();
+  return ok_shared != 0;
 }
 
 }  // namespace
 
-// Per-phase kernel; the step is read from a device counter so a captured
+// One time step of the chunks [first, first + count) (one polarisation and
+// H-source variant); the step is read from a device counter so a captured
 // graph of several steps can be replayed (k_batch_tick advances it).
-template <typename real, bool EXACT>
-__global__ void k_batch_phase(BatchArgs<real> a, int phase) {
-  run_phase<real, EXACT>(a, phase, *a.step);
+template <typename real, bool EXACT, int POL, bool HSRC>
+__global__ __launch_bounds__(512) void k_batch_march(BatchArgs<real> a, int first, int count) {
+  const int w = wave_id();
+  if (w >= count) return;
+  march_chunk<real, EXACT, POL, HSRC>(a, a.chunks[first + w], *a.step);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_batch_post(BatchArgs<real> a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < a.n_post) post_node(a, idx, *a.step);
+}
+
+// Fold the DFT samples still in the ring after an advance that ended at
+// step_end.
+template <typename real>
+__global__ __launch_bounds__(256) void k_batch_dft_flush(BatchArgs<real> a, int64_t step_end) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n_dft) return;
+  const SimDev<real>& s = a.sims[a.dft_sim[e]];
+  const int64_t last = min(step_end, s.n_steps) - 1;
+  if (last < 0) return;
+  const int64_t q_last = last / s.dft[a.dft[e].mon].interval;
+  if (q_last + 1 > a.folded[e]) {
+    dft_fold(a, s, e, a.folded[e], q_last);
+    a.folded[e] = q_last + 1;
+  }
 }
 
 __global__ void k_batch_tick(int64_t* step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step = *step + 1;
 }
 
-// Persistent kernel: all steps [step0, step1) in one cooperative launch.
+__global__ void k_batch_sync_reset(unsigned* bar, int* fail) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    bar[0] = 0;
+    *fail = 0;
+  }
+}
+
+// Persistent kernel: steps [step0, step1) in one launch; per step the march
+// over all chunks, a grid barrier, the post pass, a grid barrier. Every
+// wavefront loops over its share of the chunks, every thread over its share
+// of the post nodes.
 template <typename real, bool EXACT>
-__global__ void k_batch_persistent(BatchArgs<real> a, int64_t step0, int64_t step1) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+__global__ __launch_bounds__(512) void k_batch_persistent(BatchArgs<real> a, int64_t step0,
+                                                          int64_t step1) {
+  const int wave = wave_id();
+  const int waves = (int)(gridDim.x * blockDim.x) >> 6;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int threads = gridDim.x * blockDim.x;
   for (int64_t st = step0; st < step1; ++st) {
-    run_phase<real, EXACT>(a, 0, st);
-    grid.sync();
-    run_phase<real, EXACT>(a, 1, st);
-    grid.sync();
+    for (int ci = wave; ci < a.n_chunks; ci += waves) {
+      const MarchChunk c = a.chunks[ci];
+      const SimDev<real>& s = a.sims[c.sim];
+      const bool hsrc = s.src_h.n_groups > 0;
+      if (s.pol == 0) {
+        if (hsrc) march_chunk<real, EXACT, 0, true>(a, c, st);
+        else march_chunk<real, EXACT, 0, false>(a, c, st);
+      } else {
+        if (hsrc) march_chunk<real, EXACT, 1, true>(a, c, st);
+        else march_chunk<real, EXACT, 1, false>(a, c, st);
+      }
+    }
+    if (!grid_barrier(a.bar, a.fail, a.wait_ticks)) break;
+    for (int idx = tid; idx < a.n_post; idx += threads) post_node(a, idx, st);
+    if (!grid_barrier(a.bar, a.fail, a.wait_ticks)) break;
   }
 }
 
 template <typename real>
-void launch_batch_phase(const BatchArgs<real>& a, int phase, bool exact, int grid, int block,
-                        hipStream_t s) {
-  if (exact) k_batch_phase<real, true><<<grid, block, 0, s>>>(a, phase);
-  else k_batch_phase<real, false><<<grid, block, 0, s>>>(a, phase);
+void launch_batch_march(const BatchArgs<real>& a, const ChunkGroup& g, bool exact,
+                        int waves_per_block, hipStream_t s) {
+  if (g.count <= 0) return;
+  const int blocks = (g.count + waves_per_block - 1) / waves_per_block;
+  const int threads = waves_per_block * 64;
+#define PF_MARCH(E, P, H) \
+  k_batch_march<real, E, P, H><<<blocks, threads, 0, s>>>(a, g.first, g.count)
+  if (exact) {
+    if (g.pol == 0) { if (g.hsrc) PF_MARCH(true, 0, true); else PF_MARCH(true, 0, false); }
+    else { if (g.hsrc) PF_MARCH(true, 1, true); else PF_MARCH(true, 1, false); }
+  } else {
+    if (g.pol == 0) { if (g.hsrc) PF_MARCH(false, 0, true); else PF_MARCH(false, 0, false); }
+    else { if (g.hsrc) PF_MARCH(false, 1, true); else PF_MARCH(false, 1, false); }
+  }
+#undef PF_MARCH
+}
+
+template <typename real>
+void launch_batch_post(const BatchArgs<real>& a, hipStream_t s) {
+  if (a.n_post > 0) k_batch_post<real><<<(a.n_post + 255) / 256, 256, 0, s>>>(a);
+}
+
+template <typename real>
+void launch_batch_dft_flush(const BatchArgs<real>& a, int64_t step_end, hipStream_t s) {
+  if (a.n_dft > 0) k_batch_dft_flush<real><<<(a.n_dft + 255) / 256, 256, 0, s>>>(a, step_end);
 }
 
 void launch_batch_tick(int64_t* step, hipStream_t s) { k_batch_tick<<<1, 64, 0, s>>>(step); }
 
-// Workgroups of the persistent kernel: every workgroup must be resident at
-// once for the grid barrier, so the grid is capped at PERSISTENT_MAX_PER_CU
-// workgroups per CU (below what the occupancy calculator allows) and at the
-// number of work chunks.
-constexpr int PERSISTENT_MAX_PER_CU = 4;
+void launch_batch_sync_reset(unsigned* bar, int* fail, hipStream_t s) {
+  k_batch_sync_reset<<<1, 64, 0, s>>>(bar, fail);
+}
 
 template <typename real>
-hipError_t batch_persistent_grid(bool exact, int block, int n_chunks, int* per_cu, int* n_cu,
-                                 int* grid) {
+hipError_t batch_persistent_capacity(bool exact, int waves_per_block, int* per_cu, int* n_cu) {
   const void* fn = exact ? reinterpret_cast<const void*>(&k_batch_persistent<real, true>)
                          : reinterpret_cast<const void*>(&k_batch_persistent<real, false>);
   int dev = 0;
   *per_cu = 0;
   *n_cu = 0;
-  *grid = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, block, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, waves_per_block * 64, 0);
   if (e != hipSuccess) return e;
   e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  e = hipDeviceGetAttribute(n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e != hipSuccess) return e;
-  if (*per_cu < 1 || *n_cu < 1) return hipErrorInvalidConfiguration;
-  *grid = std::max(1, std::min(std::min(*per_cu, PERSISTENT_MAX_PER_CU) * *n_cu, n_chunks));
-  return hipSuccess;
+  return hipDeviceGetAttribute(n_cu, hipDeviceAttributeMultiprocessorCount, dev);
 }
 
 template <typename real>
-hipError_t launch_batch_persistent(BatchArgs<real>& a, bool exact, int block, int64_t step0,
-                                   int64_t step1, hipStream_t s) {
-  int per_cu = 0, n_cu = 0, grid = 0;
-  hipError_t e = batch_persistent_grid<real>(exact, block, a.n_chunks, &per_cu, &n_cu, &grid);
-  if (e != hipSuccess) return e;
-  const void* fn = exact ? reinterpret_cast<const void*>(&k_batch_persistent<real, true>)
-                         : reinterpret_cast<const void*>(&k_batch_persistent<real, false>);
-  void* args[] = {&a, &step0, &step1};
-  return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, s);
+void launch_batch_persistent(const BatchArgs<real>& a, bool exact, int waves_per_block, int blocks,
+                             int64_t step0, int64_t step1, hipStream_t s) {
+  if (exact)
+    k_batch_persistent<real, true><<<blocks, waves_per_block * 64, 0, s>>>(a, step0, step1);
+  else
+    k_batch_persistent<real, false><<<blocks, waves_per_block * 64, 0, s>>>(a, step0, step1);
 }
 
-template void launch_batch_phase<float>(const BatchArgs<float>&, int, bool, int, int, hipStream_t);
-template void launch_batch_phase<double>(const BatchArgs<double>&, int, bool, int, int, hipStream_t);
-template hipError_t launch_batch_persistent<float>(BatchArgs<float>&, bool, int, int64_t, int64_t,
-                                                   hipStream_t);
-template hipError_t launch_batch_persistent<double>(BatchArgs<double>&, bool, int, int64_t, int64_t,
-                                                    hipStream_t);
-template hipError_t batch_persistent_grid<float>(bool, int, int, int*, int*, int*);
-template hipError_t batch_persistent_grid<double>(bool, int, int, int*, int*, int*);
+template void launch_batch_march<float>(const BatchArgs<float>&, const ChunkGroup&, bool, int,
+                                        hipStream_t);
+template void launch_batch_march<double>(const BatchArgs<double>&, const ChunkGroup&, bool, int,
+                                         hipStream_t);
+template void launch_batch_post<float>(const BatchArgs<float>&, hipStream_t);
+template void launch_batch_post<double>(const BatchArgs<double>&, hipStream_t);
+template void launch_batch_dft_flush<float>(const BatchArgs<float>&, int64_t, hipStream_t);
+template void launch_batch_dft_flush<double>(const BatchArgs<double>&, int64_t, hipStream_t);
+template void launch_batch_persistent<float>(const BatchArgs<float>&, bool, int, int, int64_t,
+                                             int64_t, hipStream_t);
+template void launch_batch_persistent<double>(const BatchArgs<double>&, bool, int, int, int64_t,
+                                              int64_t, hipStream_t);
+template hipError_t batch_persistent_capacity<float>(bool, int, int*, int*);
+template hipError_t batch_persistent_capacity<double>(bool, int, int*, int*);
 
 }  // namespace pfhip

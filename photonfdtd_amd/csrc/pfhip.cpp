@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <climits>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -201,8 +204,19 @@ class Stepper final : public pf_handle {
   void checkpoint_enable(int interval) override;
   void adjoint(int n_cot, const pf_cotangent* cot, double* grad) override;
   // batched 2-D stepping support
-  void describe(pfhip::SimDev<real>& s, int pol);
+  // Nodes of this simulation that need the batched post pass.
+  struct PostNodes {
+    std::vector<std::pair<int, int>> ade;    // (cell, (medium << 26) | index)
+    std::vector<std::pair<int, int>> esrc;   // (cell, E-side source group)
+    struct Dft {
+      int cell, mon, slot, comp;
+      int64_t r;
+    };
+    std::vector<Dft> dft;
+  };
+  void describe(pfhip::SimDev<real>& s, int pol, PostNodes& post);
   void reset(hipStream_t s);
+  void set_level(int p) { cur_ = p; }
   hipStream_t stream() const { return stream_; }
   int64_t ncell() const { return shape[0] * shape[1] * shape[2]; }
   void timing_read(double* ms, int64_t* launches) override;
@@ -757,26 +771,46 @@ void Stepper<real>::load_state(const real* src) {
 }
 
 template <typename real>
-void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol) {
+void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
   if (fused_) throw ArgError{"batched stepping needs the generic layout (PF_FLAG_GENERIC)"};
   if (shape[2] != 1 || shape[0] < 2 || shape[1] < 2)
     throw ArgError{"batched stepping supports xy-plane 2-D grids (nx, ny >= 2, nz == 1)"};
   if (shape[0] * shape[1] >= (int64_t(1) << 31)) throw ArgError{"grid too large for a batch"};
   if (pol != 0 && pol != 1) throw ArgError{"polarisation must be 0 (TE) or 1 (TM)"};
+  if (!slab_ok_) throw ArgError{"batched stepping needs CPML slabs of the form [0, lo) U [hi, n)"};
   if (dft_.size() > (size_t)pfhip::BATCH_MAX_DFT) throw ArgError{"too many DFT monitors for a batch"};
   if (ade_.size() > (size_t)pfhip::BATCH_MAX_ADE) throw ArgError{"too many dispersive media for a batch"};
   if (!flux_.empty()) throw ArgError{"flux monitors are not supported in a batch"};
+  const int64_t n = ncell();
+  const int ny = (int)shape[1];
+  // second time level of the fields and of the H-side psi
+  if (!lvl_[1]) {
+    lvl_[1] = arena_.zeros<real>(6 * (size_t)n, stream_);
+    for (int c = 0; c < 6; ++c) f_[1][c] = lvl_[1] + (size_t)c * n;
+    fields_dev_[1] = arena_.upload(f_[1], 6, stream_);
+  }
+  if (!psi_h_lvl1_ && psi_len_h_) psi_h_lvl1_ = arena_.zeros<real>(psi_len_h_, stream_);
   std::memset(&s, 0, sizeof(s));
-  for (int c = 0; c < 6; ++c) s.f[c] = f_[0][c];
+  for (int c = 0; c < 6; ++c) {
+    s.f[0][c] = f_[0][c];
+    s.f[1][c] = f_[1][c];
+  }
+  for (int q = 0; q < 6; ++q) {
+    s.psiE[q] = args_.psi[q];
+    real* h0 = args_.psi[6 + q];
+    s.psiH[0][q] = h0;
+    s.psiH[1][q] = h0 ? psi_h_lvl1_ + (h0 - psi_side_[1]) : nullptr;
+  }
   s.ce = args_.ce;
-  for (int q = 0; q < pfhip::PSI_COUNT; ++q) s.psi[q] = args_.psi[q];
-  for (int a = 0; a < 3; ++a) {
+  for (int a = 0; a < 2; ++a) {
     s.b_e[a] = args_.b_e[a];
     s.c_e[a] = args_.c_e[a];
     s.b_h[a] = args_.b_h[a];
     s.c_h[a] = args_.c_h[a];
-    s.map_e[a] = args_.map_e[a];
-    s.map_h[a] = args_.map_h[a];
+    for (int q = 0; q < 2; ++q) {
+      s.slab_e[a][q] = slab_e_[a][q];
+      s.slab_h[a][q] = slab_h_[a][q];
+    }
     s.pe[a] = args_.pe[a];
     s.ph[a] = args_.ph[a];
     s.d[a] = args_.d[a];
@@ -789,60 +823,76 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol) {
   s.n_steps = n_steps_;
   s.src_e = src_e_;
   s.src_h = src_h_;
+  s.src_lo = INT64_MAX;
+  s.src_hi = -1;
   for (int side = 0; side < 2; ++side) {
     const pfhip::SourceGroupsDev& g = side ? src_h_ : src_e_;
-    s.src_lo[side] = INT64_MAX;
-    s.src_hi[side] = -1;
-    if (g.n_groups > 0) {
-      std::vector<int64_t> cells(g.n_groups);
-      HIP_CHECK(hipMemcpy(cells.data(), g.cell, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-      s.src_lo[side] = *std::min_element(cells.begin(), cells.end());
-      s.src_hi[side] = *std::max_element(cells.begin(), cells.end());
+    if (g.n_groups == 0) continue;
+    std::vector<int64_t> cells(g.n_groups);
+    std::vector<int32_t> comps(g.n_groups);
+    HIP_CHECK(hipMemcpy(cells.data(), g.cell, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(comps.data(), g.comp, comps.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int q = 0; q < g.n_groups; ++q) {
+      const bool active = pol == 0 ? (comps[q] == 0 || comps[q] == 1 || comps[q] == 5)
+                                   : (comps[q] == 2 || comps[q] == 3 || comps[q] == 4);
+      if (!active) throw ArgError{"a source excites the idle polarisation group"};
+      if (side == 0) post.esrc.push_back({(int)cells[q], q});
+    }
+    if (side == 1) {
+      s.src_lo = *std::min_element(cells.begin(), cells.end());
+      s.src_hi = *std::max_element(cells.begin(), cells.end());
     }
   }
-  if (!ade_.empty()) {
-    std::vector<int> code((size_t)ncell(), -1);
-    for (size_t m = 0; m < ade_.size(); ++m) {
-      if (ade_[m].n >= (int64_t(1) << 26)) throw ArgError{"dispersive region too large for a batch"};
-      std::vector<int64_t> cells(ade_[m].n);
-      HIP_CHECK(hipMemcpy(cells.data(), ade_[m].cells, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-      for (int64_t t = 0; t < ade_[m].n; ++t) code[cells[t]] = (int)(m << 26) | (int)t;
-      pfhip::AdeDev<real>& A = s.ade[m];
-      A.n = ade_[m].n;
-      A.n_poles = ade_[m].n_poles;
-      A.abc = ade_[m].abc;
-      A.inv_eps = ade_[m].inv_eps;
-      A.Pcur = ade_[m].Pcur;
-      A.Pprev = ade_[m].Pprev;
-    }
-    s.ade_code = arena_.upload(code.data(), code.size(), stream_);
-    s.n_ade = (int)ade_.size();
+  for (size_t m = 0; m < ade_.size(); ++m) {
+    if (ade_[m].n >= (int64_t(1) << 26)) throw ArgError{"dispersive region too large for a batch"};
+    std::vector<int64_t> cells(ade_[m].n);
+    HIP_CHECK(hipMemcpy(cells.data(), ade_[m].cells, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < ade_[m].n; ++t)
+      post.ade.push_back({(int)cells[t], (int)(m << 26) | (int)t});
+    pfhip::AdeDev<real>& A = s.ade[m];
+    A.n = ade_[m].n;
+    A.n_poles = ade_[m].n_poles;
+    A.abc = ade_[m].abc;
+    A.inv_eps = ade_[m].inv_eps;
+    A.Pcur = ade_[m].Pcur;
+    A.Pprev = ade_[m].Pprev;
   }
-  s.n_dft = (int)dft_.size();
   for (size_t m = 0; m < dft_.size(); ++m) {
+    const Dft& M = dft_[m];
     pfhip::DftDev& d = s.dft[m];
-    d.n_comp = dft_[m].n_comp;
-    for (int c = 0; c < 6; ++c) d.comps[c] = c < d.n_comp ? dft_[m].comps[c] : -1;
-    d.box = dft_[m].box;
-    for (int a = 0; a < 2; ++a) {
-      d.lo[a] = (int)d.box.start[a];
-      d.hi[a] = (int)(d.box.start[a] + (d.box.count[a] - 1) * d.box.stride[a]);
+    d.n_comp = M.n_comp;
+    for (int c = 0; c < 6; ++c) d.comps[c] = c < M.n_comp ? M.comps[c] : -1;
+    d.nbox = M.nbox;
+    d.n_freq = M.n_freq;
+    d.interval = M.interval;
+    d.phase_e = M.phase_e;
+    d.phase_h = M.phase_h;
+    d.dt = M.dt;
+    d.acc = M.acc;
+    for (int c = 0; c < M.n_comp; ++c) {
+      const int comp = M.comps[c];
+      const bool active = pol == 0 ? (comp == 0 || comp == 1 || comp == 5)
+                                   : (comp == 2 || comp == 3 || comp == 4);
+      if (!active) continue;   // idle components stay zero: their accumulators too
+      for (int64_t bi = 0; bi < M.box.count[0]; ++bi)
+        for (int64_t bj = 0; bj < M.box.count[1]; ++bj) {
+          const int i = (int)(M.box.start[0] + bi * M.box.stride[0]);
+          const int j = (int)(M.box.start[1] + bj * M.box.stride[1]);
+          post.dft.push_back({i * ny + j, (int)m, c, comp, bi * M.box.count[1] + bj});
+        }
     }
-    d.n_freq = dft_[m].n_freq;
-    d.interval = dft_[m].interval;
-    d.phase_e = dft_[m].phase_e;
-    d.phase_h = dft_[m].phase_h;
-    d.dt = dft_[m].dt;
-    d.acc = dft_[m].acc;
   }
+  HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 template <typename real>
 void Stepper<real>::reset(hipStream_t s) {
   const size_t nf = 6 * (size_t)ncell();
-  HIP_CHECK(hipMemsetAsync(lvl_[0], 0, nf * sizeof(real), s));
+  for (int p = 0; p < 2; ++p)
+    if (lvl_[p]) HIP_CHECK(hipMemsetAsync(lvl_[p], 0, nf * sizeof(real), s));
   if (psi_len_e_) HIP_CHECK(hipMemsetAsync(psi_side_[0], 0, psi_len_e_ * sizeof(real), s));
   if (psi_len_h_) HIP_CHECK(hipMemsetAsync(psi_side_[1], 0, psi_len_h_ * sizeof(real), s));
+  if (psi_h_lvl1_) HIP_CHECK(hipMemsetAsync(psi_h_lvl1_, 0, psi_len_h_ * sizeof(real), s));
   for (Ade& a : ade_) {
     const size_t n = 3 * (size_t)a.n_poles * a.n;
     if (n) {
@@ -855,6 +905,7 @@ void Stepper<real>::reset(hipStream_t s) {
     m.count = 0;
     m.sample0 = 0;
   }
+  cur_ = 0;
 }
 
 template <typename real>
@@ -1065,9 +1116,9 @@ void Stepper<real>::flux_read(double* dst) {
 // ---------------------------------------------------------------------------
 struct BatchBase {
   virtual ~BatchBase() = default;
-  virtual void configure(int mode, int block, int chunk) = 0;
+  virtual void configure(int mode, int waves_per_block, int rows) = 0;
   virtual void reset() = 0;
-  virtual void persistent_grid(int block, int* per_cu, int* n_cu, int* grid) = 0;
+  virtual void persistent_grid(int waves_per_block, int* per_cu, int* n_cu, int* grid) = 0;
   virtual void advance(int64_t n) = 0;
   virtual pf_handle* sim(int i) = 0;
   virtual int size() const = 0;
@@ -1083,18 +1134,73 @@ class Batch final : public BatchBase {
       if ((descs[i].exact != 0) != exact_) throw ArgError{"a batch needs one arithmetic mode"};
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     std::vector<pfhip::SimDev<real>> host(n);
+    std::vector<pfhip::PostEntry> post;
+    std::vector<int> src_list, dft_sim, dft_cell;
+    std::vector<pfhip::DftEntry> dft;
     for (int i = 0; i < n; ++i) {
       pf_desc d = descs[i];
       d.flags |= PF_FLAG_GENERIC;
       sims_.emplace_back(new Stepper<real>(d));
-      sims_.back()->describe(host[i], pols[i]);
+      typename Stepper<real>::PostNodes nodes;
+      sims_.back()->describe(host[i], pols[i], nodes);
+      n_steps_.push_back(host[i].n_steps);
+      shapes_.push_back({host[i].nx, host[i].ny});
+      pols_.push_back(pols[i]);
+      hsrc_.push_back(host[i].src_h.n_groups > 0);
       n_steps_max_ = std::max<int64_t>(n_steps_max_, host[i].n_steps);
+      // post nodes of this simulation, one entry per cell in cell order
+      std::vector<int> cells;
+      for (auto& a : nodes.ade) cells.push_back(a.first);
+      for (auto& e : nodes.esrc) cells.push_back(e.first);
+      for (auto& e : nodes.dft) cells.push_back(e.cell);
+      std::sort(cells.begin(), cells.end());
+      cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+      std::sort(nodes.esrc.begin(), nodes.esrc.end());
+      std::stable_sort(nodes.dft.begin(), nodes.dft.end(),
+                       [](const auto& x, const auto& y) { return x.cell < y.cell; });
+      std::sort(nodes.ade.begin(), nodes.ade.end());
+      size_t ia = 0, is = 0, id = 0;
+      for (int cell : cells) {
+        pfhip::PostEntry e{i, cell, -1, 0, 0, 0, 0};
+        if (ia < nodes.ade.size() && nodes.ade[ia].first == cell) e.ade_code = nodes.ade[ia++].second;
+        e.src0 = (int)src_list.size();
+        for (; is < nodes.esrc.size() && nodes.esrc[is].first == cell; ++is)
+          src_list.push_back(nodes.esrc[is].second);
+        e.src1 = (int)src_list.size();
+        e.dft0 = (int)dft.size();
+        for (; id < nodes.dft.size() && nodes.dft[id].cell == cell; ++id) {
+          const auto& q = nodes.dft[id];
+          dft.push_back({q.mon, q.slot, q.comp, q.r});
+          dft_sim.push_back(i);
+          dft_cell.push_back(cell);
+        }
+        e.dft1 = (int)dft.size();
+        post.push_back(e);
+      }
     }
+    if (post.size() > (size_t)INT32_MAX / 2 || dft.size() > (size_t)INT32_MAX / 64)
+      throw ArgError{"too many post-pass nodes in a batch"};
     sims_dev_ = arena_.upload(host.data(), host.size(), stream_);
+    args_.post = post.empty() ? nullptr : arena_.upload(post.data(), post.size(), stream_);
+    args_.n_post = (int)post.size();
+    args_.src_list = src_list.empty() ? nullptr : arena_.upload(src_list.data(), src_list.size(), stream_);
+    args_.dft = dft.empty() ? nullptr : arena_.upload(dft.data(), dft.size(), stream_);
+    args_.n_dft = (int)dft.size();
+    args_.dft_sim = dft.empty() ? nullptr : arena_.upload(dft_sim.data(), dft_sim.size(), stream_);
+    args_.dft_cell = dft.empty() ? nullptr : arena_.upload(dft_cell.data(), dft_cell.size(), stream_);
+    args_.ring = dft.empty() ? nullptr
+                             : arena_.zeros<real>((size_t)pfhip::BATCH_DFT_RING * dft.size(), stream_);
+    args_.folded = dft.empty() ? nullptr : arena_.zeros<int64_t>(dft.size(), stream_);
     step_dev_ = arena_.zeros<int64_t>(1, stream_);
+    bar_ = arena_.zeros<unsigned>(2, stream_);
+    fail_ = arena_.zeros<int>(1, stream_);
+    int dev = 0, khz = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    wait_ticks_ = (uint64_t)std::max(khz, 1) * 2000;   // 2 s per barrier wait
     HIP_CHECK(hipEventCreate(&ev_[0]));
     HIP_CHECK(hipEventCreate(&ev_[1]));
-    configure(1, 256, 4096);
+    configure(2, 4, 64);
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
   ~Batch() override {
@@ -1114,71 +1220,93 @@ class Batch final : public BatchBase {
   int size() const override { return (int)sims_.size(); }
   double last_ms() const override { return last_ms_; }
 
-  // mode 0: one launch per phase; 1: persistent cooperative kernel;
-  // 2: per-phase launches captured as a graph of graph_steps_ steps.
-  void configure(int mode, int block, int chunk) override {
-    if (mode < 0 || mode > 2 || (block != 256 && block != 512 && block != 1024) || chunk < 256)
+  // mode 0: one march launch per step; 1: one persistent launch for all
+  // steps of an advance; 2: march launches replayed from a captured graph of
+  // graph_steps_ steps. rows: owned rows per wavefront chunk.
+  void configure(int mode, int waves_per_block, int rows) override {
+    if (mode < 0 || mode > 2 || (waves_per_block != 1 && waves_per_block != 2 &&
+                                 waves_per_block != 4 && waves_per_block != 8) || rows < 1)
       throw ArgError{"bad batch configuration"};
-    if (mode == 1)
-      throw HipError{PF_E_STATE, "the cooperative persistent mode is disabled: a full-occupancy "
-                                 "launch went silent on a 28M-cell batch; use mode 0 or 2"};
     drop_graph();
     mode_ = mode;
-    block_ = block;
-    if (chunk != chunk_cells_) {
-      chunk_cells_ = chunk;
-      std::vector<int> cs;
-      std::vector<int> cf;
-      for (size_t i = 0; i < sims_.size(); ++i)
-        for (int64_t c = 0; c < sims_[i]->ncell(); c += chunk) {
-          cs.push_back((int)i);
-          cf.push_back((int)c);
+    wpb_ = waves_per_block;
+    if (rows != rows_) {
+      // chunks grouped by kernel variant (polarisation, H-side sources)
+      rows_ = rows;
+      std::vector<pfhip::MarchChunk> cs;
+      groups_.clear();
+      for (int pol = 0; pol < 2; ++pol)
+        for (int hs = 0; hs < 2; ++hs) {
+          pfhip::ChunkGroup g{(int)cs.size(), 0, pol, hs};
+          for (size_t s = 0; s < sims_.size(); ++s) {
+            if (pols_[s] != pol || (int)hsrc_[s] != hs) continue;
+            const int nx = shapes_[s][0], ny = shapes_[s][1];
+            for (int i0 = 0; i0 < nx; i0 += rows)
+              for (int j = 0; j < ny; j += pfhip::MARCH_OWNED)
+                cs.push_back({(int)s, j - 1, i0, std::min(i0 + rows, nx)});
+          }
+          g.count = (int)cs.size() - g.first;
+          if (g.count > 0) groups_.push_back(g);
         }
-      chunk_sim_ = arena_.upload(cs.data(), cs.size(), stream_);
-      chunk_first_ = arena_.upload(cf.data(), cf.size(), stream_);
+      chunks_ = arena_.upload(cs.data(), cs.size(), stream_);
       n_chunks_ = (int)cs.size();
     }
     args_.sims = sims_dev_;
-    args_.chunk_sim = chunk_sim_;
-    args_.chunk_first = chunk_first_;
+    args_.chunks = chunks_;
     args_.n_chunks = n_chunks_;
-    args_.chunk_cells = chunk_cells_;
     args_.step = step_dev_;
+    args_.bar = bar_;
+    args_.fail = fail_;
+    args_.wait_ticks = wait_ticks_;
+  }
+
+  void reset() override { reset_all(); }
+
+  // Workgroups of the persistent launch: at most half of what the occupancy
+  // calculator says fits at once, so all of them are resident together.
+  void persistent_grid(int waves_per_block, int* per_cu, int* n_cu, int* grid) override {
+    HIP_CHECK(pfhip::batch_persistent_capacity<real>(exact_, waves_per_block, per_cu, n_cu));
+    if (*per_cu < 1 || *n_cu < 1) throw HipError{PF_E_STATE, "persistent kernel does not fit a CU"};
+    const int need = (n_chunks_ + waves_per_block - 1) / waves_per_block;
+    *grid = std::max(1, std::min(std::max(1, *per_cu * *n_cu / 2), need));
   }
 
   void advance(int64_t n) override {
     if (n < 0 || step_ + n > n_steps_max_) throw ArgError{"step range exceeds the batch"};
     HIP_CHECK(hipMemcpyAsync(step_dev_, &step_, sizeof(int64_t), hipMemcpyHostToDevice, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
-    const int grid = std::min(n_chunks_, 4096);
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
     if (mode_ == 1) {
-      HIP_CHECK(pfhip::launch_batch_persistent<real>(args_, exact_, block_, step_, step_ + n, stream_));
+      int per_cu = 0, n_cu = 0, grid = 0;
+      persistent_grid(wpb_, &per_cu, &n_cu, &grid);
+      pfhip::launch_batch_sync_reset(bar_, fail_, stream_);
+      pfhip::launch_batch_persistent<real>(args_, exact_, wpb_, grid, step_, step_ + n, stream_);
     } else {
       int64_t done = 0;
       if (mode_ == 2) {
-        if (!graph_exec_) build_graph(grid);
+        if (!graph_exec_) build_graph();
         for (; done + graph_steps_ <= n; done += graph_steps_)
           HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
       }
-      for (; done < n; ++done) {
-        pfhip::launch_batch_phase<real>(args_, 0, exact_, grid, block_, stream_);
-        pfhip::launch_batch_phase<real>(args_, 1, exact_, grid, block_, stream_);
-        pfhip::launch_batch_tick(step_dev_, stream_);
-      }
+      for (; done < n; ++done) launch_step();
     }
+    pfhip::launch_batch_dft_flush<real>(args_, step_ + n, stream_);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
     float t = 0.f;
     HIP_CHECK(hipEventElapsedTime(&t, ev_[0], ev_[1]));
     last_ms_ = t;
+    if (mode_ == 1) {
+      int failed = 0;
+      HIP_CHECK(hipMemcpy(&failed, fail_, sizeof(int), hipMemcpyDeviceToHost));
+      if (failed)
+        throw HipError{PF_E_STATE, "persistent launch: a grid barrier wait expired (workgroups "
+                                   "were not all resident); the batch state is undefined, reset it"};
+    }
     step_ += n;
-  }
-
-  void reset() override { reset_all(); }
-  void persistent_grid(int block, int* per_cu, int* n_cu, int* grid) override {
-    HIP_CHECK(pfhip::batch_persistent_grid<real>(exact_, block, n_chunks_, per_cu, n_cu, grid));
+    for (size_t i = 0; i < sims_.size(); ++i)
+      sims_[i]->set_level((int)(std::min(step_, n_steps_[i]) & 1));
   }
 
  private:
@@ -1186,16 +1314,21 @@ class Batch final : public BatchBase {
     for (auto& s : sims_) s->reset(stream_);
     step_ = 0;
     HIP_CHECK(hipMemsetAsync(step_dev_, 0, sizeof(int64_t), stream_));
+    if (args_.n_dft > 0)
+      HIP_CHECK(hipMemsetAsync(args_.folded, 0, (size_t)args_.n_dft * sizeof(int64_t), stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
-  void build_graph(int grid) {
+  // One step: the march variants, the post pass, the step counter.
+  void launch_step() {
+    for (const pfhip::ChunkGroup& g : groups_)
+      pfhip::launch_batch_march<real>(args_, g, exact_, wpb_, stream_);
+    pfhip::launch_batch_post<real>(args_, stream_);
+    pfhip::launch_batch_tick(step_dev_, stream_);
+  }
+  void build_graph() {
     hipGraph_t g;
     HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < graph_steps_; ++s) {
-      pfhip::launch_batch_phase<real>(args_, 0, exact_, grid, block_, stream_);
-      pfhip::launch_batch_phase<real>(args_, 1, exact_, grid, block_, stream_);
-      pfhip::launch_batch_tick(step_dev_, stream_);
-    }
+    for (int s = 0; s < graph_steps_; ++s) launch_step();
     HIP_CHECK(hipStreamEndCapture(stream_, &g));
     HIP_CHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(g));
@@ -1206,15 +1339,22 @@ class Batch final : public BatchBase {
   }
 
   std::vector<std::unique_ptr<Stepper<real>>> sims_;
+  std::vector<int64_t> n_steps_;
+  std::vector<std::array<int, 2>> shapes_;
+  std::vector<int> pols_;
+  std::vector<bool> hsrc_;
+  std::vector<pfhip::ChunkGroup> groups_;
   DeviceArena arena_;
   hipStream_t stream_ = nullptr;
   pfhip::SimDev<real>* sims_dev_ = nullptr;
-  int* chunk_sim_ = nullptr;
-  int* chunk_first_ = nullptr;
-  int n_chunks_ = 0, chunk_cells_ = 0;
+  pfhip::MarchChunk* chunks_ = nullptr;
+  int n_chunks_ = 0, rows_ = 0;
   int64_t* step_dev_ = nullptr;
+  unsigned* bar_ = nullptr;
+  int* fail_ = nullptr;
+  uint64_t wait_ticks_ = 0;
   int64_t step_ = 0, n_steps_max_ = 0;
-  int mode_ = 1, block_ = 256;
+  int mode_ = 2, wpb_ = 4;
   bool exact_;
   int graph_steps_ = 50;
   hipGraphExec_t graph_exec_ = nullptr;
@@ -1336,9 +1476,9 @@ int pf_batch_destroy(pf_batch* b) {
   return guarded([&] { delete b; });
 }
 
-int pf_batch_configure(pf_batch* b, int32_t mode, int32_t block, int32_t chunk) {
+int pf_batch_configure(pf_batch* b, int32_t mode, int32_t waves_per_block, int32_t rows) {
   if (!b) return fail(PF_E_ARG, "null batch");
-  return guarded([&] { b->impl->configure(mode, block, chunk); });
+  return guarded([&] { b->impl->configure(mode, waves_per_block, rows); });
 }
 
 int pf_batch_reset(pf_batch* b) {
@@ -1346,10 +1486,10 @@ int pf_batch_reset(pf_batch* b) {
   return guarded([&] { b->impl->reset(); });
 }
 
-int pf_batch_persistent_grid(pf_batch* b, int32_t block, int32_t* per_cu, int32_t* n_cu,
+int pf_batch_persistent_grid(pf_batch* b, int32_t waves_per_block, int32_t* per_cu, int32_t* n_cu,
                              int32_t* grid) {
   if (!b || !per_cu || !n_cu || !grid) return fail(PF_E_ARG, "null argument");
-  return guarded([&] { b->impl->persistent_grid(block, per_cu, n_cu, grid); });
+  return guarded([&] { b->impl->persistent_grid(waves_per_block, per_cu, n_cu, grid); });
 }
 
 int pf_batch_advance(pf_batch* b, int64_t n) {

@@ -132,8 +132,7 @@ struct SourceGroupsDev {
 struct DftDev {
   int n_comp;
   int comps[6];
-  BoxDev box;
-  int lo[2], hi[2];       // first and last (i, j) node of the box
+  int64_t nbox;
   int n_freq, interval;
   const double* phase_e;
   const double* phase_h;
@@ -154,52 +153,96 @@ struct AdeDev {
 constexpr int BATCH_MAX_DFT = 8;
 constexpr int BATCH_MAX_ADE = 8;
 
-// One simulation of a batch: an xy-plane grid (nz = 1), in-place fields.
+// One simulation of a batch: an xy-plane grid (nz = 1) with two time levels
+// of fields and of the H-side CPML psi (the march reads level p and writes
+// level p ^ 1, p = step & 1); E-side psi and ADE state are updated in place.
 template <typename real>
 struct SimDev {
-  real* f[6];
+  real* f[2][6];
+  real* psiH[2][6];       // H-side psi slots (PSI_HXY - 6 ...) per level
+  real* psiE[6];          // E-side psi slots (PSI_EXY ...)
   const real* ce;
-  real* psi[PSI_COUNT];
-  const real* b_e[3];
-  const real* c_e[3];
-  const real* b_h[3];
-  const real* c_h[3];
-  const int* map_e[3];
-  const int* map_h[3];
-  int pe[3], ph[3];
+  const real* b_e[2];
+  const real* c_e[2];
+  const real* b_h[2];
+  const real* c_h[2];
+  int slab_e[2][2], slab_h[2][2];   // CPML nodes of x / y: [0, lo) U [hi, n)
+  int pe[2], ph[2];                 // compact CPML extent per axis
   int nx, ny;
-  real d[3], inv_d[3], ch;
+  real d[2], inv_d[2], ch;
   int pol;                // 0: TE (Ex, Ey, Hz), 1: TM (Ez, Hx, Hy)
   int64_t n_steps;
   SourceGroupsDev src_e, src_h;
-  int64_t src_lo[2], src_hi[2];   // range of source cells per side (0: E, 1: H)
-  const int* ade_code;    // per node: (medium << 26) | index, or -1
-  int n_ade;
+  int64_t src_lo, src_hi; // range of H-side source cells
   AdeDev<real> ade[BATCH_MAX_ADE];
-  int n_dft;
   DftDev dft[BATCH_MAX_DFT];
 };
+
+// Work of one wavefront: lane l covers column j0 + l (lanes 1..62 are owned,
+// lanes 0 and 63 only supply neighbours), rows [i0, i1) are owned and row
+// i0 - 1 is recomputed for its H.
+struct MarchChunk {
+  int sim, j0, i0, i1;
+};
+constexpr int MARCH_OWNED = 62;
+
+// A node that needs work after the curl update, in the reference's order:
+// ADE correction (code >= 0: (medium << 26) | index), E-side soft sources
+// (source groups [src0, src1) of src_list), DFT samples (dft entries
+// [dft0, dft1)).
+struct PostEntry {
+  int sim, cell, ade_code;
+  int src0, src1, dft0, dft1;
+};
+
+// One DFT sample stream: component `comp` of monitor `mon` at box index r.
+struct DftEntry {
+  int mon, slot, comp;
+  int64_t r;
+};
+
+constexpr int BATCH_DFT_RING = 32;   // samples buffered per DFT entry before a fold
 
 template <typename real>
 struct BatchArgs {
   const SimDev<real>* sims;
-  const int* chunk_sim;
-  const int* chunk_first;
+  const MarchChunk* chunks;
   int n_chunks;
-  int chunk_cells;
-  const int64_t* step;    // device step counter (per-phase kernels)
+  const PostEntry* post;
+  int n_post;
+  const int* src_list;
+  const DftEntry* dft;
+  int n_dft;
+  const int* dft_sim;      // simulation of each DFT entry
+  const int* dft_cell;     // cell of each DFT entry
+  real* ring;              // [BATCH_DFT_RING][n_dft]
+  int64_t* folded;         // samples folded into acc, per DFT entry
+  const int64_t* step;     // device step counter (per-step launches)
+  unsigned* bar;           // persistent mode: [arrived, generation]
+  int* fail;               // persistent mode: set when a barrier wait expired
+  uint64_t wait_ticks;     // persistent mode: bound of one barrier wait (wall clock)
+};
+
+// Chunks [first, first + count) of one kernel variant: polarisation and
+// whether the simulation has H-side sources.
+struct ChunkGroup {
+  int first, count, pol, hsrc;
 };
 
 template <typename real>
-void launch_batch_phase(const BatchArgs<real>& a, int phase, bool exact, int grid, int block,
-                        hipStream_t s);
+void launch_batch_march(const BatchArgs<real>& a, const ChunkGroup& g, bool exact,
+                        int waves_per_block, hipStream_t s);
+template <typename real>
+void launch_batch_post(const BatchArgs<real>& a, hipStream_t s);
+template <typename real>
+void launch_batch_dft_flush(const BatchArgs<real>& a, int64_t step_end, hipStream_t s);
 void launch_batch_tick(int64_t* step, hipStream_t s);
+void launch_batch_sync_reset(unsigned* bar, int* fail, hipStream_t s);
 template <typename real>
-hipError_t launch_batch_persistent(BatchArgs<real>& a, bool exact, int block, int64_t step0,
-                                   int64_t step1, hipStream_t s);
+void launch_batch_persistent(const BatchArgs<real>& a, bool exact, int waves_per_block, int blocks,
+                             int64_t step0, int64_t step1, hipStream_t s);
 template <typename real>
-hipError_t batch_persistent_grid(bool exact, int block, int n_chunks, int* per_cu, int* n_cu,
-                                 int* grid);
+hipError_t batch_persistent_capacity(bool exact, int waves_per_block, int* per_cu, int* n_cu);
 
 template <typename real> void launch_generic_h(const StepArgs<real>& a, bool exact, hipStream_t s);
 template <typename real> void launch_generic_e(const StepArgs<real>& a, bool exact, hipStream_t s);

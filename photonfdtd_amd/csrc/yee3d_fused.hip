@@ -17,6 +17,7 @@
 // runs stay bit-identical to the generic kernels; otherwise differences are
 // scaled by 1/cell size.
 #include "pfhip_internal.h"
+#include "yee_common.h"
 
 namespace pfhip {
 
@@ -41,34 +42,6 @@ __device__ __forceinline__ int64_t pidx_z(int i, int j, int m, int ny, int pz) {
   return ((int64_t)i * ny + j) * pz + m;
 }
 
-// One curl component: f + K (dA - dB) with the PML slab terms after the bulk
-// term (same order as the generic kernels).
-template <typename real>
-__device__ __forceinline__ real curl_update(real f, real K, bool onA, real dA, bool pmlA,
-                                            real psiA, real bA, real cA, real* newA, bool onB,
-                                            real dB, bool pmlB, real psiB, real bB, real cB,
-                                            real* newB) {
-  if (!onA && !onB) return f;
-  real cb;
-  if (onA) {
-    cb = dA;
-    if (onB) cb = cb - dB;
-  } else {
-    cb = -dB;
-  }
-  f = f + cb * K;
-  if (onA && pmlA) {
-    const real p = bA * psiA + cA * dA;
-    *newA = p;
-    f = f + K * p;
-  }
-  if (onB && pmlB) {
-    const real p = bB * psiB + cB * dB;
-    *newB = p;
-    f = f - K * p;
-  }
-  return f;
-}
 
 }  // namespace
 

@@ -50,13 +50,8 @@ def _has_ade(sim):
     return any(type(s.medium).__name__ == "DispersiveMedium" for s in sim.structures)
 
 
-PERSISTENT_OFF = pytest.mark.skip(reason="cooperative persistent mode disabled: a full-occupancy "
-                                          "launch went silent on a large batch")
-
-
 @pytest.mark.parametrize("precision", ["float32", "float64"])
-@pytest.mark.parametrize("mode", ["phases", pytest.param("persistent", marks=PERSISTENT_OFF),
-                                  "graph"])
+@pytest.mark.parametrize("mode", ["stream", "persistent", "graph"])
 def test_te_batch_matches_single_runs(precision, mode):
     from photonfdtd_amd.batch import SimBatch
     sims = _scenes(precision, "Ex", n=4, seed=1)
@@ -64,7 +59,7 @@ def test_te_batch_matches_single_runs(precision, mode):
     assert any(_has_ade(s) for s in sims)
     print(f"\n[case] TE batch {precision} {mode} shapes={[s.grid.shape[:2] for s in sims]}",
           flush=True)
-    b = SimBatch(sims, mode=mode, chunk=1024)
+    b = SimBatch(sims, mode=mode, waves=2, rows=16)
     try:
         b.advance()
         for i, s in enumerate(sims):
@@ -73,13 +68,13 @@ def test_te_batch_matches_single_runs(precision, mode):
         b.close()
 
 
-@pytest.mark.parametrize("mode", [pytest.param("persistent", marks=PERSISTENT_OFF), "graph"])
+@pytest.mark.parametrize("mode", ["persistent", "graph"])
 def test_tm_batch_matches_single_runs(mode):
     from photonfdtd_amd.batch import SimBatch
     sims = _scenes("float32", "Ez", n=4, seed=5)
     assert any(_has_ade(s) for s in sims)
     print(f"\n[case] TM batch {mode} shapes={[s.grid.shape[:2] for s in sims]}", flush=True)
-    b = SimBatch(sims, mode=mode, chunk=2048, block=512)
+    b = SimBatch(sims, mode=mode, waves=8, rows=7)
     try:
         b.advance(120)
         b.advance()                       # in two pieces
@@ -93,7 +88,7 @@ def test_mixed_polarisations_in_one_batch():
     from photonfdtd_amd.batch import SimBatch
     sims = _scenes("float32", "Ex", n=2, seed=7) + _scenes("float32", "Ez", n=2, seed=8)
     print("\n[case] mixed TE/TM batch", flush=True)
-    b = SimBatch(sims, mode="phases")
+    b = SimBatch(sims, mode="stream", waves=1, rows=128)
     try:
         b.advance()
         for i, s in enumerate(sims):
@@ -108,9 +103,9 @@ def test_tune_resets_state():
     print("\n[case] tune then run", flush=True)
     b = SimBatch(sims)
     try:
-        best = b.tune(trial_steps=60, log=lambda m: print(m, flush=True))
-        assert best["mode"] in ("graph", "phases") and best["ms_per_step"] > 0
-        assert len(best["trials"]) == 12
+        best = b.tune(trial_steps=60, rows=(16, 64), log=lambda m: print(m, flush=True))
+        assert best["mode"] in ("graph", "stream", "persistent") and best["ms_per_step"] > 0
+        assert len(best["trials"]) == 18
         b.advance()
         for i, s in enumerate(sims):
             _assert_same(b, i, s, "tuned")
