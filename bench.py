@@ -173,11 +173,11 @@ def parity_and_cpu(n=48, steps=600):
 PROFILES = ROOT / "profiles"
 
 
-def measured_traffic(n, precision):
+def measured_traffic(name):
     """HBM bytes per launch of the step kernel from the committed counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
     the counter reads for the full-field gather copy), or None."""
-    f = PROFILES / f"dispersive_slab_{n}_{precision}_traffic.json"
+    f = PROFILES / f"{name}_traffic.json"
     if not f.exists():
         return None
     return json.loads(f.read_text()).get("hbm_bytes_per_launch")
@@ -213,11 +213,12 @@ def summarize(prof_dir, kind, out=None):
     return text
 
 
-def traffic_from_counters(fetch_dir, write_dir, n=512, precision="float32"):
-    """HBM bytes per launch of the step kernel from one FETCH_SIZE and one
-    WRITE_SIZE rocprofv3 run of this benchmark. Each counter is scaled by its
-    reading for the full-field gather (a copy of n^3 values each way) of the
-    same run. Writes profiles/dispersive_slab_<n>_<precision>_traffic.json."""
+def traffic_from_counters(fetch_dir, write_dir, kernel, copy_cells, alg_bytes, itemsize, name,
+                          workload):
+    """HBM bytes per launch of `kernel` from one FETCH_SIZE and one WRITE_SIZE
+    rocprofv3 run. Each counter is scaled by its reading for the full-field
+    k_gather_box of the same run (a copy of copy_cells values each way).
+    Writes profiles/<name>_traffic.json."""
     import csv
     import glob
 
@@ -225,30 +226,26 @@ def traffic_from_counters(fetch_dir, write_dir, n=512, precision="float32"):
         path = sorted(glob.glob(str(Path(d) / "**" / "*counter_collection.csv"), recursive=True))[0]
         return [(r["Kernel_Name"], float(r["Counter_Value"]), int(r["Grid_Size"]))
                 for r in csv.DictReader(open(path))]
-    cells = n ** 3
-    item = 4 if precision == "float32" else 8
-    true_kib = cells * item / 1024
+    true_kib = copy_cells * itemsize / 1024
     fe, wr = dispatches(fetch_dir), dispatches(write_dir)
-    copy_f = [v for k, v, g in fe if "k_gather_box" in k and g == cells][0]
-    copy_w = [v for k, v, g in wr if "k_gather_box" in k and g == cells][0]
-    step_f = [v for k, v, g in fe if "k_yee_march" in k]
-    step_w = [v for k, v, g in wr if "k_yee_march" in k]
+    copy_f = [v for k, v, g in fe if "k_gather_box" in k and g >= copy_cells][0]
+    copy_w = [v for k, v, g in wr if "k_gather_box" in k and g >= copy_cells][0]
+    step_f = [v for k, v, g in fe if kernel in k]
+    step_w = [v for k, v, g in wr if kernel in k]
     fetch = sum(step_f) / len(step_f) * (true_kib / copy_f) * 1024
     write = sum(step_w) / len(step_w) * (true_kib / copy_w) * 1024
-    alg = cells * 13 * item
-    out = {"kernel": "k_yee_march", "workload": f"3-D dispersive slab {n}^3 {precision}",
+    out = {"kernel": kernel, "workload": workload,
            "fetch_kib_raw_mean": sum(step_f) / len(step_f),
            "write_kib_raw_mean": sum(step_w) / len(step_w),
-           "copy_calibration": {"kernel": f"k_gather_box full field ({cells} values each way)",
+           "copy_calibration": {"kernel": f"k_gather_box full field ({copy_cells} values each way)",
                                 "fetch_kib_raw": copy_f, "write_kib_raw": copy_w,
                                 "true_kib": true_kib, "fetch_factor": true_kib / copy_f,
                                 "write_factor": true_kib / copy_w},
            "hbm_bytes_per_launch": round(fetch + write), "fetch_bytes": round(fetch),
-           "write_bytes": round(write), "algorithmic_bytes_per_launch": alg,
-           "traffic_over_algorithmic": (fetch + write) / alg}
+           "write_bytes": round(write), "algorithmic_bytes_per_launch": alg_bytes,
+           "traffic_over_algorithmic": (fetch + write) / alg_bytes}
     PROFILES.mkdir(exist_ok=True)
-    (PROFILES / f"dispersive_slab_{n}_{precision}_traffic.json").write_text(
-        json.dumps(out, indent=1) + "\n")
+    (PROFILES / f"{name}_traffic.json").write_text(json.dumps(out, indent=1) + "\n")
     print(json.dumps(out, indent=1))
 
 
@@ -307,8 +304,13 @@ def bench_batched(args):
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
-    tuned = b.tune(trial_steps=min(100, total), modes=args.batch_modes.split(","), log=log)
-    trials = tuned.pop("trials")
+    if args.batch_config:
+        m, w, r = args.batch_config.split(",")
+        b.configure(m, int(w), int(r))
+        tuned, trials = dict(mode=m, waves=int(w), rows=int(r)), []
+    else:
+        tuned = b.tune(trial_steps=min(100, total), modes=args.batch_modes.split(","), log=log)
+        trials = tuned.pop("trials")
     print(f"[bench] tuned {tuned}", file=sys.stderr, flush=True)
     b.advance(args.warmup)
     t0 = time.perf_counter()
@@ -316,6 +318,10 @@ def bench_batched(args):
     wall = time.perf_counter() - t0
     dev_ms = b.last_ms()
     finite = all(np.isfinite(v).all() for i in range(n_sims) for v in b.dft(i, "out").values())
+    # one full-field read of the largest scene: a plain copy kernel (k_gather_box) of known size
+    # that calibrates the FETCH_SIZE / WRITE_SIZE counters
+    big = int(np.argmax([s.grid.n_cells for s in sims]))
+    b.field(big, "Ey")
     b.close()
     itemsize = 4 if args.precision == "float32" else 8
     bytes_per_update = 7 * itemsize          # TE: read Ex, Ey, Hz, ce; write Ex, Ey, Hz
@@ -341,7 +347,9 @@ def bench_batched(args):
                    "device": device_info()[0]},
         "roofline": {"bound": "hbm", "kernel": f"k_batch_march ({tuned['mode']})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (measured_traffic(f"batched2d_te_{args.precision}")
+                                 if tuned["mode"] != "persistent" else None),
                      "bytes_per_cell_update": bytes_per_update,
                      "kernel_ms_per_step": round(dev_ms / args.steps, 4)},
         "dft_finite": bool(finite),
@@ -364,6 +372,8 @@ def main():
     ap.add_argument("--workload", default="slab3d", choices=("slab3d", "batched2d"),
                     help="slab3d: the 3-D dispersive slab (headline); batched2d: 64 uneven "
                          "2-D TE scenes in one launch")
+    ap.add_argument("--batch-config", default=None, metavar="MODE,WAVES,ROWS",
+                    help="fixed batched launch setting instead of the autotuner")
     ap.add_argument("--batch-modes", default="graph,stream,persistent",
                     help="launch modes the batched autotuner may pick from")
     ap.add_argument("--precision", default="float32", choices=("float32", "float64"))
@@ -377,7 +387,23 @@ def main():
         summarize(*args.summarize)
         return
     if args.traffic:
-        traffic_from_counters(*args.traffic, n=args.n, precision=args.precision)
+        item = 4 if args.precision == "float32" else 8
+        if args.workload == "batched2d":
+            import photonfdtd_amd as pf
+            sims = batched_2d_scenes(pf, n_sims=64, steps=10, seed=0)
+            cells = [s.grid.n_cells for s in sims]
+            # one launch of the TE march variant covers every scene of this workload
+            traffic_from_counters(*args.traffic, kernel="k_batch_march", copy_cells=max(cells),
+                                  alg_bytes=sum(cells) * 7 * item, itemsize=item,
+                                  name=f"batched2d_te_{args.precision}",
+                                  workload=f"batched 2-D TE, 64 scenes, {sum(cells)} cells, "
+                                           f"{args.precision}")
+        else:
+            n = args.n
+            traffic_from_counters(*args.traffic, kernel="k_yee_march", copy_cells=n ** 3,
+                                  alg_bytes=n ** 3 * 13 * item, itemsize=item,
+                                  name=f"dispersive_slab_{n}_{args.precision}",
+                                  workload=f"3-D dispersive slab {n}^3 {args.precision}")
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this stepper runs on one GPU; multi-GPU runs are not "
@@ -455,7 +481,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.n, args.precision),
+                     "traffic": measured_traffic(f"dispersive_slab_{args.n}_{args.precision}"),
                      "bytes_per_cell_update": bytes_per_update,
                      "kernel_ms_per_step": round(step_ms, 4)},
         "kernel_ms": {name: {"total_ms": round(t, 3), "launch_groups": int(c)}

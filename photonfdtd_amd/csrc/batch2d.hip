@@ -330,27 +330,32 @@ __device__ __forceinline__ void march_chunk(const BatchArgs<real>& a, const Marc
   else march_tm<real, EXACT, HSRC>(s, c, p, step);
 }
 
-// Fold DFT samples q0..q1 of entry e from the ring into its accumulators:
-// acc += (phase * v) * dt per sample in order (simulation.py:1116-1133).
+// Fold the DFT samples of steps [s_lo, s_hi) (clamped to the simulation's
+// own step count) of entry e at frequency f from the ring into its
+// accumulator: acc += (phase * v) * dt per sample in step order
+// (simulation.py:1116-1133).
 template <typename real>
-__device__ void dft_fold(const BatchArgs<real>& a, const SimDev<real>& s, int e, int64_t q0,
-                         int64_t q1) {
+__device__ void dft_fold(const BatchArgs<real>& a, int e, int f, int64_t s_lo, int64_t s_hi) {
+  const SimDev<real>& s = a.sims[a.dft_sim[e]];
   const DftEntry D = a.dft[e];
   const DftDev& d = s.dft[D.mon];
-  const double* ph = D.comp >= 3 ? d.phase_h : d.phase_e;
-  double* A = d.acc + 2 * ((int64_t)D.slot * d.n_freq * d.nbox + D.r);
-  for (int f = 0; f < d.n_freq; ++f) {
-    double* acc = A + 2 * (int64_t)f * d.nbox;
-    double re = acc[0], im = acc[1];
-    for (int64_t q = q0; q <= q1; ++q) {
-      const double v = (double)a.ring[(q % BATCH_DFT_RING) * a.n_dft + e];
-      const double* pq = ph + 2 * (q * d.n_freq + f);
-      re = re + (pq[0] * v) * d.dt;
-      im = im + (pq[1] * v) * d.dt;
-    }
-    acc[0] = re;
-    acc[1] = im;
+  if (f >= d.n_freq) return;
+  const int64_t hi = min(s_hi, s.n_steps);
+  if (hi <= s_lo) return;
+  const int64_t iv = d.interval;
+  const int64_t q0 = (s_lo + iv - 1) / iv, q1 = (hi - 1) / iv;
+  if (q1 < q0) return;
+  const double* ph = (D.comp >= 3 ? d.phase_h : d.phase_e) + 2 * f;
+  double* acc = d.acc + 2 * (((int64_t)D.slot * d.n_freq + f) * d.nbox + D.r);
+  double re = acc[0], im = acc[1];
+  for (int64_t q = q0; q <= q1; ++q) {
+    const double v = (double)a.ring[(q % BATCH_DFT_RING) * a.n_dft + e];
+    const double* pq = ph + 2 * q * d.n_freq;
+    re = re + (pq[0] * v) * d.dt;
+    im = im + (pq[1] * v) * d.dt;
   }
+  acc[0] = re;
+  acc[1] = im;
 }
 
 // Post pass of one node: ADE correction of the curl-updated E (E^n from
@@ -380,10 +385,6 @@ __device__ void post_node(const BatchArgs<real>& a, int idx, int64_t step) {
     if (interval > 1 && step % interval != 0) continue;
     const int64_t q = step / interval;
     a.ring[(q % BATCH_DFT_RING) * a.n_dft + e] = s.f[po][D.comp][cell];
-    if (q + 1 - a.folded[e] == BATCH_DFT_RING) {
-      dft_fold(a, s, e, a.folded[e], q);
-      a.folded[e] = q + 1;
-    }
   }
 }
 
@@ -435,7 +436,7 @@ __device__ bool grid_barrier(unsigned* bar, int* fail, uint64_t wait_ticks) {
 
 // One time step of the chunks [first, first + count) (one polarisation and
 // H-source variant); the step is read from a device counter so a captured
-// graph of several steps can be replayed (k_batch_tick advances it).
+// graph of several steps can be replayed (the post pass advances it).
 template <typename real, bool EXACT, int POL, bool HSRC>
 __global__ __launch_bounds__(512) void k_batch_march(BatchArgs<real> a, int first, int count) {
   const int w = wave_id();
@@ -443,30 +444,40 @@ __global__ __launch_bounds__(512) void k_batch_march(BatchArgs<real> a, int firs
   march_chunk<real, EXACT, POL, HSRC>(a, a.chunks[first + w], *a.step);
 }
 
+// Post pass of one step. The last workgroup to finish advances the device
+// step counter (every workgroup has read it by then), so no separate launch
+// is needed for it.
 template <typename real>
 __global__ __launch_bounds__(256) void k_batch_post(BatchArgs<real> a) {
+  __shared__ int last;
+  const int64_t step = *a.step;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < a.n_post) post_node(a, idx, *a.step);
-}
-
-// Fold the DFT samples still in the ring after an advance that ended at
-// step_end.
-template <typename real>
-__global__ __launch_bounds__(256) void k_batch_dft_flush(BatchArgs<real> a, int64_t step_end) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.n_dft) return;
-  const SimDev<real>& s = a.sims[a.dft_sim[e]];
-  const int64_t last = min(step_end, s.n_steps) - 1;
-  if (last < 0) return;
-  const int64_t q_last = last / s.dft[a.dft[e].mon].interval;
-  if (q_last + 1 > a.folded[e]) {
-    dft_fold(a, s, e, a.folded[e], q_last);
-    a.folded[e] = q_last + 1;
+  if (idx < a.n_post) post_node(a, idx, step);
+  // Every thread of this workgroup has read the counter before the barrier,
+  // and every workgroup counts itself only after that, so the last one may
+  // advance it. The next launch sees the new value at the kernel boundary,
+  // so relaxed atomics suffice (no cache write-back per workgroup).
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned done =
+        __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last = done == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a.step, step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-__global__ void k_batch_tick(int64_t* step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *step = *step + 1;
+// Fold the ring samples of the last `span` steps (the device counter holds
+// the number of steps done): one thread per (DFT entry, frequency).
+template <typename real>
+__global__ __launch_bounds__(256) void k_batch_dft_fold(BatchArgs<real> a, int span, int n_freq) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)a.n_dft * n_freq) return;
+  const int64_t s_hi = *a.step;
+  dft_fold(a, (int)(t / n_freq), (int)(t % n_freq), s_hi - span, s_hi);
 }
 
 __global__ void k_batch_sync_reset(unsigned* bar, int* fail) {
@@ -477,16 +488,18 @@ __global__ void k_batch_sync_reset(unsigned* bar, int* fail) {
 }
 
 // Persistent kernel: steps [step0, step1) in one launch; per step the march
-// over all chunks, a grid barrier, the post pass, a grid barrier. Every
+// over all chunks, a grid barrier, the post pass, a grid barrier; every
+// BATCH_DFT_RING steps and at the end the DFT fold and a grid barrier. Every
 // wavefront loops over its share of the chunks, every thread over its share
-// of the post nodes.
+// of the post nodes and fold items. The device step counter is not used.
 template <typename real, bool EXACT>
 __global__ __launch_bounds__(512) void k_batch_persistent(BatchArgs<real> a, int64_t step0,
-                                                          int64_t step1) {
+                                                          int64_t step1, int n_freq) {
   const int wave = wave_id();
   const int waves = (int)(gridDim.x * blockDim.x) >> 6;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int threads = gridDim.x * blockDim.x;
+  int64_t folded = step0;
   for (int64_t st = step0; st < step1; ++st) {
     for (int ci = wave; ci < a.n_chunks; ci += waves) {
       const MarchChunk c = a.chunks[ci];
@@ -500,9 +513,15 @@ __global__ __launch_bounds__(512) void k_batch_persistent(BatchArgs<real> a, int
         else march_chunk<real, EXACT, 1, false>(a, c, st);
       }
     }
-    if (!grid_barrier(a.bar, a.fail, a.wait_ticks)) break;
+    if (!grid_barrier(a.bar, a.fail, a.wait_ticks)) return;
     for (int idx = tid; idx < a.n_post; idx += threads) post_node(a, idx, st);
-    if (!grid_barrier(a.bar, a.fail, a.wait_ticks)) break;
+    if (!grid_barrier(a.bar, a.fail, a.wait_ticks)) return;
+    if (st + 1 - folded == BATCH_DFT_RING || st + 1 == step1) {
+      for (int64_t t = tid; t < (int64_t)a.n_dft * n_freq; t += threads)
+        dft_fold(a, (int)(t / n_freq), (int)(t % n_freq), folded, st + 1);
+      folded = st + 1;
+      if (!grid_barrier(a.bar, a.fail, a.wait_ticks)) return;
+    }
   }
 }
 
@@ -526,15 +545,15 @@ void launch_batch_march(const BatchArgs<real>& a, const ChunkGroup& g, bool exac
 
 template <typename real>
 void launch_batch_post(const BatchArgs<real>& a, hipStream_t s) {
-  if (a.n_post > 0) k_batch_post<real><<<(a.n_post + 255) / 256, 256, 0, s>>>(a);
+  k_batch_post<real><<<std::max(1, (a.n_post + 255) / 256), 256, 0, s>>>(a);
 }
 
 template <typename real>
-void launch_batch_dft_flush(const BatchArgs<real>& a, int64_t step_end, hipStream_t s) {
-  if (a.n_dft > 0) k_batch_dft_flush<real><<<(a.n_dft + 255) / 256, 256, 0, s>>>(a, step_end);
+void launch_batch_dft_fold(const BatchArgs<real>& a, int span, int n_freq, hipStream_t s) {
+  const int64_t n = (int64_t)a.n_dft * n_freq;
+  if (n > 0 && span > 0)
+    k_batch_dft_fold<real><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, span, n_freq);
 }
-
-void launch_batch_tick(int64_t* step, hipStream_t s) { k_batch_tick<<<1, 64, 0, s>>>(step); }
 
 void launch_batch_sync_reset(unsigned* bar, int* fail, hipStream_t s) {
   k_batch_sync_reset<<<1, 64, 0, s>>>(bar, fail);
@@ -556,11 +575,11 @@ hipError_t batch_persistent_capacity(bool exact, int waves_per_block, int* per_c
 
 template <typename real>
 void launch_batch_persistent(const BatchArgs<real>& a, bool exact, int waves_per_block, int blocks,
-                             int64_t step0, int64_t step1, hipStream_t s) {
+                             int64_t step0, int64_t step1, int n_freq, hipStream_t s) {
   if (exact)
-    k_batch_persistent<real, true><<<blocks, waves_per_block * 64, 0, s>>>(a, step0, step1);
+    k_batch_persistent<real, true><<<blocks, waves_per_block * 64, 0, s>>>(a, step0, step1, n_freq);
   else
-    k_batch_persistent<real, false><<<blocks, waves_per_block * 64, 0, s>>>(a, step0, step1);
+    k_batch_persistent<real, false><<<blocks, waves_per_block * 64, 0, s>>>(a, step0, step1, n_freq);
 }
 
 template void launch_batch_march<float>(const BatchArgs<float>&, const ChunkGroup&, bool, int,
@@ -569,12 +588,12 @@ template void launch_batch_march<double>(const BatchArgs<double>&, const ChunkGr
                                          hipStream_t);
 template void launch_batch_post<float>(const BatchArgs<float>&, hipStream_t);
 template void launch_batch_post<double>(const BatchArgs<double>&, hipStream_t);
-template void launch_batch_dft_flush<float>(const BatchArgs<float>&, int64_t, hipStream_t);
-template void launch_batch_dft_flush<double>(const BatchArgs<double>&, int64_t, hipStream_t);
+template void launch_batch_dft_fold<float>(const BatchArgs<float>&, int, int, hipStream_t);
+template void launch_batch_dft_fold<double>(const BatchArgs<double>&, int, int, hipStream_t);
 template void launch_batch_persistent<float>(const BatchArgs<float>&, bool, int, int, int64_t,
-                                             int64_t, hipStream_t);
+                                             int64_t, int, hipStream_t);
 template void launch_batch_persistent<double>(const BatchArgs<double>&, bool, int, int, int64_t,
-                                              int64_t, hipStream_t);
+                                              int64_t, int, hipStream_t);
 template hipError_t batch_persistent_capacity<float>(bool, int, int*, int*);
 template hipError_t batch_persistent_capacity<double>(bool, int, int*, int*);
 

@@ -1190,8 +1190,10 @@ class Batch final : public BatchBase {
     args_.dft_cell = dft.empty() ? nullptr : arena_.upload(dft_cell.data(), dft_cell.size(), stream_);
     args_.ring = dft.empty() ? nullptr
                              : arena_.zeros<real>((size_t)pfhip::BATCH_DFT_RING * dft.size(), stream_);
-    args_.folded = dft.empty() ? nullptr : arena_.zeros<int64_t>(dft.size(), stream_);
+    for (const auto& h : host)
+      for (int m = 0; m < pfhip::BATCH_MAX_DFT; ++m) n_freq_ = std::max(n_freq_, h.dft[m].n_freq);
     step_dev_ = arena_.zeros<int64_t>(1, stream_);
+    done_ = arena_.zeros<unsigned>(1, stream_);
     bar_ = arena_.zeros<unsigned>(2, stream_);
     fail_ = arena_.zeros<int>(1, stream_);
     int dev = 0, khz = 0;
@@ -1222,7 +1224,7 @@ class Batch final : public BatchBase {
 
   // mode 0: one march launch per step; 1: one persistent launch for all
   // steps of an advance; 2: march launches replayed from a captured graph of
-  // graph_steps_ steps. rows: owned rows per wavefront chunk.
+  // BATCH_DFT_RING steps and their DFT fold. rows: owned rows per wavefront chunk.
   void configure(int mode, int waves_per_block, int rows) override {
     if (mode < 0 || mode > 2 || (waves_per_block != 1 && waves_per_block != 2 &&
                                  waves_per_block != 4 && waves_per_block != 8) || rows < 1)
@@ -1255,6 +1257,7 @@ class Batch final : public BatchBase {
     args_.chunks = chunks_;
     args_.n_chunks = n_chunks_;
     args_.step = step_dev_;
+    args_.done = done_;
     args_.bar = bar_;
     args_.fail = fail_;
     args_.wait_ticks = wait_ticks_;
@@ -1280,17 +1283,24 @@ class Batch final : public BatchBase {
       int per_cu = 0, n_cu = 0, grid = 0;
       persistent_grid(wpb_, &per_cu, &n_cu, &grid);
       pfhip::launch_batch_sync_reset(bar_, fail_, stream_);
-      pfhip::launch_batch_persistent<real>(args_, exact_, wpb_, grid, step_, step_ + n, stream_);
+      if (n > 0)
+        pfhip::launch_batch_persistent<real>(args_, exact_, wpb_, grid, step_, step_ + n, n_freq_,
+                                             stream_);
     } else {
       int64_t done = 0;
       if (mode_ == 2) {
         if (!graph_exec_) build_graph();
-        for (; done + graph_steps_ <= n; done += graph_steps_)
+        for (; done + pfhip::BATCH_DFT_RING <= n; done += pfhip::BATCH_DFT_RING)
           HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
       }
-      for (; done < n; ++done) launch_step();
+      // DFT samples are folded at least every BATCH_DFT_RING steps and at the end
+      while (done < n) {
+        const int span = (int)std::min<int64_t>(pfhip::BATCH_DFT_RING, n - done);
+        for (int k = 0; k < span; ++k) launch_step();
+        pfhip::launch_batch_dft_fold<real>(args_, span, n_freq_, stream_);
+        done += span;
+      }
     }
-    pfhip::launch_batch_dft_flush<real>(args_, step_ + n, stream_);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ev_[1], stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1314,21 +1324,22 @@ class Batch final : public BatchBase {
     for (auto& s : sims_) s->reset(stream_);
     step_ = 0;
     HIP_CHECK(hipMemsetAsync(step_dev_, 0, sizeof(int64_t), stream_));
-    if (args_.n_dft > 0)
-      HIP_CHECK(hipMemsetAsync(args_.folded, 0, (size_t)args_.n_dft * sizeof(int64_t), stream_));
+    HIP_CHECK(hipMemsetAsync(done_, 0, sizeof(unsigned), stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
-  // One step: the march variants, the post pass, the step counter.
+  // One step: the march variants, then the post pass (which advances the
+  // device step counter).
   void launch_step() {
     for (const pfhip::ChunkGroup& g : groups_)
       pfhip::launch_batch_march<real>(args_, g, exact_, wpb_, stream_);
     pfhip::launch_batch_post<real>(args_, stream_);
-    pfhip::launch_batch_tick(step_dev_, stream_);
   }
+  // BATCH_DFT_RING steps and the DFT fold of their samples.
   void build_graph() {
     hipGraph_t g;
     HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < graph_steps_; ++s) launch_step();
+    for (int s = 0; s < pfhip::BATCH_DFT_RING; ++s) launch_step();
+    pfhip::launch_batch_dft_fold<real>(args_, pfhip::BATCH_DFT_RING, n_freq_, stream_);
     HIP_CHECK(hipStreamEndCapture(stream_, &g));
     HIP_CHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(g));
@@ -1350,13 +1361,14 @@ class Batch final : public BatchBase {
   pfhip::MarchChunk* chunks_ = nullptr;
   int n_chunks_ = 0, rows_ = 0;
   int64_t* step_dev_ = nullptr;
+  unsigned* done_ = nullptr;
+  int n_freq_ = 0;
   unsigned* bar_ = nullptr;
   int* fail_ = nullptr;
   uint64_t wait_ticks_ = 0;
   int64_t step_ = 0, n_steps_max_ = 0;
   int mode_ = 2, wpb_ = 4;
   bool exact_;
-  int graph_steps_ = 50;
   hipGraphExec_t graph_exec_ = nullptr;
   pfhip::BatchArgs<real> args_{};
   hipEvent_t ev_[2] = {nullptr, nullptr};

@@ -201,7 +201,7 @@ struct DftEntry {
   int64_t r;
 };
 
-constexpr int BATCH_DFT_RING = 32;   // samples buffered per DFT entry before a fold
+constexpr int BATCH_DFT_RING = 32;   // steps of DFT samples buffered before a fold
 
 template <typename real>
 struct BatchArgs {
@@ -216,8 +216,8 @@ struct BatchArgs {
   const int* dft_sim;      // simulation of each DFT entry
   const int* dft_cell;     // cell of each DFT entry
   real* ring;              // [BATCH_DFT_RING][n_dft]
-  int64_t* folded;         // samples folded into acc, per DFT entry
-  const int64_t* step;     // device step counter (per-step launches)
+  int64_t* step;           // device step counter (per-step launches)
+  unsigned* done;          // workgroups of the post pass that finished this step
   unsigned* bar;           // persistent mode: [arrived, generation]
   int* fail;               // persistent mode: set when a barrier wait expired
   uint64_t wait_ticks;     // persistent mode: bound of one barrier wait (wall clock)
@@ -235,12 +235,11 @@ void launch_batch_march(const BatchArgs<real>& a, const ChunkGroup& g, bool exac
 template <typename real>
 void launch_batch_post(const BatchArgs<real>& a, hipStream_t s);
 template <typename real>
-void launch_batch_dft_flush(const BatchArgs<real>& a, int64_t step_end, hipStream_t s);
-void launch_batch_tick(int64_t* step, hipStream_t s);
+void launch_batch_dft_fold(const BatchArgs<real>& a, int span, int n_freq, hipStream_t s);
 void launch_batch_sync_reset(unsigned* bar, int* fail, hipStream_t s);
 template <typename real>
 void launch_batch_persistent(const BatchArgs<real>& a, bool exact, int waves_per_block, int blocks,
-                             int64_t step0, int64_t step1, hipStream_t s);
+                             int64_t step0, int64_t step1, int n_freq, hipStream_t s);
 template <typename real>
 hipError_t batch_persistent_capacity(bool exact, int waves_per_block, int* per_cu, int* n_cu);
 
