@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 3
+#define PFHIP_ABI_VERSION 4
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -192,6 +192,9 @@ int pf_batch_persistent_grid(pf_batch* b, int32_t waves_per_block, int32_t* per_
 int pf_batch_advance(pf_batch* b, int64_t n);
 /* Device time (HIP events on the batch stream) of the last pf_batch_advance. */
 int pf_batch_last_ms(pf_batch* b, double* ms);
+/* Replace ce = dt / (eps_r eps0) of simulation `sim` (nx*ny values of the
+ * batch dtype): a new permittivity for the next run after pf_batch_reset. */
+int pf_batch_write_ce(pf_batch* b, int32_t sim, const void* ce);
 int pf_batch_dft_read(pf_batch* b, int32_t sim, int32_t mon, int32_t slot, double* dst);
 int pf_batch_read_field(pf_batch* b, int32_t sim, int32_t comp, void* dst);
 /* Event timing (no reference counterpart; used by the benchmark): when on,
@@ -206,6 +209,41 @@ int pf_device_info(char* name, int32_t name_len, int64_t* free_mb, int64_t* tota
 const char* pf_last_error(void);
 /* PFHIP_ABI_VERSION of the loaded library (the CUDA_BUILT analogue). */
 int32_t pf_abi_version(void);
+
+/* ---- Learned surrogate kernels (no reference counterpart) -------------
+ * Layer kernels of the spectral surrogate on the matrix cores (BF16
+ * operands, FP32 accumulation). Tensors are device pointers in NHWC order:
+ * activations BF16 (raw 16-bit), gradients FP32, weights BF16 [cout][k][k][cin]
+ * with FP32 master copies kept by the optimizer. `stream` is a hipStream_t
+ * (NULL: the default stream). Calls are asynchronous on that stream and
+ * return PF_E_ARG for bad shapes, PF_E_HIP for launch errors
+ * (pf_nn_last_error gives the text). */
+typedef struct pf_nn_conv {
+  int32_t n, h, w;       /* input batch and spatial size */
+  int32_t cin, cout;
+  int32_t ksize;         /* 1, or 3 with padding 1 */
+  int32_t stride;        /* 1 or 2; output size (h - 1) / stride + 1 */
+  int32_t relu;          /* 1: ReLU after the bias */
+  int32_t out_f32;       /* 1: output stored as FP32, else BF16 */
+} pf_nn_conv;
+int pf_nn_conv_forward(const pf_nn_conv* c, const void* x, const void* w, const float* bias,
+                       void* y, void* stream);
+/* dy: FP32 gradient w.r.t. the layer output y; dx (FP32, may be NULL) is
+ * written; dw and dbias (FP32, may be NULL) are accumulated into. */
+int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void* y,
+                        const float* dy, float* dx, float* dw, float* dbias, void* stream);
+int pf_nn_gap_forward(int32_t n, int32_t hw, int32_t ch, const void* x, void* y, void* stream);
+int pf_nn_gap_backward(int32_t n, int32_t hw, int32_t ch, const float* dy, float* dx,
+                       void* stream);
+/* loss (one FP32, zeroed by the caller) += mean (pred - target)^2; dpred written. */
+int pf_nn_mse(int32_t total, const float* pred, const float* target, float* dpred, float* loss,
+              void* stream);
+/* Adam step `step` (1-based) on FP32 master weights; refreshes the BF16 copy
+ * (may be NULL) and zeroes grad. */
+int pf_nn_adam(int64_t n, float* param, float* grad, float* m, float* v, void* param_bf16,
+               float lr, float beta1, float beta2, float eps, int32_t step, void* stream);
+int pf_nn_to_bf16(int64_t n, const float* x, void* y, void* stream);
+const char* pf_nn_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 PF_FLAG_GENERIC = 1
 
 PF_F32, PF_F64 = 0, 1
@@ -53,6 +53,11 @@ class PfDesc(C.Structure):
                 ("flags", _i32)]
 
 
+class PfNnConv(C.Structure):
+    _fields_ = [("n", _i32), ("h", _i32), ("w", _i32), ("cin", _i32), ("cout", _i32),
+                ("ksize", _i32), ("stride", _i32), ("relu", _i32), ("out_f32", _i32)]
+
+
 class PfCotangent(C.Structure):
     _fields_ = [("mon", _i32), ("slot", _i32), ("g", _P(_f64))]
 
@@ -81,6 +86,16 @@ SIGNATURES = {
     "pf_batch_persistent_grid": (_i32, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32)]),
     "pf_batch_advance": (_i32, [_vp, _i64]),
     "pf_batch_last_ms": (_i32, [_vp, _P(_f64)]),
+    "pf_batch_write_ce": (_i32, [_vp, _i32, _vp]),
+    "pf_nn_conv_forward": (_i32, [_P(PfNnConv), _vp, _vp, _vp, _vp, _vp]),
+    "pf_nn_conv_backward": (_i32, [_P(PfNnConv), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_nn_gap_forward": (_i32, [_i32, _i32, _i32, _vp, _vp, _vp]),
+    "pf_nn_gap_backward": (_i32, [_i32, _i32, _i32, _vp, _vp, _vp]),
+    "pf_nn_mse": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp]),
+    "pf_nn_adam": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float,
+                          C.c_float, _i32, _vp]),
+    "pf_nn_to_bf16": (_i32, [_i64, _vp, _vp, _vp]),
+    "pf_nn_last_error": (C.c_char_p, []),
     "pf_batch_dft_read": (_i32, [_vp, _i32, _i32, _i32, _P(_f64)]),
     "pf_batch_read_field": (_i32, [_vp, _i32, _i32, _vp]),
 }

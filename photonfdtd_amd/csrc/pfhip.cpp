@@ -217,6 +217,10 @@ class Stepper final : public pf_handle {
   void describe(pfhip::SimDev<real>& s, int pol, PostNodes& post);
   void reset(hipStream_t s);
   void set_level(int p) { cur_ = p; }
+  void write_ce(const void* src, hipStream_t s) {
+    HIP_CHECK(hipMemcpyAsync(const_cast<real*>(args_.ce), src, (size_t)ncell() * sizeof(real),
+                             hipMemcpyHostToDevice, s));
+  }
   hipStream_t stream() const { return stream_; }
   int64_t ncell() const { return shape[0] * shape[1] * shape[2]; }
   void timing_read(double* ms, int64_t* launches) override;
@@ -1123,6 +1127,7 @@ struct BatchBase {
   virtual pf_handle* sim(int i) = 0;
   virtual int size() const = 0;
   virtual double last_ms() const = 0;
+  virtual void write_ce(int i, const void* ce) = 0;
 };
 
 template <typename real>
@@ -1221,6 +1226,11 @@ class Batch final : public BatchBase {
   }
   int size() const override { return (int)sims_.size(); }
   double last_ms() const override { return last_ms_; }
+  void write_ce(int i, const void* ce) override {
+    if (i < 0 || i >= (int)sims_.size()) throw ArgError{"simulation index out of range"};
+    sims_[i]->write_ce(ce, stream_);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
 
   // mode 0: one march launch per step; 1: one persistent launch for all
   // steps of an advance; 2: march launches replayed from a captured graph of
@@ -1512,6 +1522,11 @@ int pf_batch_advance(pf_batch* b, int64_t n) {
 int pf_batch_last_ms(pf_batch* b, double* ms) {
   if (!b || !ms) return fail(PF_E_ARG, "null argument");
   return guarded([&] { *ms = b->impl->last_ms(); });
+}
+
+int pf_batch_write_ce(pf_batch* b, int32_t sim, const void* ce) {
+  if (!b || !ce) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { b->impl->write_ce(sim, ce); });
 }
 
 int pf_batch_dft_read(pf_batch* b, int32_t sim, int32_t mon, int32_t slot, double* dst) {
