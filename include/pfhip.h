@@ -232,9 +232,6 @@ int pf_nn_conv_forward(const pf_nn_conv* c, const void* x, const void* w, const 
  * written; dw and dbias (FP32, may be NULL) are accumulated into. */
 int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void* y,
                         const float* dy, float* dx, float* dw, float* dbias, void* stream);
-int pf_nn_gap_forward(int32_t n, int32_t hw, int32_t ch, const void* x, void* y, void* stream);
-int pf_nn_gap_backward(int32_t n, int32_t hw, int32_t ch, const float* dy, float* dx,
-                       void* stream);
 /* loss (one FP32, zeroed by the caller) += mean (pred - target)^2; dpred written. */
 int pf_nn_mse(int32_t total, const float* pred, const float* target, float* dpred, float* loss,
               void* stream);

@@ -38,7 +38,7 @@ def network(x_nhw, params: Sequence[Tuple[torch.Tensor, torch.Tensor]],
     """Forward pass; params are (w [cout, k, k, cin] BF16-valued float32, b)."""
     cur = bf16(x_nhw.unsqueeze(-1))
     for li, ((ci, co, k, s, relu), (w, b)) in enumerate(zip(layers, params)):
-        if li == n_encoder:
-            cur = bf16(cur.mean(dim=(1, 2), keepdim=True))
+        if li == n_encoder:                      # flatten NHWC into the 1x1 head
+            cur = cur.reshape(cur.shape[0], 1, 1, -1)
         cur = conv_layer(cur, w, b, s, bool(relu), li == len(layers) - 1)
     return cur.reshape(cur.shape[0], -1)

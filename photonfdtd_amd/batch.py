@@ -25,7 +25,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 import numpy as np
 
 from . import _native as N
-from .hip_backend import build_desc, exact_mode, validate
+from .hip_backend import build_desc, ce_of, exact_mode, validate
 from .monitors import DFTMonitor, FieldMonitor, FluxMonitor
 from .simulation import Result
 
@@ -75,11 +75,16 @@ class SimBatch:
         pols = (C.c_int32 * n)()
         keep: List = []
         self.slots: List[Dict[str, tuple]] = []
+        built: Dict[int, tuple] = {}        # the same Simulation object shares its tables
         for i, sim in enumerate(sims):
-            d, k, t, slots, _ = build_desc(sim, self.npdt, exact_mode(self.npdt), N.PF_FLAG_GENERIC)
-            keep += [d, k, t]
+            if id(sim) not in built:
+                d, k, t, slots, _ = build_desc(sim, self.npdt, exact_mode(self.npdt),
+                                               N.PF_FLAG_GENERIC)
+                built[id(sim)] = (d, slots, polarisation(sim))
+                keep += [d, k, t]
+            d, slots, pol = built[id(sim)]
             descs[i] = d
-            pols[i] = polarisation(sim)
+            pols[i] = pol
             self.slots.append(slots)
         h = C.c_void_p()
         N.check(self.lib, self.lib.pf_batch_create(descs, pols, n, C.byref(h)), "pf_batch_create")
@@ -150,6 +155,14 @@ class SimBatch:
         n = self.n_steps - self.step if n is None else int(n)
         N.check(self.lib, self.lib.pf_batch_advance(self.h, n), "pf_batch_advance")
         self.step += n
+
+    def set_eps(self, i: int, eps: np.ndarray) -> None:
+        """New permittivity grid for simulation i (takes effect from the next
+        step; call ``reset`` to start a fresh run)."""
+        sim = self.sims[i]
+        ce = ce_of(sim, np.asarray(eps).reshape(sim.grid.shape), self.npdt)
+        N.check(self.lib, self.lib.pf_batch_write_ce(self.h, int(i), ce.ctypes.data),
+                "pf_batch_write_ce")
 
     def last_ms(self) -> float:
         """Device time of the last ``advance`` (HIP events on the batch stream)."""

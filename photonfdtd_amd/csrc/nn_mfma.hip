@@ -200,24 +200,6 @@ __global__ __launch_bounds__(256) void k_bias_grad(Geom g, const float* dy, cons
   if (threadIdx.x == 0) dbias[co] += part[0];
 }
 
-// Global average pool NHWC BF16 -> [n][c] BF16.
-__global__ void k_gap_fwd(int n, int hw, int c, const uint16_t* x, uint16_t* y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * c) return;
-  const int b = i / c, ch = i % c;
-  float s = 0.f;
-  for (int p = 0; p < hw; ++p) s += bf2f(x[((size_t)b * hw + p) * c + ch]);
-  y[i] = f2bf(s / (float)hw);
-}
-
-__global__ void k_gap_bwd(int n, int hw, int c, const float* dy, float* dx) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)n * hw * c) return;
-  const int ch = (int)(i % c);
-  const int b = (int)(i / ((size_t)hw * c));
-  dx[i] = dy[(size_t)b * c + ch] / (float)hw;
-}
-
 // Mean squared error over n x d: loss[0] = mean (p - t)^2, dp = 2 (p - t) / (n d).
 __global__ void k_mse(int total, const float* p, const float* tgt, float* dp, float* loss) {
   __shared__ float part[256];
@@ -323,22 +305,6 @@ int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const
   }
   if (dbias) pfnn::k_bias_grad<<<g.cout, 256, 0, s>>>(g, dy, y, c->relu, c->out_f32, dbias);
   return nn_status("pf_nn_conv_backward");
-}
-
-int pf_nn_gap_forward(int32_t n, int32_t hw, int32_t ch, const void* x, void* y, void* stream) {
-  if (n <= 0 || hw <= 0 || ch <= 0 || !x || !y) return nn_fail("pf_nn_gap_forward: bad arguments");
-  pfnn::k_gap_fwd<<<(n * ch + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(
-      n, hw, ch, static_cast<const uint16_t*>(x), static_cast<uint16_t*>(y));
-  return nn_status("pf_nn_gap_forward");
-}
-
-int pf_nn_gap_backward(int32_t n, int32_t hw, int32_t ch, const float* dy, float* dx,
-                       void* stream) {
-  if (n <= 0 || hw <= 0 || ch <= 0 || !dy || !dx) return nn_fail("pf_nn_gap_backward: bad arguments");
-  const int64_t total = (int64_t)n * hw * ch;
-  pfnn::k_gap_bwd<<<(unsigned)((total + 255) / 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
-      n, hw, ch, dy, dx);
-  return nn_status("pf_nn_gap_backward");
 }
 
 int pf_nn_mse(int32_t total, const float* pred, const float* target, float* dpred, float* loss,

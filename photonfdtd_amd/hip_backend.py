@@ -70,14 +70,19 @@ def _snap_box(shape, ds, plane=None, zplane=None):
     return start, count, stride
 
 
+def ce_of(sim, eps, npdt) -> np.ndarray:
+    """ce = dt / (eps_r eps0) in the field dtype (``simulation.py:953-954``)."""
+    eps = np.asarray(eps, dtype=sim.dtypes["eps_r"])
+    return np.ascontiguousarray((sim.dt / (eps * EPS_0)).astype(npdt, copy=False))
+
+
 def build_tables(sim, npdt) -> Dict:
     grid = sim.grid
     dt = sim.dt
     n_steps = sim.n_steps
     nx, ny, nz = grid.shape
-    eps = np.asarray(sim.eps_r, dtype=sim.dtypes["eps_r"])
     t: Dict = {}
-    t["ce"] = np.ascontiguousarray((dt / (eps * EPS_0)).astype(npdt, copy=False))
+    t["ce"] = ce_of(sim, sim.eps_r, npdt)
     t["cell"] = tuple(d if d > 0 else 1.0 for d in grid.cell_size)
     t["cpml"] = [np.ascontiguousarray(a, dtype=np.float64)
                  for arrs in (sim._b_e, sim._c_e, sim._b_h, sim._c_h) for a in arrs]

@@ -8,8 +8,11 @@ accumulation, FP32 master weights under Adam), and every design it proposes
 is verified by a full FDTD solve, with the plain adjoint step as fallback.
 
 Network: a strided 3x3 convolution encoder over the 128 x 128 design map
-(channels 32-64-128-256-256, resolution 128 -> 4), global average pooling and
-a two-layer spectral head (256 -> 512 -> n_freq), about 1.9 M parameters.
+(channels 32-64-128-256-128, resolution 128 -> 4) whose 4 x 4 x 128 output is
+flattened (it keeps where the material is, which decides the coupling) into a
+two-layer spectral head (2048 -> 256 -> n_freq), about 2.0 M parameters. In
+NHWC order the flatten is a reshape, so the head layers are 1 x 1
+convolutions on a 1 x 1 map.
 PyTorch provides device memory and the stream only; every layer's forward
 and backward pass and the optimizer run in this package's kernels.
 """
@@ -23,16 +26,17 @@ import numpy as np
 
 from . import _native as N
 
-# (cin, cout, ksize, stride, relu); the GAP sits between the convolutions and
-# the 1x1 head layers.
+# (cin, cout, ksize, stride, relu) of the encoder; its output is flattened
+# into the 1x1 head layers.
 ENCODER: Tuple[Tuple[int, int, int, int, int], ...] = (
     (1, 32, 3, 1, 1), (32, 32, 3, 2, 1),
     (32, 64, 3, 1, 1), (64, 64, 3, 2, 1),
     (64, 128, 3, 1, 1), (128, 128, 3, 2, 1),
     (128, 256, 3, 1, 1), (256, 256, 3, 2, 1),
-    (256, 256, 3, 2, 1),
+    (256, 128, 3, 2, 1),
 )
-HEAD_HIDDEN = 512
+FLAT = 4 * 4 * 128       # encoder output at a 128 x 128 input
+HEAD_HIDDEN = 256
 
 
 def _torch():
@@ -44,7 +48,7 @@ def _torch():
 
 
 def layer_table(n_out: int = 16) -> List[Tuple[int, int, int, int, int]]:
-    return list(ENCODER) + [(ENCODER[-1][1], HEAD_HIDDEN, 1, 1, 1), (HEAD_HIDDEN, n_out, 1, 1, 0)]
+    return list(ENCODER) + [(FLAT, HEAD_HIDDEN, 1, 1, 1), (HEAD_HIDDEN, n_out, 1, 1, 0)]
 
 
 def n_params(n_out: int = 16) -> int:
@@ -114,13 +118,12 @@ class SpectralSurrogate:
         self._acts, self._shapes = [cur], [(h, w, 1)]
         n_enc = len(ENCODER)
         for li, (ci, co, k, st, relu) in enumerate(self.layers):
-            if li == n_enc:                                    # global average pool
-                pooled = torch.empty((n, 1, 1, ci), dtype=torch.bfloat16, device=self.dev)
-                _nn_check(self.lib, self.lib.pf_nn_gap_forward(n, h * w, ci, self._ptr(cur),
-                                                               self._ptr(pooled), s),
-                          "pf_nn_gap_forward")
-                self._gap_hw = (h, w)
-                cur, h, w = pooled, 1, 1
+            if li == n_enc:                                    # flatten (NHWC: a reshape)
+                if cur[0].numel() != ci:
+                    raise ValueError(f"encoder output {tuple(cur.shape[1:])} does not flatten "
+                                     f"to {ci} features; the input must be {self.in_size}^2")
+                self._enc_shape = tuple(cur.shape)
+                cur, h, w = cur.reshape(n, 1, 1, ci), 1, 1
                 self._acts.append(cur)
                 self._shapes.append((1, 1, ci))
             d = self._desc(n, h, w, li)
@@ -146,8 +149,8 @@ class SpectralSurrogate:
         n = dy.shape[0]
         n_enc = len(ENCODER)
         g = dy.contiguous().view(*self._acts[-1].shape)
-        # activation index of the input of layer li: acts[li] before the GAP,
-        # acts[li + 1] after it (the pooled tensor is inserted)
+        # activation index of the input of layer li: acts[li] before the
+        # flatten, acts[li + 1] after it (the flattened view is inserted)
         for li in range(len(self.layers) - 1, -1, -1):
             ai = li if li < n_enc else li + 1
             x_in, y_out = self._acts[ai], self._acts[ai + 1]
@@ -159,14 +162,8 @@ class SpectralSurrogate:
             _nn_check(self.lib, self.lib.pf_nn_conv_backward(
                 C.byref(d), self._ptr(x_in), self._ptr(q["wb"]), self._ptr(y_out), self._ptr(g),
                 self._ptr(dx), self._ptr(q["gw"]), self._ptr(q["gb"]), s), "pf_nn_conv_backward")
-            if li == n_enc:                                    # through the pool
-                gh, gw_ = self._gap_hw
-                ci = self.layers[li][0]
-                dpool = torch.empty((n, gh, gw_, ci), dtype=torch.float32, device=self.dev)
-                _nn_check(self.lib, self.lib.pf_nn_gap_backward(n, gh * gw_, ci, self._ptr(dx),
-                                                                self._ptr(dpool), s),
-                          "pf_nn_gap_backward")
-                dx = dpool
+            if li == n_enc:                                    # through the flatten
+                dx = dx.reshape(self._enc_shape)
             g = dx
         return g.view(n, self._shapes[0][0], self._shapes[0][1]) if input_grad else None
 
