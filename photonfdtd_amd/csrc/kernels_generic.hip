@@ -203,24 +203,24 @@ __global__ void k_dft_fold(const real* __restrict__ ring, int64_t nbox, int n_co
                            const int32_t* __restrict__ is_h, int count, int64_t sample0,
                            const double* __restrict__ phase_e, const double* __restrict__ phase_h,
                            int n_freq, double dt, double* __restrict__ acc) {
+  // one thread per (component, frequency, box node); consecutive threads take
+  // consecutive nodes, so ring reads and accumulator updates are coalesced
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nbox * n_comp) return;
-  const int c = (int)(t / nbox);
-  const int64_t r = t - (int64_t)c * nbox;
+  if (t >= nbox * n_comp * n_freq) return;
+  const int64_t r = t % nbox;
+  const int64_t cf = t / nbox;
+  const int f = (int)(cf % n_freq), c = (int)(cf / n_freq);
   const double* ph = is_h[c] ? phase_h : phase_e;
-  double* A = acc + (int64_t)c * n_freq * nbox * 2;
-  for (int f = 0; f < n_freq; ++f) {
-    double re = A[2 * ((int64_t)f * nbox + r)];
-    double im = A[2 * ((int64_t)f * nbox + r) + 1];
-    for (int s = 0; s < count; ++s) {
-      const double v = (double)ring[((int64_t)s * n_comp + c) * nbox + r];
-      const double* p = ph + 2 * ((sample0 + s) * n_freq + f);
-      re = re + (p[0] * v) * dt;
-      im = im + (p[1] * v) * dt;
-    }
-    A[2 * ((int64_t)f * nbox + r)] = re;
-    A[2 * ((int64_t)f * nbox + r) + 1] = im;
+  double* A = acc + 2 * (((int64_t)c * n_freq + f) * nbox + r);
+  double re = A[0], im = A[1];
+  for (int s = 0; s < count; ++s) {
+    const double v = (double)ring[((int64_t)s * n_comp + c) * nbox + r];
+    const double* p = ph + 2 * ((sample0 + s) * n_freq + f);
+    re = re + (p[0] * v) * dt;
+    im = im + (p[1] * v) * dt;
   }
+  A[0] = re;
+  A[1] = im;
 }
 
 // Plane Poynting flux, stage 1: per-block partial sums of S . n.
@@ -316,7 +316,7 @@ template <typename real>
 void launch_dft_fold(const real* ring, int64_t nbox, int n_comp, const int32_t* is_h, int count,
                      int64_t sample0, const double* phase_e, const double* phase_h, int n_freq,
                      double dt, double* acc, hipStream_t s) {
-  k_dft_fold<real><<<blocks_for(nbox * n_comp, 256), 256, 0, s>>>(
+  k_dft_fold<real><<<blocks_for(nbox * n_comp * n_freq, 256), 256, 0, s>>>(
       ring, nbox, n_comp, is_h, count, sample0, phase_e, phase_h, n_freq, dt, acc);
 }
 

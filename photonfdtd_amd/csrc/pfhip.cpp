@@ -144,6 +144,28 @@ std::vector<pfhip::MarchTile> xcd_order(const std::vector<pfhip::MarchTile>& t) 
   return out;
 }
 
+// Alternative order: every x-chunk is split into 8 contiguous bands (one per
+// XCD), and each XCD takes its band of chunk 0, then of chunk 1, ..., so all
+// XCDs sweep the same x-chunk at the same time.
+std::vector<pfhip::MarchTile> xcd_band_order(const std::vector<pfhip::MarchTile>& t) {
+  std::vector<std::vector<pfhip::MarchTile>> queue(8);
+  size_t a = 0;
+  while (a < t.size()) {
+    size_t b = a;
+    while (b < t.size() && t[b].i0 == t[a].i0) ++b;   // one x-chunk
+    const size_t m = b - a;
+    for (int x = 0; x < 8; ++x)
+      for (size_t q = a + m * x / 8; q < a + m * (x + 1) / 8; ++q) queue[x].push_back(t[q]);
+    a = b;
+  }
+  size_t g = 0;
+  for (auto& q : queue) g = std::max(g, q.size());
+  std::vector<pfhip::MarchTile> out(8 * g, pfhip::MarchTile{0, 0, 0, 0, 0, 0, 0, 0});
+  for (int x = 0; x < 8; ++x)
+    for (size_t q = 0; q < queue[x].size(); ++q) out[8 * q + x] = queue[x][q];
+  return out;
+}
+
 using pfhip::BoxDev;
 using pfhip::SourceGroupsDev;
 using pfhip::StepArgs;
@@ -294,6 +316,8 @@ class Stepper final : public pf_handle {
   int* tiles_[2] = {nullptr, nullptr};   // march tiles: [0] psi-free, [1] with CPML
   int ntiles_[2] = {0, 0};
   int march_nw_ = pfhip::MARCH_NW;
+  int march_dist_ = 2;            // planes of prefetch in the march kernel
+  bool band_order_ = false;       // tile order: all XCDs on one x-chunk at a time
   bool split_ = false;           // separate psi-free launch for tiles off the CPML slabs
   // checkpoints for the adjoint (generic layout: level 0 fields + both psi sides)
   int ckpt_interval_ = 0;
@@ -500,8 +524,10 @@ Stepper<real>::Stepper(const pf_desc& d) {
     st.dt = s.dt;
     st.acc = arena_.zeros<double>((size_t)s.n_comp * s.n_freq * st.nbox * 2, stream_);
     const int64_t sample_bytes = (int64_t)s.n_comp * st.nbox * (int64_t)sizeof(real);
-    const int64_t budget = 256ll << 20;
-    st.K = (int)std::max<int64_t>(1, std::min<int64_t>(32, budget / std::max<int64_t>(sample_bytes, 1)));
+    // ring of up to 128 samples (1 GB per monitor at most): each fold rewrites
+    // the complex128 accumulators, so fewer folds save that traffic
+    const int64_t budget = 1ll << 30;
+    st.K = (int)std::max<int64_t>(1, std::min<int64_t>(128, budget / std::max<int64_t>(sample_bytes, 1)));
     st.ring = static_cast<real*>(arena_.alloc((size_t)st.K * sample_bytes));
     st.count = 0;
     st.sample0 = 0;
@@ -611,6 +637,10 @@ Stepper<real>::Stepper(const pf_desc& d) {
 
   const char* env_nw = std::getenv("PHOTONFDTD_HIP_NW");
   if (env_nw && std::atoi(env_nw) == 16) march_nw_ = 16;
+  const char* env_dist = std::getenv("PHOTONFDTD_HIP_PREFETCH");
+  if (env_dist && std::atoi(env_dist) == 1) march_dist_ = 1;
+  const char* env_order = std::getenv("PHOTONFDTD_HIP_ORDER");
+  band_order_ = env_order && env_order[0] == '1';
   const char* env_split = std::getenv("PHOTONFDTD_HIP_SPLIT");
   split_ = env_split && env_split[0] == '1';
   if (fused_ && fa_[0].variant == 2) build_march_tiles(std::min(fa_[0].xchunk, pfhip::MARCH_MAX_XCHUNK));
@@ -639,7 +669,8 @@ void Stepper<real>::build_march_tiles(int xchunk) {
       }
   const std::vector<pfhip::MarchTile>* lists[2] = {&bulk, &pml};
   for (int q = 0; q < 2; ++q) {
-    std::vector<pfhip::MarchTile> ordered = xcd_order(*lists[q]);
+    std::vector<pfhip::MarchTile> ordered =
+        band_order_ ? xcd_band_order(*lists[q]) : xcd_order(*lists[q]);
     ntiles_[q] = (int)ordered.size();
     tiles_[q] = ordered.empty() ? nullptr
                                 : reinterpret_cast<int*>(arena_.upload(ordered.data(), ordered.size(), stream_));
@@ -712,7 +743,7 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
           pfhip::MarchArgs<real> ma = ma_[in];
           ma.tiles = tiles_[q];
           ma.ntiles = ntiles_[q];
-          pfhip::launch_march<real>(ma, exact_, q == 1, march_nw_, stream_);
+          pfhip::launch_march<real>(ma, exact_, q == 1, march_nw_, march_dist_, stream_);
         }
       } else {
         pfhip::launch_fused<real>(fa_[in], exact_, stream_);

@@ -268,7 +268,7 @@ void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int 
 
 constexpr int MARCH_NW = 10;   // wavefronts per march workgroup (rows j0-1 .. j0+NW-2)
 template <typename real>
-void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, hipStream_t s);
+void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, int dist, hipStream_t s);
 template <typename real> bool fused_supported(const StepArgs<real>& a);
 template <typename real> void launch_fused(FusedArgs<real>& a, bool exact, hipStream_t s);
 template <typename real>

@@ -440,7 +440,8 @@ __device__ __forceinline__ void st(real* base, uint32_t boff, real v) {
   *reinterpret_cast<real*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
-template <typename real, bool EXACT, int NW, bool PML>
+// DIST: how many planes ahead the loads are issued (1 or 2).
+template <typename real, bool EXACT, int NW, bool PML, int DIST>
 __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
   constexpr uint32_t B = sizeof(real);
   constexpr int XS = MARCH_MAX_XCHUNK + 1;
@@ -590,7 +591,7 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
   const real K_h = -a.ch;
   // One plane: P holds plane i's memory operands; E(i) is in ecur.
   auto plane = [&](int i, const Plane& P, Plane& Next) {
-    if (i + 1 < i1) fetch(Next, i + 1);
+    if (i + DIST < i1) fetch(Next, i + DIST);
     const Psi& Q = P.q;
     const bool live = i >= i0;                // the plane before i0 only rebuilds H(i0-1)
     sEx[tid] = ecur[0];
@@ -685,21 +686,48 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
     hy_prev = hy;
     hz_prev = hz;
   };
-  // Two plane buffers alternate so no register still being loaded is copied.
-  Plane A, Bq;
-  fetch(A, ib);
-  for (int i = ib; i < i1; i += 2) {
-    if (i > ib) {
+  if constexpr (DIST == 1) {
+    // Two plane buffers alternate so no register still being loaded is copied.
+    Plane A, Bq;
+    fetch(A, ib);
+    for (int i = ib; i < i1; i += 2) {
+      if (i > ib) {
+        ecur[0] = Bq.e[0];
+        ecur[1] = Bq.e[1];
+        ecur[2] = Bq.e[2];
+      }
+      plane(i, A, Bq);
+      if (i + 1 >= i1) break;
+      ecur[0] = A.e[0];
+      ecur[1] = A.e[1];
+      ecur[2] = A.e[2];
+      plane(i + 1, Bq, A);
+    }
+  } else {
+    // Three plane buffers rotate: plane i is computed from one while planes
+    // i + 1 and i + 2 are in flight in the others. E^n(i) is copied out of
+    // the buffer of plane i - 1 before that buffer receives plane i + 2.
+    Plane A, Bq, Cq;
+    fetch(A, ib);
+    if (ib + 1 < i1) fetch(Bq, ib + 1);
+    for (int i = ib; i < i1; i += 3) {
+      if (i > ib) {
+        ecur[0] = Cq.e[0];
+        ecur[1] = Cq.e[1];
+        ecur[2] = Cq.e[2];
+      }
+      plane(i, A, Cq);
+      if (i + 1 >= i1) break;
+      ecur[0] = A.e[0];
+      ecur[1] = A.e[1];
+      ecur[2] = A.e[2];
+      plane(i + 1, Bq, A);
+      if (i + 2 >= i1) break;
       ecur[0] = Bq.e[0];
       ecur[1] = Bq.e[1];
       ecur[2] = Bq.e[2];
+      plane(i + 2, Cq, Bq);
     }
-    plane(i, A, Bq);
-    if (i + 1 >= i1) break;
-    ecur[0] = A.e[0];
-    ecur[1] = A.e[1];
-    ecur[2] = A.e[2];
-    plane(i + 1, Bq, A);
   }
 }
 
@@ -764,26 +792,31 @@ void launch_ade_pp(real* const* Ein, real* const* Eout, const int64_t* cells, in
                                                              Pprev, abc, n_poles, inv_eps);
 }
 
-template <typename real, int NW>
+template <typename real, int NW, int DIST>
 void launch_march_nw(MarchArgs<real>& a, bool exact, bool pml, hipStream_t s) {
   if (pml) {
-    if (exact) k_yee_march<real, true, NW, true><<<a.ntiles, NW * 64, 0, s>>>(a);
-    else k_yee_march<real, false, NW, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+    if (exact) k_yee_march<real, true, NW, true, DIST><<<a.ntiles, NW * 64, 0, s>>>(a);
+    else k_yee_march<real, false, NW, true, DIST><<<a.ntiles, NW * 64, 0, s>>>(a);
   } else {
-    if (exact) k_yee_march<real, true, NW, false><<<a.ntiles, NW * 64, 0, s>>>(a);
-    else k_yee_march<real, false, NW, false><<<a.ntiles, NW * 64, 0, s>>>(a);
+    if (exact) k_yee_march<real, true, NW, false, DIST><<<a.ntiles, NW * 64, 0, s>>>(a);
+    else k_yee_march<real, false, NW, false, DIST><<<a.ntiles, NW * 64, 0, s>>>(a);
   }
 }
 
 template <typename real>
-void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, hipStream_t s) {
+void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, int dist, hipStream_t s) {
   if (a.ntiles == 0) return;
-  if (nw == 16) launch_march_nw<real, 16>(a, exact, pml, s);
-  else launch_march_nw<real, MARCH_NW>(a, exact, pml, s);
+  if (nw == 16) {
+    if (dist == 2) launch_march_nw<real, 16, 2>(a, exact, pml, s);
+    else launch_march_nw<real, 16, 1>(a, exact, pml, s);
+  } else {
+    if (dist == 2) launch_march_nw<real, MARCH_NW, 2>(a, exact, pml, s);
+    else launch_march_nw<real, MARCH_NW, 1>(a, exact, pml, s);
+  }
 }
 
-template void launch_march<float>(MarchArgs<float>&, bool, bool, int, hipStream_t);
-template void launch_march<double>(MarchArgs<double>&, bool, bool, int, hipStream_t);
+template void launch_march<float>(MarchArgs<float>&, bool, bool, int, int, hipStream_t);
+template void launch_march<double>(MarchArgs<double>&, bool, bool, int, int, hipStream_t);
 template bool fused_supported<float>(const StepArgs<float>&);
 template bool fused_supported<double>(const StepArgs<double>&);
 template void launch_fused<float>(FusedArgs<float>&, bool, hipStream_t);
