@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 4
+#define PFHIP_ABI_VERSION 5
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -132,7 +132,8 @@ enum pf_kernel_class {
   PF_K_YEE_E = 1,      /* E half-step (curl H + CPML)                       */
   PF_K_YEE_FUSED = 2,  /* one launch doing a whole H+E step                 */
   PF_K_AUX = 3,        /* per-step sources, ADE, flux and DFT capture/fold  */
-  PF_NKCLASS = 4
+  PF_K_YEE_PAIR = 4,   /* one launch doing two whole steps (temporal block) */
+  PF_NKCLASS = 5
 };
 
 /* Replaces CudaStepper.new_f32/new_f64 (rust/src/cuda.rs:424-540). */

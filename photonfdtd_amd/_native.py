@@ -11,11 +11,11 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 PF_FLAG_GENERIC = 1
 
 PF_F32, PF_F64 = 0, 1
-KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux")
+KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux", "yee_pair")
 COMP_CODE = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 
 _i32, _i64, _f64, _vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p

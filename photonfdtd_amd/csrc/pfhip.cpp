@@ -132,6 +132,27 @@ std::vector<Seg> split_axis(int n, int b0, int b1, int len, bool widen) {
   return out;
 }
 
+// [0, n) as [0, b0) U [b0, b1) U [b1, n), each range cut into the fewest
+// segments of at most len nodes, of nearly equal size.
+std::vector<Seg> split_even(int n, int b0, int b1, int len) {
+  std::vector<Seg> out;
+  b0 = std::max(0, std::min(b0, n));
+  b1 = std::max(b0, std::min(b1, n));
+  auto emit = [&](int lo, int hi, bool bulk) {
+    if (hi <= lo) return;
+    const int k = (hi - lo + len - 1) / len;
+    for (int i = 0; i < k; ++i) {
+      const int s = lo + (int)((int64_t)(hi - lo) * i / k);
+      const int e = lo + (int)((int64_t)(hi - lo) * (i + 1) / k);
+      out.push_back({s, e - s, bulk});
+    }
+  };
+  emit(0, b0, false);
+  emit(b0, b1, true);
+  emit(b1, n, false);
+  return out;
+}
+
 // Order tiles so that the blocks the hardware sends to one XCD (block ids
 // congruent mod 8) are spatially contiguous; pad with empty tiles.
 std::vector<pfhip::MarchTile> xcd_order(const std::vector<pfhip::MarchTile>& t) {
@@ -257,6 +278,10 @@ class Stepper final : public pf_handle {
     real* En;
     real* Pcur;
     real* Pprev;
+    real* Pcur1 = nullptr;   // second level of the two-step march
+    real* Pprev1 = nullptr;
+    bool is_box = false;     // cells are the box [lo, hi) in x-major order
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   };
   struct Dft {
     int n_comp;
@@ -319,6 +344,24 @@ class Stepper final : public pf_handle {
   int march_dist_ = 2;            // planes of prefetch in the march kernel
   bool band_order_ = false;       // tile order: all XCDs on one x-chunk at a time
   bool split_ = false;           // separate psi-free launch for tiles off the CPML slabs
+  // two-step march (temporal blocking): one launch advances two steps
+  bool pair_ = false;
+  int pair_nw_ = 10;
+  real* psi_e_lvl_[2] = {nullptr, nullptr};  // E-side psi, two levels
+  int psi_e_cur_ = 0;                        // level holding the current E-side psi
+  int* tiles2_[2] = {nullptr, nullptr};      // pair tiles: [0] psi-free, [1] with CPML
+  int ntiles2_[2] = {0, 0};
+  pfhip::MarchArgs2<real> m2_;               // launch-invariant part of the pair arguments
+  pfhip::March2Ade<real>* ade_dev_[2] = {nullptr, nullptr};  // media, reading P level 0 / 1
+  pfhip::March2Dft<real>* dft_dev_ = nullptr;
+  int p_lvl_ = 0;                            // P level holding the current polarisation
+  void setup_pair(int xchunk);
+  void build_pair_tiles(int xchunk);
+  void set_psi_e(int lvl);
+  void step_pair(int64_t step);
+  bool src_e_sorted_ = true;      // E-side source groups strictly ordered by (component, cell)
+  int src_e_beg_[4] = {0, 0, 0, 0};   // first E-side group of each component
+  int src_e_ilo_ = 1, src_e_ihi_ = 0; // x-plane range of the E-side sources
   // checkpoints for the adjoint (generic layout: level 0 fields + both psi sides)
   int ckpt_interval_ = 0;
   std::vector<real*> ckpts_;
@@ -341,8 +384,8 @@ class Stepper final : public pf_handle {
   bool timing_ = false;
   std::vector<Timed> pending_;
   std::vector<hipEvent_t> pool_;
-  double t_ms_[PF_NKCLASS] = {0, 0, 0, 0};
-  int64_t t_count_[PF_NKCLASS] = {0, 0, 0, 0};
+  double t_ms_[PF_NKCLASS] = {0, 0, 0, 0, 0};
+  int64_t t_count_[PF_NKCLASS] = {0, 0, 0, 0, 0};
 };
 
 template <typename real>
@@ -463,6 +506,22 @@ Stepper<real>::Stepper(const pf_desc& d) {
     amp[side].push_back(d.src_amp[s]);
     wrow[side].push_back(d.src_wave[s]);
   }
+  {
+    // E-side groups for the two-step march: sorted by (component, cell),
+    // per-component ranges and the x-planes they touch
+    const std::vector<int32_t>& gc = gcomp[0];
+    const std::vector<int64_t>& gx = gcell[0];
+    for (size_t i = 1; i < gc.size(); ++i)
+      if (gc[i] < gc[i - 1] || (gc[i] == gc[i - 1] && gx[i] <= gx[i - 1])) src_e_sorted_ = false;
+    for (int c = 0; c < 4; ++c)
+      src_e_beg_[c] = (int)(std::lower_bound(gc.begin(), gc.end(), c) - gc.begin());
+    if (!src_e_sorted_) src_e_beg_[3] = (int)gc.size();
+    for (int64_t cell : gx) {
+      const int xi = (int)(cell / ((int64_t)ny * nz));
+      src_e_ilo_ = src_e_ilo_ > src_e_ihi_ ? xi : std::min(src_e_ilo_, xi);
+      src_e_ihi_ = std::max(src_e_ihi_, xi);
+    }
+  }
   double* wave = arena_.upload(d.wave, (size_t)d.n_wave * d.n_steps, stream_);
   SourceGroupsDev* groups[2] = {&src_e_, &src_h_};
   for (int side = 0; side < 2; ++side) {
@@ -496,6 +555,29 @@ Stepper<real>::Stepper(const pf_desc& d) {
     st.En = arena_.zeros<real>(3 * a.n_cells, stream_);
     st.Pcur = arena_.zeros<real>(3 * (size_t)a.n_poles * a.n_cells, stream_);
     st.Pprev = arena_.zeros<real>(3 * (size_t)a.n_poles * a.n_cells, stream_);
+    {
+      // box-shaped media (cells enumerate [lo, hi) x-major) can run in the two-step march
+      int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {-1, -1, -1};
+      for (int64_t t = 0; t < a.n_cells; ++t) {
+        const int64_t c = a.cells[t];
+        const int64_t x[3] = {c / ((int64_t)ny * nz), (c / nz) % ny, c % nz};
+        for (int q = 0; q < 3; ++q) {
+          lo[q] = std::min(lo[q], x[q]);
+          hi[q] = std::max(hi[q], x[q] + 1);
+        }
+      }
+      const int64_t e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+      bool box = e[0] * e[1] * e[2] == a.n_cells;
+      for (int64_t t = 0; box && t < a.n_cells; ++t) {
+        const int64_t x = lo[0] + t / (e[1] * e[2]), y = lo[1] + (t / e[2]) % e[1], z = lo[2] + t % e[2];
+        box = a.cells[t] == (x * ny + y) * nz + z;
+      }
+      st.is_box = box;
+      for (int q = 0; q < 3; ++q) {
+        st.lo[q] = (int)lo[q];
+        st.hi[q] = (int)hi[q];
+      }
+    }
     ade_.push_back(st);
   }
 
@@ -644,6 +726,7 @@ Stepper<real>::Stepper(const pf_desc& d) {
   const char* env_split = std::getenv("PHOTONFDTD_HIP_SPLIT");
   split_ = env_split && env_split[0] == '1';
   if (fused_ && fa_[0].variant == 2) build_march_tiles(std::min(fa_[0].xchunk, pfhip::MARCH_MAX_XCHUNK));
+  if (fused_ && fa_[0].variant == 2) setup_pair(std::min(fa_[0].xchunk, pfhip::MARCH_MAX_XCHUNK));
   HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
@@ -675,6 +758,207 @@ void Stepper<real>::build_march_tiles(int xchunk) {
     tiles_[q] = ordered.empty() ? nullptr
                                 : reinterpret_cast<int*>(arena_.upload(ordered.data(), ordered.size(), stream_));
   }
+}
+
+// Two-step march set-up: the pair kernel needs box-shaped media, E-side
+// sources sorted by (component, cell), at most MARCH2_MAX_DFT monitors and
+// no flux planes (their per-step sums would need the intermediate level).
+template <typename real>
+void Stepper<real>::setup_pair(int xchunk) {
+  const char* env = std::getenv("PHOTONFDTD_HIP_PAIR");
+  if (!env || env[0] != '1') return;   // opt-in
+  if (!flux_.empty() || !src_e_sorted_ || ade_.size() > (size_t)pfhip::MARCH2_MAX_ADE ||
+      dft_.size() > (size_t)pfhip::MARCH2_MAX_DFT)
+    return;
+  for (const Ade& a : ade_)
+    if (!a.is_box) return;
+  for (const Dft& d : dft_)
+    if (d.K > 255) return;   // ring slots are passed as 8-bit indices
+  const char* env_nw = std::getenv("PHOTONFDTD_HIP_PAIR_NW");
+  pair_nw_ = env_nw && std::atoi(env_nw) == 12 ? 12 : 10;
+  psi_e_lvl_[0] = psi_side_[0];
+  psi_e_lvl_[1] = psi_len_e_ ? arena_.zeros<real>(psi_len_e_, stream_) : nullptr;
+  for (Ade& a : ade_) {
+    const size_t n = 3 * (size_t)a.n_poles * a.n;
+    if (!n) continue;
+    a.Pcur1 = arena_.zeros<real>(n, stream_);
+    a.Pprev1 = arena_.zeros<real>(n, stream_);
+  }
+  pfhip::MarchArgs2<real>& m = m2_;
+  std::memset(&m, 0, sizeof(m));
+  const pfhip::MarchArgs<real>& ma = ma_[0];
+  m.ce = ma.ce;
+  m.coef = ma.coef;
+  m.nx = ma.nx;
+  m.ny = ma.ny;
+  m.nz = ma.nz;
+  for (int a = 0; a < 3; ++a) {
+    m.pe[a] = ma.pe[a];
+    m.ph[a] = ma.ph[a];
+    m.d[a] = ma.d[a];
+    m.inv_d[a] = ma.inv_d[a];
+    for (int q = 0; q < 2; ++q) {
+      m.slab_e[a][q] = ma.slab_e[a][q];
+      m.slab_h[a][q] = ma.slab_h[a][q];
+    }
+  }
+  m.ch = ma.ch;
+  m.n_steps = n_steps_;
+  for (int c = 0; c < 4; ++c) m.src_beg[c] = src_e_beg_[c];
+  m.src_i_lo = src_e_ilo_;
+  m.src_i_hi = src_e_ihi_;
+  m.src_cell = src_e_.cell;
+  m.src_begin = src_e_.begin;
+  m.src_amp = src_e_.amp;
+  m.src_row = src_e_.wave_row;
+  m.src_wave = src_e_.wave;
+  m.n_ade = (int)ade_.size();
+  if (!ade_.empty()) {
+    for (int lv = 0; lv < 2; ++lv) {
+      std::vector<pfhip::March2Ade<real>> v(ade_.size());
+      for (size_t i = 0; i < ade_.size(); ++i) {
+        const Ade& s = ade_[i];
+        pfhip::March2Ade<real>& A = v[i];
+        std::memset(&A, 0, sizeof(A));
+        for (int q = 0; q < 3; ++q) {
+          A.lo[q] = s.lo[q];
+          A.hi[q] = s.hi[q];
+        }
+        A.n_poles = s.n_poles;
+        A.n = s.n;
+        A.abc = s.abc;
+        A.inv_eps = s.inv_eps;
+        real* cur[2] = {s.Pcur, s.Pcur1};
+        real* prev[2] = {s.Pprev, s.Pprev1};
+        A.Pcur_in = cur[lv];
+        A.Pprev_in = prev[lv];
+        A.Pcur_out = cur[lv ^ 1];
+        A.Pprev_out = prev[lv ^ 1];
+      }
+      ade_dev_[lv] = arena_.upload(v.data(), v.size(), stream_);
+    }
+  }
+  m.n_dft = (int)dft_.size();
+  if (!dft_.empty()) {
+    std::vector<pfhip::March2Dft<real>> v(dft_.size());
+    for (size_t i = 0; i < dft_.size(); ++i) {
+      pfhip::March2Dft<real>& D = v[i];
+      std::memset(&D, 0, sizeof(D));
+      D.n_comp = dft_[i].n_comp;
+      for (int c = 0; c < dft_[i].n_comp; ++c) D.comps[c] = dft_[i].comps[c];
+      for (int q = 0; q < 3; ++q) {
+        D.start[q] = (int)dft_[i].box.start[q];
+        D.count[q] = (int)dft_[i].box.count[q];
+        D.stride[q] = (int)dft_[i].box.stride[q];
+      }
+      D.ring = dft_[i].ring;
+      D.slot_elems = (int64_t)dft_[i].n_comp * dft_[i].nbox;
+    }
+    dft_dev_ = arena_.upload(v.data(), v.size(), stream_);
+  }
+  m.dft = dft_dev_;
+  build_pair_tiles(xchunk);
+  pair_ = true;
+}
+
+// Tiles of the pair kernel: jn <= NW-4 owned rows, kn <= 60 owned z nodes.
+// A tile goes to the psi-free kernel when every node it computes (owned
+// nodes plus two on each side, on every axis) lies off the CPML slabs.
+template <typename real>
+void Stepper<real>::build_pair_tiles(int xchunk) {
+  const int n[3] = {args_.nx, args_.ny, args_.nz};
+  const int len[3] = {xchunk, pair_nw_ - 4, pfhip::MARCH2_OWNED_Z};
+  std::vector<Seg> seg[3];
+  for (int a = 0; a < 3; ++a) {
+    const int b0 = std::max(slab_e_[a][0], slab_h_[a][0]) + 2;
+    const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]) - 2;
+    seg[a] = split_even(n[a], b0, b1, len[a]);
+  }
+  std::vector<pfhip::MarchTile> bulk, pml;
+  for (const Seg& sx : seg[0])
+    for (const Seg& sy : seg[1])
+      for (const Seg& sz : seg[2]) {
+        // the kernel's launch shape assumes these bounds
+        if (sx.count < 1 || sx.count > pfhip::MARCH_MAX_XCHUNK || sy.count < 1 ||
+            sy.count > pair_nw_ - 4 || sz.count < 1 || sz.count > pfhip::MARCH2_OWNED_Z ||
+            sx.start < 0 || sx.start + sx.count > n[0] || sy.start < 0 || sy.start + sy.count > n[1] ||
+            sz.start < 0 || sz.start + sz.count > n[2])
+          throw HipError{PF_E_STATE, "pair tile outside the kernel's bounds"};
+        pfhip::MarchTile t{sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0};
+        (sx.bulk && sy.bulk && sz.bulk ? bulk : pml).push_back(t);
+      }
+  const std::vector<pfhip::MarchTile>* lists[2] = {&bulk, &pml};
+  for (int q = 0; q < 2; ++q) {
+    std::vector<pfhip::MarchTile> ordered = xcd_order(*lists[q]);
+    ntiles2_[q] = (int)ordered.size();
+    tiles2_[q] = ordered.empty() ? nullptr
+                                 : reinterpret_cast<int*>(arena_.upload(ordered.data(), ordered.size(), stream_));
+  }
+}
+
+// Point the single-step kernels at E-side psi level lvl.
+template <typename real>
+void Stepper<real>::set_psi_e(int lvl) {
+  psi_e_cur_ = lvl;
+  real* base = psi_e_lvl_[lvl];
+  for (int p = 0; p < 2; ++p) {
+    ma_[p].psiE = base;
+    for (int s = 0; s < 6; ++s)
+      fa_[p].psiE[s] = args_.psi[s] ? base + (args_.psi[s] - psi_side_[0]) : nullptr;
+  }
+}
+
+// Steps step and step+1 in one launch pair (psi-free tiles, CPML tiles).
+// Fields and H-side psi go from level cur_ to the other level, E-side psi
+// and the ADE polarisation between their own two levels. DFT samples of
+// step are written by the kernel into the ring slot reserved here; step+1
+// is captured from the output level as usual.
+template <typename real>
+void Stepper<real>::step_pair(int64_t step) {
+  const int in = cur_, out = cur_ ^ 1;
+  pfhip::MarchArgs2<real> a = m2_;
+  a.Lin = lvl_[in];
+  a.Lout = lvl_[out];
+  a.psiHin = in == 0 ? psi_side_[1] : psi_h_lvl1_;
+  a.psiHout = in == 0 ? psi_h_lvl1_ : psi_side_[1];
+  a.psiEin = psi_e_lvl_[psi_e_cur_];
+  a.psiEout = psi_e_lvl_[psi_e_cur_ ^ 1];
+  a.step = step;
+  a.ade = ade_dev_[p_lvl_];
+  for (const Dft& d : dft_) {
+    if (step % d.interval != 0) continue;
+    const int64_t sample = step / d.interval;
+    if (sample >= d.n_samples) throw ArgError{"DFT sample beyond the phase table"};
+    if (d.count != 0 && sample != d.sample0 + d.count) throw ArgError{"DFT samples must be consecutive"};
+  }
+  a.dft_slots = 0xFFFFFFFFu;
+  for (size_t m = 0; m < dft_.size(); ++m) {
+    Dft& d = dft_[m];
+    if (step % d.interval != 0) continue;
+    if (d.count == 0) d.sample0 = step / d.interval;
+    a.dft_slots &= ~(0xFFu << (8 * m));
+    a.dft_slots |= (uint32_t)d.count << (8 * m);
+    ++d.count;
+  }
+  timed(PF_K_YEE_PAIR, [&] {
+    for (int q = 0; q < 2; ++q) {
+      a.tiles = tiles2_[q];
+      a.ntiles = ntiles2_[q];
+      pfhip::launch_march2<real>(a, exact_, q == 1, pair_nw_, stream_);
+    }
+  });
+  for (Ade& m : ade_) {
+    std::swap(m.Pcur, m.Pcur1);
+    std::swap(m.Pprev, m.Pprev1);
+  }
+  p_lvl_ ^= 1;
+  set_psi_e(psi_e_cur_ ^ 1);
+  cur_ = out;
+  timed(PF_K_AUX, [&] {
+    for (Dft& d : dft_)
+      if (d.count == d.K) fold(d);
+    capture_dft(step + 1);
+  });
 }
 
 template <typename real>
@@ -928,11 +1212,20 @@ void Stepper<real>::reset(hipStream_t s) {
   if (psi_len_e_) HIP_CHECK(hipMemsetAsync(psi_side_[0], 0, psi_len_e_ * sizeof(real), s));
   if (psi_len_h_) HIP_CHECK(hipMemsetAsync(psi_side_[1], 0, psi_len_h_ * sizeof(real), s));
   if (psi_h_lvl1_) HIP_CHECK(hipMemsetAsync(psi_h_lvl1_, 0, psi_len_h_ * sizeof(real), s));
+  if (psi_e_lvl_[1]) HIP_CHECK(hipMemsetAsync(psi_e_lvl_[1], 0, psi_len_e_ * sizeof(real), s));
+  if (pair_) set_psi_e(0);
+  if (p_lvl_) {
+    for (Ade& a : ade_) {
+      std::swap(a.Pcur, a.Pcur1);
+      std::swap(a.Pprev, a.Pprev1);
+    }
+    p_lvl_ = 0;
+  }
   for (Ade& a : ade_) {
     const size_t n = 3 * (size_t)a.n_poles * a.n;
     if (n) {
-      HIP_CHECK(hipMemsetAsync(a.Pcur, 0, n * sizeof(real), s));
-      HIP_CHECK(hipMemsetAsync(a.Pprev, 0, n * sizeof(real), s));
+      for (real* P : {a.Pcur, a.Pprev, a.Pcur1, a.Pprev1})
+        if (P) HIP_CHECK(hipMemsetAsync(P, 0, n * sizeof(real), s));
     }
   }
   for (Dft& m : dft_) {
@@ -1078,12 +1371,18 @@ void Stepper<real>::advance(int64_t step0, int64_t n) {
   if (step0 < 0 || n < 0) throw ArgError{"negative step range"};
   if ((src_h_.n_groups || src_e_.n_groups) && step0 + n > n_steps_)
     throw ArgError{"step range exceeds the source waveform table"};
-  for (int64_t s = step0; s < step0 + n; ++s) {
+  for (int64_t s = step0; s < step0 + n;) {
     if (ckpt_interval_ > 0 && s % ckpt_interval_ == 0) {
       if (s / ckpt_interval_ >= (int64_t)ckpts_.size()) throw ArgError{"step beyond the checkpoint table"};
       save_state(ckpts_[s / ckpt_interval_]);
     }
-    step_once(s);
+    if (pair_ && ckpt_interval_ == 0 && s + 2 <= step0 + n) {
+      step_pair(s);
+      s += 2;
+    } else {
+      step_once(s);
+      s += 1;
+    }
     if (timing_ && pending_.size() > 4096) collect_timing();
   }
   HIP_CHECK(hipGetLastError());

@@ -89,6 +89,78 @@ struct MarchTile {
   int i0, i1, j0, jn, k0, kn, pad0, pad1;
 };
 
+// ---- two-step march (temporal blocking) ----------------------------------
+// Workgroups of NW waves; tile as MarchTile with jn <= NW-4 owned rows,
+// kn <= 60 owned z nodes and at most MARCH_MAX_XCHUNK planes.
+constexpr int MARCH2_OWNED_Z = 60;
+constexpr int MARCH2_MAX_ADE = 4;
+constexpr int MARCH2_MAX_DFT = 4;
+
+// A dispersive medium whose cells fill the box [lo, hi) in C order, so the
+// cell index t follows from (i, j, k). P arrays are [pole][comp][t].
+template <typename real>
+struct March2Ade {
+  int lo[3], hi[3];
+  int n_poles;
+  int64_t n;
+  const real* abc;
+  real inv_eps;
+  const real* Pcur_in;    // P^n
+  const real* Pprev_in;   // P^{n-1}
+  real* Pcur_out;         // P^{n+2}
+  real* Pprev_out;        // P^{n+1}
+};
+
+// Capture of the first of the two steps into a DFT ring slot: slot s is
+// ring + s * slot_elems, laid out [c][box index] for component comps[c].
+template <typename real>
+struct March2Dft {
+  int n_comp;
+  int comps[6];
+  int start[3], count[3], stride[3];
+  real* ring;
+  int64_t slot_elems;
+};
+
+template <typename real>
+struct MarchArgs2 {
+  const real* Lin;        // level read (n): Ex Ey Ez Hx Hy Hz
+  real* Lout;             // level written (n + 2)
+  const real* ce;
+  const real* psiEin;     // E-side psi, level read
+  real* psiEout;          // E-side psi, level written
+  const real* psiHin;
+  real* psiHout;
+  const real* coef;       // [axis][b_e, c_e, b_h, c_h][node]
+  int nx, ny, nz;
+  int pe[3], ph[3];
+  int slab_e[3][2], slab_h[3][2];
+  real d[3], inv_d[3], ch;
+  const int* tiles;
+  int ntiles;
+  int64_t step;           // n: the pass advances n -> n + 2
+  int64_t n_steps;        // waveform table length
+  // E-side soft sources: groups sorted by (component, cell)
+  int src_beg[4];         // groups of component c: [src_beg[c], src_beg[c + 1])
+  int src_i_lo, src_i_hi; // x range of the source cells
+  const int64_t* src_cell;
+  const int32_t* src_begin;
+  const double* src_amp;
+  const int32_t* src_row;
+  const double* src_wave;
+  // media and monitors live in device memory and are read only where a
+  // node needs them, so they take no registers in the plane loop
+  int n_ade;
+  const March2Ade<real>* ade;   // [n_ade], P levels of this pass
+  int n_dft;
+  const March2Dft<real>* dft;   // [n_dft]
+  uint32_t dft_slots;           // ring slot of monitor m in bits 8m..8m+7; 0xFF: no sample
+};
+
+// pml: the tiles touch CPML slabs; nw: 10 or 12 waves per workgroup.
+template <typename real>
+void launch_march2(const MarchArgs2<real>& a, bool exact, bool pml, int nw, hipStream_t s);
+
 // Arguments of the adjoint kernels (generic in-place layout).
 template <typename real>
 struct AdjArgs {

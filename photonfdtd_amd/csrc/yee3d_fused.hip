@@ -412,33 +412,7 @@ __global__ __launch_bounds__(TY * 64) void k_yee_fused_reg(FusedArgs<real> a) {
 // likewise for the E update; H(i-1) stays in registers. Every global load of
 // plane i+1 is issued before plane i is computed. CPML membership is
 // arithmetic (two contiguous slabs per axis), so no load depends on another.
-template <typename real>
-struct Slabs {
-  // node n of an axis is in the slab set iff n < lo or n >= hi; compact index
-  int lo, hi;
-  __device__ __forceinline__ bool in(int n) const { return n < lo || n >= hi; }
-  __device__ __forceinline__ int m(int n) const { return n < lo ? n : lo + (n - hi); }
-};
-
-// Workgroup barrier ordering only LDS: the release/acquire fences are scoped
-// to the local address space, so outstanding global loads (the next plane's
-// prefetch) stay in flight across it. __syncthreads() would wait for them.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Load real at a 32-bit byte offset from a wave-uniform base (SGPR-base
-// addressing; every in-plane offset of the march kernel is < 4 GiB).
-template <typename real>
-__device__ __forceinline__ real ld(const real* base, uint32_t boff) {
-  return *reinterpret_cast<const real*>(reinterpret_cast<const char*>(base) + boff);
-}
-template <typename real>
-__device__ __forceinline__ void st(real* base, uint32_t boff, real v) {
-  *reinterpret_cast<real*>(reinterpret_cast<char*>(base) + boff) = v;
-}
+// Slabs, lds_barrier, ld and st come from yee_common.h.
 
 // DIST: how many planes ahead the loads are issued (1 or 2).
 template <typename real, bool EXACT, int NW, bool PML, int DIST>

@@ -88,4 +88,75 @@ __device__ __forceinline__ real curl_update(real f, real K, bool onA, real dA, b
   return f;
 }
 
+// CPML membership of an axis for the march kernels: node n is in the slab
+// set iff n < lo or n >= hi; m(n) is its compact psi index.
+template <typename real>
+struct Slabs {
+  int lo, hi;
+  __device__ __forceinline__ bool in(int n) const { return n < lo || n >= hi; }
+  __device__ __forceinline__ int m(int n) const { return n < lo ? n : lo + (n - hi); }
+};
+
+// Workgroup barrier ordering only LDS: the release/acquire fences are scoped
+// to the local address space, so outstanding global loads (the next plane's
+// prefetch) stay in flight across it. __syncthreads() would wait for them.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Load real at a 32-bit byte offset from a wave-uniform base (SGPR-base
+// addressing; every in-plane offset of the march kernel is < 4 GiB).
+template <typename real>
+__device__ __forceinline__ real ld(const real* base, uint32_t boff) {
+  return *reinterpret_cast<const real*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename real>
+__device__ __forceinline__ void st(real* base, uint32_t boff, real v) {
+  *reinterpret_cast<real*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// Pointer in the global address space: accesses through it compile to
+// global_* instructions. Generic (flat) accesses count against both the
+// vector-memory and the LDS counters, which makes the compiler wait for
+// every outstanding load wherever their paths merge with others.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// Buffer-resource access: wave-uniform base, 32-bit lane byte offset. A
+// lane offset of BUF_NONE lies beyond the resource's range, so that lane's
+// load returns 0 and its store is dropped, without a memory access. Masking
+// lanes this way instead of branching keeps the count of outstanding memory
+// operations the same on every path, so the compiler can wait for exactly
+// the loads it needs and leave the next plane's prefetch in flight.
+constexpr uint32_t BUF_NONE = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+template <typename real>
+__device__ __forceinline__ real bld(const real* base, uint32_t boff);
+template <>
+__device__ __forceinline__ float bld<float>(const float* base, uint32_t boff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(base), boff, 0, 0));
+}
+template <>
+__device__ __forceinline__ double bld<double>(const double* base, uint32_t boff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(buf_rsrc(base), boff, 0, 0));
+}
+template <typename real>
+__device__ __forceinline__ void bst(real* base, uint32_t boff, real v);
+template <>
+__device__ __forceinline__ void bst<float>(float* base, uint32_t boff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), buf_rsrc(base), boff, 0, 0);
+}
+template <>
+__device__ __forceinline__ void bst<double>(double* base, uint32_t boff, double v) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), buf_rsrc(base), boff, 0, 0);
+}
+
 }  // namespace pfhip
