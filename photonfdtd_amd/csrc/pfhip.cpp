@@ -718,7 +718,7 @@ Stepper<real>::Stepper(const pf_desc& d) {
   }
 
   const char* env_nw = std::getenv("PHOTONFDTD_HIP_NW");
-  if (env_nw && std::atoi(env_nw) == 16) march_nw_ = 16;
+  if (env_nw && (std::atoi(env_nw) == 16 || std::atoi(env_nw) == 12)) march_nw_ = std::atoi(env_nw);
   const char* env_dist = std::getenv("PHOTONFDTD_HIP_PREFETCH");
   if (env_dist && std::atoi(env_dist) == 1) march_dist_ = 1;
   const char* env_order = std::getenv("PHOTONFDTD_HIP_ORDER");

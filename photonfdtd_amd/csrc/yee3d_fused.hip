@@ -783,6 +783,9 @@ void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, int dist, hi
   if (nw == 16) {
     if (dist == 2) launch_march_nw<real, 16, 2>(a, exact, pml, s);
     else launch_march_nw<real, 16, 1>(a, exact, pml, s);
+  } else if (nw == 12) {
+    if (dist == 2) launch_march_nw<real, 12, 2>(a, exact, pml, s);
+    else launch_march_nw<real, 12, 1>(a, exact, pml, s);
   } else {
     if (dist == 2) launch_march_nw<real, MARCH_NW, 2>(a, exact, pml, s);
     else launch_march_nw<real, MARCH_NW, 1>(a, exact, pml, s);
