@@ -419,7 +419,9 @@ template <typename real, bool EXACT, int NW, bool PML, int DIST>
 __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
   constexpr uint32_t B = sizeof(real);
   constexpr int XS = MARCH_MAX_XCHUNK + 1;
-  __shared__ real sEx[NW * 64], sEz[NW * 64], sHx[NW * 64], sHz[NW * 64];
+  // Row exchange, double-buffered by plane parity so one barrier per plane
+  // suffices: plane i publishes H(i) and E^n(i+1) together.
+  __shared__ real sEx[2][NW * 64], sEz[2][NW * 64], sHx[2][NW * 64], sHz[2][NW * 64];
   __shared__ real sX[4][XS];    // b_e, c_e, b_h, c_h along x for the chunk
   if ((int)blockIdx.x >= a.ntiles) return;
   const MarchTile tile = reinterpret_cast<const MarchTile*>(a.tiles)[blockIdx.x];
@@ -568,11 +570,9 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
     if (i + DIST < i1) fetch(Next, i + DIST);
     const Psi& Q = P.q;
     const bool live = i >= i0;                // the plane before i0 only rebuilds H(i0-1)
-    sEx[tid] = ecur[0];
-    sEz[tid] = ecur[2];
-    lds_barrier();
-    const real ex_jp = w < NW - 1 ? sEx[min(tid + 64, NW * 64 - 1)] : real(0);
-    const real ez_jp = w < NW - 1 ? sEz[min(tid + 64, NW * 64 - 1)] : real(0);
+    const int bi = i & 1;
+    const real ex_jp = w < NW - 1 ? sEx[bi][min(tid + 64, NW * 64 - 1)] : real(0);
+    const real ez_jp = w < NW - 1 ? sEz[bi][min(tid + 64, NW * 64 - 1)] : real(0);
     const real ex_kp = __shfl_down(ecur[0], 1);
     const real ey_kp = __shfl_down(ecur[1], 1);
     // ---- H^{n+1/2}(i) ----
@@ -613,11 +613,13 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
         if (pyh && xh_in && yh_in) st(a.psiHout + hoff[5] + i * ys_h, cyh_o, nps[5]);
       }
     }
-    sHx[tid] = hx;
-    sHz[tid] = hz;
+    sHx[bi][tid] = hx;
+    sHz[bi][tid] = hz;
+    sEx[bi ^ 1][tid] = P.e[0];    // E^n(i+1) for the H update of plane i+1
+    sEz[bi ^ 1][tid] = P.e[2];
     lds_barrier();
-    const real hx_jm = w > 0 ? sHx[tid - 64] : real(0);
-    const real hz_jm = w > 0 ? sHz[tid - 64] : real(0);
+    const real hx_jm = w > 0 ? sHx[bi][tid - 64] : real(0);
+    const real hz_jm = w > 0 ? sHz[bi][tid - 64] : real(0);
     const real hx_km = __shfl_up(hx, 1);
     const real hy_km = __shfl_up(hy, 1);
     // ---- E^{n+1}(i) ----
@@ -660,6 +662,9 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
     hy_prev = hy;
     hz_prev = hz;
   };
+  sEx[ib & 1][tid] = ecur[0];
+  sEz[ib & 1][tid] = ecur[2];
+  lds_barrier();
   if constexpr (DIST == 1) {
     // Two plane buffers alternate so no register still being loaded is copied.
     Plane A, Bq;
