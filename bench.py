@@ -10,7 +10,7 @@ A step is one full Yee time step of the whole grid (H and E updates, CPML,
 ADE poles, sources and DFT accumulation). ``value`` is Gcell-updates/s over
 the timed steps, fields resident in HBM. Printed as one JSON line, together
 with the roofline of the stencil kernels (HIP events on the stepper's stream),
-the CPU baseline (the NumPy float64 oracle on a bounded 48^3 x 600-step sample) and the
+the CPU baseline (the NumPy float64 oracle on a bounded 64^3 x 1000-step sample) and the
 float64/float32 error of the GPU stepper against that oracle run.
 """
 from __future__ import annotations
@@ -126,7 +126,7 @@ def _rel(a, b):
                  max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))
 
 
-def parity_and_cpu(n=48, steps=600):
+def parity_and_cpu(n=64, steps=1000):
     """Oracle run on an n^3 sample (timed: the CPU baseline) and the GPU
     float64 / float32 errors against it (DFT planes and final fields). The
     reference operation order in float32 (the oracle run at float32) is
