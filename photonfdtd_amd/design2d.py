@@ -253,63 +253,78 @@ def predict(net, X: np.ndarray, batch: int = 256) -> np.ndarray:
     return np.concatenate(out) * net.y_sd + net.y_mu
 
 
-def propose(net, rho: np.ndarray, n: int, rng, iters: int = 40) -> np.ndarray:
-    """n continuous candidates: normalised gradient ascent on the predicted
-    objective (the network's input gradient, one batched backward pass per
-    iteration) from rho, each candidate with its own step size and start
-    perturbation; snapshots along the paths are the candidates."""
+def propose_from(net, starts: np.ndarray, iters: int = 30, step: float = 0.02,
+                 snapshots: int = 3) -> np.ndarray:
+    """Candidates from normalised gradient ascent on the network's predicted
+    objective, one path per starting design (one batched forward/backward
+    per iteration); the iterates at `snapshots` evenly spaced iterations of
+    every path are the candidates, in order (snapshot, start)."""
     import torch
-    per = max(1, n // 4)
-    x = np.clip(rho[None] + rng.uniform(0.0, 0.15, (per, 1, 1)) *
-                rng.standard_normal((per,) + rho.shape), 0, 1).astype(np.float32)
-    step = rng.uniform(0.005, 0.05, (per, 1, 1)).astype(np.float32)
-    dy = torch.tensor(np.tile(net.y_sd / N_FREQ, (per, 1)).astype(np.float32), device="cuda")
-    snaps = []
-    keep_at = {iters // 4, iters // 2, 3 * iters // 4, iters}
+    x = np.asarray(starts, dtype=np.float32).reshape(-1, N_DESIGN, N_DESIGN).copy()
+    dy = torch.tensor(np.tile(net.y_sd / N_FREQ, (len(x), 1)).astype(np.float32), device="cuda")
+    keep = {max(1, round(iters * (q + 1) / snapshots)) for q in range(snapshots)}
+    out = []
     for it in range(1, iters + 1):
-        xd = torch.tensor(x, device="cuda")
-        net.forward(xd)
+        net.forward(torch.tensor(x, device="cuda"))
         net.zero_grad()
         g = net.backward(dy, input_grad=True).cpu().numpy()
-        scale = np.abs(g).reshape(per, -1).max(axis=1)[:, None, None] + 1e-20
+        scale = np.abs(g).reshape(len(x), -1).max(axis=1)[:, None, None] + 1e-20
         x = np.clip(x + step * g / scale, 0, 1).astype(np.float32)
-        if it in keep_at:
-            snaps.append(x.copy())
+        if it in keep:
+            out.append(x.copy())
     net.zero_grad()
-    return np.concatenate(snaps)[:n]
+    return np.concatenate(out)
 
 
 def surrogate_design(pf, net, labeler: Labeler, ref_flux: np.ndarray, rho0: np.ndarray,
-                     rounds: int, target: Optional[float] = None, candidates: int = 256,
+                     rounds: int, target: Optional[float] = None, candidates: int = 48,
+                     pool: Optional[tuple] = None, iters: int = 30, step: float = 0.02,
                      finetune_epochs: int = 3, tol: float = 0.02, seed: int = 0,
                      steps: int = 4000, precision: str = "float32",
                      log: Optional[Callable] = None) -> Dict:
-    """Rounds of: propose candidates with the network, verify all of them
-    with one batched FDTD run, keep the best verified design and fine-tune the
-    network on the verified batch. If the network's relative error on the
-    kept design exceeds `tol`, also take one plain adjoint step from it."""
-    rng = np.random.default_rng(seed)
+    """Rounds of: start the network's gradient ascent from the best verified
+    designs known so far (the labelled training set, if given as `pool` =
+    (designs, spectra), plus every design verified in earlier rounds), verify
+    the candidates with one batched FDTD run, keep the best verified design
+    and fine-tune the network on the verified batch. If the network's
+    relative error on the kept design exceeds `tol`, also take one plain
+    adjoint step from it."""
     best = np.array(rho0, dtype=np.float32)
     best_j = float(objective(labeler(best[None]))[0])
+    if pool is not None:
+        P = np.asarray(pool[0], dtype=np.float32).reshape(-1, N_DESIGN, N_DESIGN)
+        PJ = objective(pool[1]).astype(np.float64)
+    else:
+        P = np.empty((0, N_DESIGN, N_DESIGN), np.float32)
+        PJ = np.empty(0)
+    P = np.concatenate([P, best[None]])
+    PJ = np.concatenate([PJ, [best_j]])
+    n_start = max(1, candidates // 3)
     hist = [dict(round=-1, objective=best_j, wall_s=0.0, fallback=False)]
     t0 = time.perf_counter()
     for r in range(rounds):
-        cand = propose(net, best, candidates, rng)
+        top = np.argsort(PJ)[::-1][:n_start]
+        cand = propose_from(net, P[top], iters=iters, step=step)[:candidates]
         pred = objective(predict(net, cand))
         spec = labeler(cand)
         ver = objective(spec)
+        P = np.concatenate([P, cand])
+        PJ = np.concatenate([PJ, ver])
         k = int(np.argmax(ver))
         err = abs(pred[k] - ver[k]) / max(abs(ver[k]), 1e-12)
         fallback = err > tol
         if ver[k] > best_j:
             best, best_j = cand[k], float(ver[k])
-        train(net, cand, spec, epochs=finetune_epochs, batch=64, lr=5e-4, seed=seed + r)
+        train(net, cand, spec, epochs=finetune_epochs, batch=min(64, len(cand)), lr=5e-4,
+              seed=seed + r)
         if fallback:
-            step = adjoint_design(pf, ref_flux, best, iters=1, steps=steps, precision=precision)
-            rho_a = step["rho"]
+            step_a = adjoint_design(pf, ref_flux, best, iters=1, steps=steps, precision=precision)
+            rho_a = step_a["rho"].astype(np.float32)
             ja = float(objective(labeler(rho_a[None]))[0])
+            P = np.concatenate([P, rho_a[None]])
+            PJ = np.concatenate([PJ, [ja]])
             if ja > best_j:
-                best, best_j = rho_a.astype(np.float32), ja
+                best, best_j = rho_a, ja
         hist.append(dict(round=r, objective=best_j, predicted=float(pred[k]), error=float(err),
                          fallback=bool(fallback), wall_s=time.perf_counter() - t0))
         if log:

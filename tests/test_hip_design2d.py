@@ -73,3 +73,46 @@ def test_design_gradient_matches_central_differences():
         fd = (value(rp) - value(rm)) / (2 * h)
         print(f"[fd] {idx}: adjoint {grad[idx]:.6e} fd {fd:.6e}", flush=True)
         assert fd == pytest.approx(grad[idx], rel=1e-3, abs=1e-5 * np.abs(grad).max())
+
+
+def test_trained_network_improves_on_its_best_designs():
+    """What the trained surrogate gains over the method without it. From the
+    16 best of 2048 FDTD-labelled designs, 30 steps of gradient ascent on the
+    trained network's predicted transmission give designs whose FDTD-verified
+    transmission beats their starting designs for most starts and beats the
+    same ascent on an untrained network (same architecture and target
+    scaling): the network carries design knowledge that the verification
+    budget alone (re-checking the best known designs) does not."""
+    import torch
+    import photonfdtd_amd as pf
+    from photonfdtd_amd import design2d as D
+    from photonfdtd_amd.surrogate import SpectralSurrogate
+    steps = 4000
+    print("\n[case] surrogate gain over its training designs", flush=True)
+    ref = D.reference_flux(pf, steps)
+    lab = D.Labeler(pf, ref, batch=256, steps=steps)
+    try:
+        X = D.training_designs(2048)
+        Y = lab(X)
+        J = D.objective(Y)
+        net = SpectralSurrogate(seed=0)
+        D.train(net, X, Y, epochs=15, batch=64, lr=1e-3)
+        torch.cuda.synchronize()
+        untrained = SpectralSurrogate(seed=1)
+        untrained.y_mu, untrained.y_sd = net.y_mu, net.y_sd
+        top = np.argsort(J)[::-1][:16]
+        got = {}
+        for name, m in (("trained", net), ("untrained", untrained)):
+            cand = D.propose_from(m, X[top], iters=30, step=0.02, snapshots=1)
+            got[name] = D.objective(lab(cand))
+    finally:
+        lab.close()
+    start = J[top]
+    gain_t = int((got["trained"] > start).sum())
+    gain_u = int((got["untrained"] > start).sum())
+    print(f"[surrogate] start mean {start.mean():.4f}; trained ascent mean "
+          f"{got['trained'].mean():.4f} ({gain_t}/16 improved), untrained "
+          f"{got['untrained'].mean():.4f} ({gain_u}/16 improved)", flush=True)
+    assert gain_t >= 12 and gain_t > gain_u
+    assert got["trained"].mean() > start.mean()
+    assert got["trained"].mean() > got["untrained"].mean() + 0.005
