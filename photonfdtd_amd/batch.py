@@ -16,6 +16,13 @@ Hz; TM: Ez, Hx, Hy); the other group stays exactly zero in 2-D, as it does in
 the reference's six-component update. Supported per simulation: CPML, soft
 sources, dispersive (ADE) media and DFT monitors. FieldMonitor and
 FluxMonitor need per-simulation host stops and are rejected.
+
+A batch of 3-D grids (every nz >= 2) runs the single-run march kernel once
+per step over the tiles of all its simulations (each tile names its
+simulation), then each simulation's ADE, source and DFT launches on the
+batch stream; only the per-step stream mode exists for 3-D. The same
+bit-for-bit parity with single runs holds. 3-D batches need the march
+kernel's conditions: E-side sources only and CPML on slabs.
 """
 from __future__ import annotations
 
@@ -49,24 +56,39 @@ def polarisation(sim) -> int:
 def _check(sim) -> np.dtype:
     npdt = validate(sim)
     nx, ny, nz = sim.grid.shape
-    if nz != 1 or nx < 2 or ny < 2:
-        raise ValueError(f"batched stepping needs an xy-plane 2-D grid, got shape {sim.grid.shape}")
+    if nx < 2 or ny < 2:
+        raise ValueError(f"batched stepping needs nx, ny >= 2, got shape {sim.grid.shape}")
     for m in sim.monitors:
         if isinstance(m, (FieldMonitor, FluxMonitor)):
             raise ValueError(f"monitor {m.name!r}: a batch supports DFTMonitor only")
-    polarisation(sim)
+    if nz == 1:
+        polarisation(sim)
+    elif any(s.component.startswith("H") for s in sim.sources):
+        raise ValueError("a 3-D batch needs E-side sources only (the march kernel's condition)")
     return npdt
 
 
 class SimBatch:
-    """n 2-D simulations of one precision advanced together on the device."""
+    """n simulations of one precision advanced together on the device: all
+    2-D (xy-plane) or all 3-D. For 3-D, ``waves`` is the wavefronts per march
+    workgroup (10, 12 or 16) and ``rows`` the x planes per tile (1-128);
+    ``None`` picks the defaults of the grid kind."""
 
-    def __init__(self, sims: Sequence, mode: str = "graph", waves: int = 4, rows: int = 64):
+    def __init__(self, sims: Sequence, mode: Optional[str] = None, waves: Optional[int] = None,
+                 rows: Optional[int] = None):
         if not sims:
             raise ValueError("empty batch")
         dts = {_check(s) for s in sims}
         if len(dts) != 1:
             raise ValueError("a batch needs one compute precision")
+        kinds = {s.grid.shape[2] > 1 for s in sims}
+        if len(kinds) != 1:
+            raise ValueError("a batch holds 2-D grids only or 3-D grids only")
+        self.is3d = kinds.pop()
+        d_mode, d_waves, d_rows = ("stream", 10, 128) if self.is3d else ("graph", 4, 64)
+        mode = d_mode if mode is None else mode
+        waves = d_waves if waves is None else waves
+        rows = d_rows if rows is None else rows
         self.npdt = dts.pop()
         self.sims = list(sims)
         self.lib = N.load_library()
@@ -79,8 +101,8 @@ class SimBatch:
         for i, sim in enumerate(sims):
             if id(sim) not in built:
                 d, k, t, slots, _ = build_desc(sim, self.npdt, exact_mode(self.npdt),
-                                               N.PF_FLAG_GENERIC)
-                built[id(sim)] = (d, slots, polarisation(sim))
+                                               0 if self.is3d else N.PF_FLAG_GENERIC)
+                built[id(sim)] = (d, slots, 0 if self.is3d else polarisation(sim))
                 keep += [d, k, t]
             d, slots, pol = built[id(sim)]
             descs[i] = d
@@ -107,7 +129,8 @@ class SimBatch:
 
     def configure(self, mode: str, waves: int = 4, rows: int = 64) -> None:
         """Launch mode, wavefronts per workgroup (1, 2, 4, 8) and owned rows
-        per wavefront chunk."""
+        per wavefront chunk; for 3-D: "stream", march wavefronts per
+        workgroup (10, 12, 16) and x planes per tile."""
         if mode not in MODES:
             raise ValueError(f"mode must be one of {sorted(MODES)}")
         N.check(self.lib, self.lib.pf_batch_configure(self.h, MODES[mode], int(waves), int(rows)),
@@ -128,12 +151,19 @@ class SimBatch:
         return dict(per_cu=per_cu.value, n_cu=n_cu.value, grid=grid.value)
 
     def tune(self, trial_steps: int = 100,
-             modes: Sequence[str] = ("graph", "stream", "persistent"),
-             waves: Sequence[int] = (2, 4, 8), rows: Sequence[int] = (16, 32, 64, 128),
+             modes: Optional[Sequence[str]] = None,
+             waves: Optional[Sequence[int]] = None, rows: Optional[Sequence[int]] = None,
              log: Optional[Callable[[str], None]] = None) -> Dict:
         """Autotuner: time every (mode, wavefronts per workgroup, rows per
         chunk) candidate over ``trial_steps`` steps from a reset state with
         HIP events, keep the fastest, and reset the batch to step 0."""
+        if self.is3d:
+            d_modes, d_waves, d_rows = ("stream",), (10, 12, 16), (32, 64, 128)
+        else:
+            d_modes, d_waves, d_rows = ("graph", "stream", "persistent"), (2, 4, 8), (16, 32, 64, 128)
+        modes = d_modes if modes is None else modes
+        waves = d_waves if waves is None else waves
+        rows = d_rows if rows is None else rows
         trial_steps = max(1, min(int(trial_steps), self.n_steps))
         trials = []
         for m in modes:
@@ -199,7 +229,7 @@ class SimBatch:
         return out
 
 
-def run_batch(sims: Sequence, mode: str = "graph", tune: bool = False,
+def run_batch(sims: Sequence, mode: Optional[str] = None, tune: bool = False,
               trial_steps: int = 100) -> List[Result]:
     """Run every simulation to its own step count; one ``Result`` each."""
     b = SimBatch(sims, mode=mode)

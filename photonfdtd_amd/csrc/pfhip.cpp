@@ -81,6 +81,13 @@ class DeviceArena {
     for (void* p : ptrs_) (void)hipFree(p);
     ptrs_.clear();
   }
+  // Free one allocation of this arena (no work may still use it).
+  void free(void* p) {
+    auto it = std::find(ptrs_.begin(), ptrs_.end(), p);
+    if (it == ptrs_.end()) return;
+    HIP_CHECK(hipFree(p));
+    ptrs_.erase(it);
+  }
 
  private:
   std::vector<void*> ptrs_;
@@ -235,7 +242,7 @@ class Stepper final : public pf_handle {
       }
       for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
       arena_.release();
-      (void)hipStreamDestroy(stream_);
+      if (own_stream_) (void)hipStreamDestroy(stream_);
     }
   }
   void advance(int64_t step0, int64_t n) override;
@@ -267,6 +274,15 @@ class Stepper final : public pf_handle {
   hipStream_t stream() const { return stream_; }
   int64_t ncell() const { return shape[0] * shape[1] * shape[2]; }
   void timing_read(double* ms, int64_t* launches) override;
+  // batched 3-D stepping support: the batch launches the march kernel for
+  // all its simulations, then each simulation runs the rest of its step.
+  void check_march_batch() const;
+  std::vector<pfhip::MarchTile> march_tile_list(int nw, int xchunk) const;
+  const pfhip::MarchArgs<real>& march_args(int p) const { return ma_[p]; }
+  int level() const { return cur_; }
+  int march_dist() const { return march_dist_; }
+  void after_march(int64_t step);
+  void adopt_stream(hipStream_t s);
 
  private:
   struct Ade {
@@ -323,6 +339,7 @@ class Stepper final : public pf_handle {
 
   DeviceArena arena_;
   hipStream_t stream_ = nullptr;
+  bool own_stream_ = true;       // false once a batch lends its stream
   bool exact_;
   int64_t n_steps_;
   StepArgs<real> args_;         // generic in-place kernels (level 0)
@@ -1020,7 +1037,7 @@ void Stepper<real>::timing_read(double* ms, int64_t* launches) {
 template <typename real>
 void Stepper<real>::step_once(int64_t step, bool monitors) {
   if (fused_) {
-    const int in = cur_, out = cur_ ^ 1;
+    const int in = cur_;
     timed(PF_K_YEE_FUSED, [&] {
       if (fa_[in].variant == 2) {
         for (int q = 0; q < 2; ++q) {
@@ -1033,16 +1050,8 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
         pfhip::launch_fused<real>(fa_[in], exact_, stream_);
       }
     });
-    timed(PF_K_AUX, [&] {
-      for (Ade& a : ade_)
-        pfhip::launch_ade_pp<real>(f_[in], f_[out], a.cells, a.n, a.Pcur, a.Pprev, a.abc,
-                                   a.n_poles, a.inv_eps, stream_);
-      cur_ = out;
-      pfhip::launch_inject<real>(fields_dev_[cur_], src_e_, n_steps_, step, stream_);
-      for (Flux& f : flux_)
-        pfhip::launch_flux<real>(f_[cur_], f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
-                                 f.dt, f.partial, f.n_part, f.total, stream_);
-    });
+    after_march(step);
+    return;
   } else {
     timed(PF_K_YEE_H, [&] { pfhip::launch_generic_h<real>(args_, exact_, stream_); });
     timed(PF_K_AUX, [&] {
@@ -1063,6 +1072,62 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
     });
   }
   if (monitors) timed(PF_K_AUX, [&] { capture_dft(step); });
+}
+
+// Rest of a fused step once the march has written level cur_ ^ 1: ADE
+// polarisation update, E-side sources, flux planes and DFT samples.
+template <typename real>
+void Stepper<real>::after_march(int64_t step) {
+  const int in = cur_, out = cur_ ^ 1;
+  timed(PF_K_AUX, [&] {
+    for (Ade& a : ade_)
+      pfhip::launch_ade_pp<real>(f_[in], f_[out], a.cells, a.n, a.Pcur, a.Pprev, a.abc,
+                                 a.n_poles, a.inv_eps, stream_);
+    cur_ = out;
+    pfhip::launch_inject<real>(fields_dev_[cur_], src_e_, n_steps_, step, stream_);
+    for (Flux& f : flux_)
+      pfhip::launch_flux<real>(f_[cur_], f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
+                               f.dt, f.partial, f.n_part, f.total, stream_);
+  });
+  timed(PF_K_AUX, [&] { capture_dft(step); });
+}
+
+// A simulation can join a batched 3-D march when it runs the march kernel
+// (fused ping-pong layout on CPML slabs, no H-side sources) and has no flux
+// planes (a batch has no call to read them).
+template <typename real>
+void Stepper<real>::check_march_batch() const {
+  if (shape[2] < 2) throw ArgError{"3-D batched stepping needs nz >= 2"};
+  if (!fused_ || fa_[0].variant != 2)
+    throw ArgError{"3-D batched stepping needs the march kernel: no H-side sources and CPML "
+                   "slabs of the form [0, lo) U [hi, n)"};
+  if (!flux_.empty()) throw ArgError{"flux monitors are not supported in a batch"};
+  if (pair_) throw ArgError{"3-D batched stepping uses the single-step march (unset PHOTONFDTD_HIP_PAIR)"};
+}
+
+// Every march tile of this grid for workgroups of nw waves and x-chunks of
+// xchunk planes (one list, all tiles through the CPML variant).
+template <typename real>
+std::vector<pfhip::MarchTile> Stepper<real>::march_tile_list(int nw, int xchunk) const {
+  const int n[3] = {args_.nx, args_.ny, args_.nz};
+  const int len[3] = {xchunk, nw - 2, 62};
+  std::vector<Seg> seg[3];
+  for (int a = 0; a < 3; ++a) seg[a] = split_axis(n[a], 0, 0, len[a], false);
+  std::vector<pfhip::MarchTile> out;
+  for (const Seg& sx : seg[0])
+    for (const Seg& sy : seg[1])
+      for (const Seg& sz : seg[2])
+        out.push_back({sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0});
+  return out;
+}
+
+// Run on the stream of a batch from now on (the batch owns and destroys it).
+template <typename real>
+void Stepper<real>::adopt_stream(hipStream_t s) {
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  if (own_stream_) HIP_CHECK(hipStreamDestroy(stream_));
+  stream_ = s;
+  own_stream_ = false;
 }
 
 template <typename real>
@@ -1715,6 +1780,150 @@ class Batch final : public BatchBase {
   double last_ms_ = 0.0;
 };
 
+// ---------------------------------------------------------------------------
+// Batch of 3-D simulations: one march launch per step covers the tiles of
+// every simulation still running (the tile's pad0 field names its
+// simulation, whose march arguments come from a device table per time
+// level); each simulation then runs its own ADE, source and DFT launches on
+// the batch stream. Simulations are ordered by decreasing step count so the
+// running ones own a prefix of the tile list; each simulation's tiles are in
+// XCD order, padded to a multiple of 8.
+// ---------------------------------------------------------------------------
+template <typename real>
+class Batch3 final : public BatchBase {
+ public:
+  Batch3(const pf_desc* descs, int n) : exact_(descs[0].exact != 0) {
+    if (n < 1) throw ArgError{"empty batch"};
+    for (int i = 1; i < n; ++i)
+      if ((descs[i].exact != 0) != exact_) throw ArgError{"a batch needs one arithmetic mode"};
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (int i = 0; i < n; ++i) {
+      pf_desc d = descs[i];
+      d.flags &= ~PF_FLAG_GENERIC;
+      sims_.emplace_back(new Stepper<real>(d));
+      sims_.back()->check_march_batch();
+      sims_.back()->adopt_stream(stream_);
+      n_steps_.push_back(d.n_steps);
+      n_steps_max_ = std::max<int64_t>(n_steps_max_, d.n_steps);
+    }
+    dist_ = sims_[0]->march_dist();
+    for (auto& s : sims_)
+      if (s->march_dist() != dist_) throw ArgError{"a batch needs one march prefetch distance"};
+    for (int p = 0; p < 2; ++p) {
+      std::vector<pfhip::MarchArgs<real>> t;
+      for (auto& s : sims_) t.push_back(s->march_args(p));
+      table_[p] = arena_.upload(t.data(), t.size(), stream_);
+    }
+    order_.resize(n);
+    for (int i = 0; i < n; ++i) order_[i] = i;
+    std::stable_sort(order_.begin(), order_.end(),
+                     [&](int a, int b) { return n_steps_[a] > n_steps_[b]; });
+    HIP_CHECK(hipEventCreate(&ev_[0]));
+    HIP_CHECK(hipEventCreate(&ev_[1]));
+    configure(0, pfhip::MARCH_NW, pfhip::MARCH_MAX_XCHUNK);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  ~Batch3() override {
+    for (hipEvent_t e : ev_)
+      if (e) (void)hipEventDestroy(e);
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      sims_.clear();
+      arena_.release();
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+  pf_handle* sim(int i) override {
+    if (i < 0 || i >= (int)sims_.size()) throw ArgError{"simulation index out of range"};
+    return sims_[i].get();
+  }
+  int size() const override { return (int)sims_.size(); }
+  double last_ms() const override { return last_ms_; }
+  void write_ce(int i, const void* ce) override {
+    if (i < 0 || i >= (int)sims_.size()) throw ArgError{"simulation index out of range"};
+    sims_[i]->write_ce(ce, stream_);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+
+  // mode: 0 only (one march launch per step); waves_per_block: wavefronts
+  // per march workgroup (10, 12 or 16); rows: x planes per tile (1 ..
+  // MARCH_MAX_XCHUNK).
+  void configure(int mode, int waves_per_block, int rows) override {
+    if (mode != 0) throw ArgError{"3-D batches run in stream mode (0)"};
+    if ((waves_per_block != 10 && waves_per_block != 12 && waves_per_block != 16) || rows < 1 ||
+        rows > pfhip::MARCH_MAX_XCHUNK)
+      throw ArgError{"bad 3-D batch configuration: waves 10, 12 or 16; rows 1 .. 128"};
+    if (waves_per_block == nw_ && rows == xchunk_) return;
+    std::vector<pfhip::MarchTile> all;
+    prefix_.assign(1, 0);
+    for (int i : order_) {
+      std::vector<pfhip::MarchTile> t = xcd_order(sims_[i]->march_tile_list(waves_per_block, rows));
+      for (pfhip::MarchTile& x : t) x.pad0 = i;
+      all.insert(all.end(), t.begin(), t.end());
+      prefix_.push_back(all.size());
+    }
+    if (all.size() > (size_t)INT32_MAX) throw ArgError{"too many march tiles in a batch"};
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    if (tiles_) arena_.free(tiles_);
+    tiles_ = reinterpret_cast<int*>(arena_.upload(all.data(), all.size(), stream_));
+    nw_ = waves_per_block;
+    xchunk_ = rows;
+  }
+
+  void reset() override {
+    for (auto& s : sims_) s->reset(stream_);
+    step_ = 0;
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+
+  void persistent_grid(int, int*, int*, int*) override {
+    throw ArgError{"3-D batches have no persistent mode"};
+  }
+
+  void advance(int64_t n) override {
+    if (n < 0 || step_ + n > n_steps_max_) throw ArgError{"step range exceeds the batch"};
+    HIP_CHECK(hipEventRecord(ev_[0], stream_));
+    for (int64_t s = step_; s < step_ + n; ++s) {
+      size_t running = 0;   // simulations with steps left: a prefix of order_
+      while (running < order_.size() && n_steps_[order_[running]] > s) ++running;
+      const int p = (int)(s & 1);
+      pfhip::MarchArgs<real> L;
+      std::memset(&L, 0, sizeof(L));
+      L.tiles = tiles_;
+      L.ntiles = (int)prefix_[running];
+      L.sims = table_[p];
+      if (L.ntiles > 0) pfhip::launch_march<real>(L, exact_, true, nw_, dist_, stream_);
+      for (size_t r = 0; r < running; ++r) {
+        Stepper<real>& sim = *sims_[order_[r]];
+        if (sim.level() != p) throw HipError{PF_E_STATE, "batch simulation out of step"};
+        sim.after_march(s);
+      }
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(ev_[1], stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    float t = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&t, ev_[0], ev_[1]));
+    last_ms_ = t;
+    step_ += n;
+  }
+
+ private:
+  std::vector<std::unique_ptr<Stepper<real>>> sims_;
+  std::vector<int64_t> n_steps_;
+  std::vector<int> order_;          // simulations by decreasing step count
+  std::vector<size_t> prefix_;      // tiles of the first r simulations of order_
+  DeviceArena arena_;
+  hipStream_t stream_ = nullptr;
+  pfhip::MarchArgs<real>* table_[2] = {nullptr, nullptr};   // per-simulation args reading level p
+  int* tiles_ = nullptr;
+  int nw_ = 0, xchunk_ = 0, dist_ = 2;
+  int64_t step_ = 0, n_steps_max_ = 0;
+  bool exact_;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  double last_ms_ = 0.0;
+};
+
 template <typename F>
 int guarded(F&& f) {
   try {
@@ -1816,10 +2025,19 @@ int pf_batch_create(const pf_desc* descs, const int32_t* pols, int32_t n, pf_bat
   return guarded([&] {
     for (int i = 1; i < n; ++i)
       if (descs[i].dtype != descs[0].dtype) throw ArgError{"a batch needs one dtype"};
+    int n3 = 0;   // 3-D grids (nz >= 2)
+    for (int i = 0; i < n; ++i) n3 += descs[i].shape[2] > 1;
+    if (n3 != 0 && n3 != n) throw ArgError{"a batch holds 2-D grids only or 3-D grids only"};
     std::unique_ptr<pf_batch> b(new pf_batch);
-    if (descs[0].dtype == PF_F32) b->impl.reset(new Batch<float>(descs, n, pols));
-    else if (descs[0].dtype == PF_F64) b->impl.reset(new Batch<double>(descs, n, pols));
-    else throw ArgError{"dtype must be PF_F32 or PF_F64"};
+    if (descs[0].dtype == PF_F32) {
+      if (n3) b->impl.reset(new Batch3<float>(descs, n));
+      else b->impl.reset(new Batch<float>(descs, n, pols));
+    } else if (descs[0].dtype == PF_F64) {
+      if (n3) b->impl.reset(new Batch3<double>(descs, n));
+      else b->impl.reset(new Batch<double>(descs, n, pols));
+    } else {
+      throw ArgError{"dtype must be PF_F32 or PF_F64"};
+    }
     *out = b.release();
   });
 }

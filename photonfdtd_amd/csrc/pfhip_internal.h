@@ -79,12 +79,16 @@ struct MarchArgs {
   int slab_e[3][2], slab_h[3][2];
   real d[3], inv_d[3], ch;
   int xchunk;
-  const int* tiles;       // [ntiles][8]: i0, i1, j0, jn, k0, kn, -, - (host-built)
+  const int* tiles;       // [ntiles][8]: i0, i1, j0, jn, k0, kn, sim, - (host-built)
   int ntiles;
+  // batched launch: per-simulation arguments indexed by the tile's sim field
+  // (nullptr for a single simulation)
+  const MarchArgs* sims;
 };
 
 // Tile of the march kernel: planes [i0, i1), owned rows j0..j0+jn-1 and owned
-// z nodes k0..k0+kn-1 (jn <= NW-2, kn <= 62). i1 <= i0 marks padding.
+// z nodes k0..k0+kn-1 (jn <= NW-2, kn <= 62). i1 <= i0 marks padding. pad0
+// is the simulation index in a batched launch.
 struct MarchTile {
   int i0, i1, j0, jn, k0, kn, pad0, pad1;
 };

@@ -415,17 +415,21 @@ __global__ __launch_bounds__(TY * 64) void k_yee_fused_reg(FusedArgs<real> a) {
 // Slabs, lds_barrier, ld and st come from yee_common.h.
 
 // DIST: how many planes ahead the loads are issued (1 or 2).
-template <typename real, bool EXACT, int NW, bool PML, int DIST>
-__global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> a) {
+// BATCH: the launch covers the tiles of several simulations; tile.pad0 names
+// the simulation, whose arguments come from the device table arg.sims (read
+// once per workgroup, before the plane loop).
+template <typename real, bool EXACT, int NW, bool PML, int DIST, bool BATCH = false>
+__global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
   constexpr uint32_t B = sizeof(real);
   constexpr int XS = MARCH_MAX_XCHUNK + 1;
   // Row exchange, double-buffered by plane parity so one barrier per plane
   // suffices: plane i publishes H(i) and E^n(i+1) together.
   __shared__ real sEx[2][NW * 64], sEz[2][NW * 64], sHx[2][NW * 64], sHz[2][NW * 64];
   __shared__ real sX[4][XS];    // b_e, c_e, b_h, c_h along x for the chunk
-  if ((int)blockIdx.x >= a.ntiles) return;
-  const MarchTile tile = reinterpret_cast<const MarchTile*>(a.tiles)[blockIdx.x];
+  if ((int)blockIdx.x >= arg.ntiles) return;
+  const MarchTile tile = reinterpret_cast<const MarchTile*>(arg.tiles)[blockIdx.x];
   if (tile.i1 <= tile.i0) return;
+  const MarchArgs<real> a = BATCH ? arg.sims[tile.pad0] : arg;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nyz = (size_t)ny * nz, ncell = (size_t)nx * nyz;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
@@ -773,6 +777,11 @@ void launch_ade_pp(real* const* Ein, real* const* Eout, const int64_t* cells, in
 
 template <typename real, int NW, int DIST>
 void launch_march_nw(MarchArgs<real>& a, bool exact, bool pml, hipStream_t s) {
+  if (a.sims) {   // batch: CPML variant for every tile
+    if (exact) k_yee_march<real, true, NW, true, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+    else k_yee_march<real, false, NW, true, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+    return;
+  }
   if (pml) {
     if (exact) k_yee_march<real, true, NW, true, DIST><<<a.ntiles, NW * 64, 0, s>>>(a);
     else k_yee_march<real, false, NW, true, DIST><<<a.ntiles, NW * 64, 0, s>>>(a);

@@ -1,4 +1,4 @@
-"""Host-side checks of the batched 2-D front end (no GPU needed)."""
+"""Host-side checks of the batched stepping front end (no GPU needed)."""
 import numpy as np
 import pytest
 
@@ -28,8 +28,11 @@ def test_polarisation_from_sources():
 def test_rejects_unsupported_scenes():
     import photonfdtd_amd as pf
     from photonfdtd_amd.batch import SimBatch
-    with pytest.raises(ValueError, match="2-D"):
-        SimBatch([_sim(pf, size=(0.8e-6, 0.8e-6, 0.8e-6))])
+    cube = (0.8e-6, 0.8e-6, 0.8e-6)
+    with pytest.raises(ValueError, match="2-D grids only or 3-D grids only"):
+        SimBatch([_sim(pf), _sim(pf, size=cube)])
+    with pytest.raises(ValueError, match="E-side sources only"):
+        SimBatch([_sim(pf, "Hz", size=cube)])
     with pytest.raises(ValueError, match="DFTMonitor only"):
         SimBatch([_sim(pf, monitors=[pf.FieldMonitor(name="f", components=("Ex",))])])
     with pytest.raises(ValueError, match="empty"):
