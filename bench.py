@@ -180,7 +180,8 @@ def measured_traffic(name):
     f = PROFILES / f"{name}_traffic.json"
     if not f.exists():
         return None
-    return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+    d = json.loads(f.read_text())
+    return d.get("hbm_bytes_per_step", d.get("hbm_bytes_per_launch"))
 
 
 def summarize(prof_dir, kind, out=None):
@@ -214,11 +215,12 @@ def summarize(prof_dir, kind, out=None):
 
 
 def traffic_from_counters(fetch_dir, write_dir, kernel, copy_cells, alg_bytes, itemsize, name,
-                          workload):
-    """HBM bytes per launch of `kernel` from one FETCH_SIZE and one WRITE_SIZE
-    rocprofv3 run. Each counter is scaled by its reading for the full-field
-    k_gather_box of the same run (a copy of copy_cells values each way).
-    Writes profiles/<name>_traffic.json."""
+                          workload, copy_fetch_cells=None):
+    """HBM bytes per step of `kernel` (all its variants) from one FETCH_SIZE
+    and one WRITE_SIZE rocprofv3 run. Each counter is scaled by its reading
+    for the full-field k_gather_box of the same run (a copy of copy_cells
+    values each way; copy_fetch_cells: the values its reads cover when the
+    field rows are padded, whole rows). Writes profiles/<name>_traffic.json."""
     import csv
     import glob
 
@@ -227,22 +229,32 @@ def traffic_from_counters(fetch_dir, write_dir, kernel, copy_cells, alg_bytes, i
         return [(r["Kernel_Name"], float(r["Counter_Value"]), int(r["Grid_Size"]))
                 for r in csv.DictReader(open(path))]
     true_kib = copy_cells * itemsize / 1024
+    true_fetch_kib = (copy_fetch_cells or copy_cells) * itemsize / 1024
     fe, wr = dispatches(fetch_dir), dispatches(write_dir)
     copy_f = [v for k, v, g in fe if "k_gather_box" in k and g >= copy_cells][0]
     copy_w = [v for k, v, g in wr if "k_gather_box" in k and g >= copy_cells][0]
-    step_f = [v for k, v, g in fe if kernel in k]
-    step_w = [v for k, v, g in wr if kernel in k]
-    fetch = sum(step_f) / len(step_f) * (true_kib / copy_f) * 1024
-    write = sum(step_w) / len(step_w) * (true_kib / copy_w) * 1024
-    out = {"kernel": kernel, "workload": workload,
-           "fetch_kib_raw_mean": sum(step_f) / len(step_f),
-           "write_kib_raw_mean": sum(step_w) / len(step_w),
+    def per_step(rows):
+        # every variant of the step kernel launches once per step (the 3-D
+        # march: psi-free tiles and CPML tiles): mean per variant, summed
+        by = {}
+        for k, v, g in rows:
+            if kernel in k:
+                by.setdefault(k, []).append(v)
+        return sum(sum(v) / len(v) for v in by.values()), sorted(by)
+    step_f, variants = per_step(fe)
+    step_w, _ = per_step(wr)
+    fetch = step_f * (true_fetch_kib / copy_f) * 1024
+    write = step_w * (true_kib / copy_w) * 1024
+    out = {"kernel": kernel, "variants": variants, "workload": workload,
+           "fetch_kib_raw_per_step": step_f,
+           "write_kib_raw_per_step": step_w,
            "copy_calibration": {"kernel": f"k_gather_box full field ({copy_cells} values each way)",
                                 "fetch_kib_raw": copy_f, "write_kib_raw": copy_w,
-                                "true_kib": true_kib, "fetch_factor": true_kib / copy_f,
+                                "true_fetch_kib": true_fetch_kib, "true_write_kib": true_kib,
+                                "fetch_factor": true_fetch_kib / copy_f,
                                 "write_factor": true_kib / copy_w},
-           "hbm_bytes_per_launch": round(fetch + write), "fetch_bytes": round(fetch),
-           "write_bytes": round(write), "algorithmic_bytes_per_launch": alg_bytes,
+           "hbm_bytes_per_step": round(fetch + write), "fetch_bytes": round(fetch),
+           "write_bytes": round(write), "algorithmic_bytes_per_step": alg_bytes,
            "traffic_over_algorithmic": (fetch + write) / alg_bytes}
     PROFILES.mkdir(exist_ok=True)
     (PROFILES / f"{name}_traffic.json").write_text(json.dumps(out, indent=1) + "\n")
@@ -639,10 +651,14 @@ def main():
                                            f"{args.precision}")
         else:
             n = args.n
+            # the march layout pads z rows to 64 slots per 62 nodes: the
+            # calibrating gather reads whole padded rows
+            zrow = 64 * ((n + 61) // 62)
             traffic_from_counters(*args.traffic, kernel="k_yee_march", copy_cells=n ** 3,
                                   alg_bytes=n ** 3 * 13 * item, itemsize=item,
                                   name=f"dispersive_slab_{n}_{args.precision}",
-                                  workload=f"3-D dispersive slab {n}^3 {args.precision}")
+                                  workload=f"3-D dispersive slab {n}^3 {args.precision}",
+                                  copy_fetch_cells=n * n * zrow)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this stepper runs on one GPU; multi-GPU runs are not "

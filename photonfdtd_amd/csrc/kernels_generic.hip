@@ -181,7 +181,8 @@ __global__ void k_ade_apply(real* const* __restrict__ fields, const int64_t* __r
 // out[(c * nbox) + flat box index] = field_c[node].
 template <typename real>
 __global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __restrict__ comps,
-                             int n_comp, BoxDev box, int64_t nyz, int nz, real* __restrict__ out) {
+                             int n_comp, BoxDev box, int64_t plane, int row, int zpad,
+                             real* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
   if (t >= nbox * n_comp) return;
@@ -190,8 +191,9 @@ __global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __
   const int64_t bk = r % box.count[2];
   const int64_t bj = (r / box.count[2]) % box.count[1];
   const int64_t bi = r / (box.count[2] * box.count[1]);
-  const int64_t node = (box.start[0] + bi * box.stride[0]) * nyz +
-                       (box.start[1] + bj * box.stride[1]) * nz + (box.start[2] + bk * box.stride[2]);
+  const int64_t node = (box.start[0] + bi * box.stride[0]) * plane +
+                       (box.start[1] + bj * box.stride[1]) * row +
+                       zslot(box.start[2] + bk * box.stride[2], zpad);
   out[t] = fields[comps[c]][node];
 }
 
@@ -230,6 +232,7 @@ __global__ __launch_bounds__(256) void k_flux_partial(const real* __restrict__ a
                                                       const real* __restrict__ b1,
                                                       const real* __restrict__ b2, int axis,
                                                       int64_t index, int nx, int ny, int nz,
+                                                      int zrow, int zpad,
                                                       double* __restrict__ partial) {
   __shared__ double red[256];
   const int64_t n1 = axis == 0 ? ny : nx;
@@ -239,9 +242,9 @@ __global__ __launch_bounds__(256) void k_flux_partial(const real* __restrict__ a
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
     const int64_t u = t / n2, v = t - (t / n2) * n2;
     int64_t node;
-    if (axis == 0) node = (index * ny + u) * nz + v;
-    else if (axis == 1) node = (u * ny + index) * nz + v;
-    else node = (u * ny + v) * nz + index;
+    if (axis == 0) node = (index * ny + u) * zrow + zslot(v, zpad);
+    else if (axis == 1) node = (u * ny + index) * zrow + zslot(v, zpad);
+    else node = (u * ny + v) * zrow + zslot(index, zpad);
     s += (double)a1[node] * (double)a2[node] - (double)b1[node] * (double)b2[node];
   }
   red[threadIdx.x] = s;
@@ -306,10 +309,10 @@ void launch_ade_apply(real* const* fields_dev, const int64_t* cells, int64_t n, 
 
 template <typename real>
 void launch_gather_box(real* const* fields_dev, const int32_t* comps, int n_comp, const BoxDev& box,
-                       int64_t nyz, int nz, real* out, hipStream_t s) {
+                       int64_t plane, int row, int zpad, real* out, hipStream_t s) {
   const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
   k_gather_box<real><<<blocks_for(nbox * n_comp, 256), 256, 0, s>>>(fields_dev, comps, n_comp, box,
-                                                                     nyz, nz, out);
+                                                                     plane, row, zpad, out);
 }
 
 template <typename real>
@@ -322,13 +325,14 @@ void launch_dft_fold(const real* ring, int64_t nbox, int n_comp, const int32_t* 
 
 template <typename real>
 void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int ny, int nz,
-                 double dA, double dt, double* partial, int n_part, double* total, hipStream_t s) {
+                 int zrow, int zpad, double dA, double dt, double* partial, int n_part,
+                 double* total, hipStream_t s) {
   // S . n: x: Ey Hz - Ez Hy; y: Ez Hx - Ex Hz; z: Ex Hy - Ey Hx (simulation.py:1446-1457)
   static const int t1[3][4] = {{1, 5, 2, 4}, {2, 3, 0, 5}, {0, 4, 1, 3}};
   const int* c = t1[axis];
   k_flux_partial<real><<<n_part, 256, 0, s>>>(fields_host[c[0]], fields_host[c[1]],
                                               fields_host[c[2]], fields_host[c[3]], axis, index, nx,
-                                              ny, nz, partial);
+                                              ny, nz, zrow, zpad, partial);
   k_flux_finish<<<1, 64, 0, s>>>(partial, n_part, dA, dt, total);
 }
 
@@ -342,12 +346,12 @@ void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int 
   template void launch_ade_apply<real>(real* const*, const int64_t*, int64_t, const real*, real*,  \
                                        real*, const real*, int, real, hipStream_t);                \
   template void launch_gather_box<real>(real* const*, const int32_t*, int, const BoxDev&, int64_t, \
-                                        int, real*, hipStream_t);                                  \
+                                        int, int, real*, hipStream_t);                             \
   template void launch_dft_fold<real>(const real*, int64_t, int, const int32_t*, int, int64_t,     \
                                       const double*, const double*, int, double, double*,          \
                                       hipStream_t);                                                \
-  template void launch_flux<real>(real* const*, int, int64_t, int, int, int, double, double,       \
-                                  double*, int, double*, hipStream_t);
+  template void launch_flux<real>(real* const*, int, int64_t, int, int, int, int, int, double,     \
+                                  double, double*, int, double*, hipStream_t);
 
 PF_INSTANTIATE(float)
 PF_INSTANTIATE(double)

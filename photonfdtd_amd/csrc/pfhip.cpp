@@ -160,6 +160,19 @@ std::vector<Seg> split_even(int n, int b0, int b1, int len) {
   return out;
 }
 
+// Dense rows [nx][ny][nz] -> padded rows [nx][ny][zrow] (pad slots zero).
+template <typename real>
+std::vector<real> pad_rows(const real* src, int nx, int ny, int nz, int zrow) {
+  std::vector<real> out((size_t)nx * ny * zrow, real(0));
+  for (int64_t r = 0; r < (int64_t)nx * ny; ++r)
+    for (int k = 0; k < nz; ++k) out[(size_t)(r * zrow + pfhip::zslot(k, 1))] = src[r * nz + k];
+  return out;
+}
+// Dense cell index -> index in padded rows.
+int64_t pad_cell(int64_t c, int nz, int zrow) {
+  return (c / nz) * zrow + pfhip::zslot(c % nz, 1);
+}
+
 // Order tiles so that the blocks the hardware sends to one XCD (block ids
 // congruent mod 8) are spatially contiguous; pad with empty tiles.
 std::vector<pfhip::MarchTile> xcd_order(const std::vector<pfhip::MarchTile>& t) {
@@ -241,6 +254,12 @@ class Stepper final : public pf_handle {
         (void)hipEventDestroy(t.b);
       }
       for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+      for (hipEvent_t e : fork_ev_)
+        if (e) (void)hipEventDestroy(e);
+      if (side_) {
+        (void)hipStreamSynchronize(side_);
+        (void)hipStreamDestroy(side_);
+      }
       arena_.release();
       if (own_stream_) (void)hipStreamDestroy(stream_);
     }
@@ -270,6 +289,12 @@ class Stepper final : public pf_handle {
   void write_ce(const void* src, hipStream_t s) {
     HIP_CHECK(hipMemcpyAsync(const_cast<real*>(args_.ce), src, (size_t)ncell() * sizeof(real),
                              hipMemcpyHostToDevice, s));
+    if (zpad_) {
+      HIP_CHECK(hipStreamSynchronize(s));
+      const std::vector<real> p = pad_rows(static_cast<const real*>(src), (int)shape[0],
+                                           (int)shape[1], (int)shape[2], zrow_);
+      HIP_CHECK(hipMemcpy(ce_f_, p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
+    }
   }
   hipStream_t stream() const { return stream_; }
   int64_t ncell() const { return shape[0] * shape[1] * shape[2]; }
@@ -350,6 +375,12 @@ class Stepper final : public pf_handle {
   pfhip::FusedArgs<real> fa_[2]; // fused-step arguments reading level p
   bool slab_ok_ = true;          // CPML nodes form [0, lo) U [hi, n) on every axis
   pfhip::MarchArgs<real> ma_[2]; // march-kernel arguments reading level p
+  // Field levels of the march use padded z rows (pfhip::zslot) when zpad_:
+  // rows of zrow_ slots, fcell_ elements per component, ce in ce_f_.
+  bool zpad_ = false;
+  int zrow_ = 0;
+  int64_t fcell_ = 0;
+  real* ce_f_ = nullptr;
   real* lvl_[2] = {nullptr, nullptr};   // six components per time level
   real* psi_side_[2] = {nullptr, nullptr};  // E-side psi, H-side psi (level 0)
   real* psi_h_lvl1_ = nullptr;           // H-side psi, level 1
@@ -358,9 +389,12 @@ class Stepper final : public pf_handle {
   int* tiles_[2] = {nullptr, nullptr};   // march tiles: [0] psi-free, [1] with CPML
   int ntiles_[2] = {0, 0};
   int march_nw_ = pfhip::MARCH_NW;
+  int pml_nw_ = pfhip::MARCH_NW;  // wavefronts per workgroup of the CPML-tile launch
   int march_dist_ = 2;            // planes of prefetch in the march kernel
   bool band_order_ = false;       // tile order: all XCDs on one x-chunk at a time
   bool split_ = false;           // separate psi-free launch for tiles off the CPML slabs
+  hipStream_t side_ = nullptr;   // second stream for the CPML-tile launch of a split step
+  hipEvent_t fork_ev_[2] = {nullptr, nullptr};
   // two-step march (temporal blocking): one launch advances two steps
   bool pair_ = false;
   int pair_nw_ = 10;
@@ -434,6 +468,8 @@ Stepper<real>::Stepper(const pf_desc& d) {
   args_.ny = ny;
   args_.nz = nz;
   args_.nyz = (int64_t)ny * nz;
+  zrow_ = nz;
+  fcell_ = ncell;
   for (int a = 0; a < 3; ++a) {
     args_.d[a] = (real)d.cell[a];
     args_.inv_d[a] = (real)(1.0 / d.cell[a]);
@@ -655,8 +691,43 @@ Stepper<real>::Stepper(const pf_desc& d) {
   const bool want_fused = !(env_fused && env_fused[0] == '0') && !(d.flags & PF_FLAG_GENERIC);
   if (want_fused && pfhip::fused_supported<real>(args_) && src_h_.n_groups == 0) {
     fused_ = true;
-    lvl_[1] = arena_.zeros<real>(6 * (size_t)ncell, stream_);
-    for (int c = 0; c < 6; ++c) f_[1][c] = lvl_[1] + (size_t)c * ncell;
+    const char* env_var = std::getenv("PHOTONFDTD_HIP_VARIANT");
+    int variant = env_var ? std::atoi(env_var) : 2;
+    if (variant == 2 && !slab_ok_) variant = 1;
+    // Padded z rows for the march (whole-line row stores); the two-step
+    // march uses dense rows.
+    const char* env_zpad = std::getenv("PHOTONFDTD_HIP_ZPAD");
+    const char* env_pair = std::getenv("PHOTONFDTD_HIP_PAIR");
+    zpad_ = variant == 2 && !(env_zpad && env_zpad[0] == '0') && !(env_pair && env_pair[0] == '1');
+    if (zpad_) {
+      zrow_ = pfhip::zpad_row(nz);
+      fcell_ = (int64_t)nx * ny * zrow_;
+      arena_.free(lvl_[0]);
+      lvl_[0] = arena_.zeros<real>(6 * (size_t)fcell_, stream_);
+      for (int c = 0; c < 6; ++c) {
+        f_[0][c] = lvl_[0] + (size_t)c * fcell_;
+        args_.f[c] = nullptr;   // the generic kernels never run on this stepper
+      }
+      fields_dev_[0] = arena_.upload(f_[0], 6, stream_);
+      const std::vector<real> cep = pad_rows(static_cast<const real*>(d.ce), nx, ny, nz, zrow_);
+      ce_f_ = arena_.upload(cep.data(), cep.size(), stream_);
+      // cell indices of the ADE media and the E-side sources in padded rows
+      size_t ia = 0;
+      for (int m = 0; m < d.n_ade; ++m) {
+        const pf_ade_desc& ad = d.ade[m];
+        if (ad.n_cells <= 0) continue;
+        std::vector<int64_t> cells(ad.cells, ad.cells + ad.n_cells);
+        for (int64_t& c : cells) c = pad_cell(c, nz, zrow_);
+        ade_[ia++].cells = arena_.upload(cells.data(), cells.size(), stream_);
+      }
+      if (src_e_.n_groups > 0) {
+        std::vector<int64_t> cells = gcell[0];
+        for (int64_t& c : cells) c = pad_cell(c, nz, zrow_);
+        src_e_.cell = arena_.upload(cells.data(), cells.size(), stream_);
+      }
+    }
+    lvl_[1] = arena_.zeros<real>(6 * (size_t)fcell_, stream_);
+    for (int c = 0; c < 6; ++c) f_[1][c] = lvl_[1] + (size_t)c * fcell_;
     fields_dev_[1] = arena_.upload(f_[1], 6, stream_);
     psi_h_lvl1_ = arena_.zeros<real>(psi_len_h_, stream_);
     real* psi_h1[6];
@@ -664,9 +735,6 @@ Stepper<real>::Stepper(const pf_desc& d) {
       psi_h1[s] = psi_h_lvl1_ ? psi_h_lvl1_ + (args_.psi[6 + s] - args_.psi[6]) : nullptr;
     const char* env_xc = std::getenv("PHOTONFDTD_HIP_XCHUNK");
     const int xchunk = env_xc ? std::max(1, std::atoi(env_xc)) : 128;
-    const char* env_var = std::getenv("PHOTONFDTD_HIP_VARIANT");
-    int variant = env_var ? std::atoi(env_var) : 2;
-    if (variant == 2 && !slab_ok_) variant = 1;
     for (int p = 0; p < 2; ++p) {
       pfhip::FusedArgs<real>& fa = fa_[p];
       std::memset(&fa, 0, sizeof(fa));
@@ -706,7 +774,9 @@ Stepper<real>::Stepper(const pf_desc& d) {
       std::memset(&ma, 0, sizeof(ma));
       ma.Lin = lvl_[p];
       ma.Lout = lvl_[p ^ 1];
-      ma.ce = args_.ce;
+      ma.ce = zpad_ ? ce_f_ : args_.ce;
+      ma.zrow = zrow_;
+      ma.zpad = zpad_ ? 1 : 0;
       ma.psiE = psi_side_[0];
       ma.psiHin = p == 0 ? psi_side_[1] : psi_h_lvl1_;
       ma.psiHout = p == 0 ? psi_h_lvl1_ : psi_side_[1];
@@ -734,14 +804,30 @@ Stepper<real>::Stepper(const pf_desc& d) {
     }
   }
 
+  // 12-wave workgroups by default with padded rows (measured fastest at
+  // 512^3 float32 there), 10 with dense rows
   const char* env_nw = std::getenv("PHOTONFDTD_HIP_NW");
-  if (env_nw && (std::atoi(env_nw) == 16 || std::atoi(env_nw) == 12)) march_nw_ = std::atoi(env_nw);
+  if (zpad_) march_nw_ = 12;
+  if (env_nw && (std::atoi(env_nw) == 16 || std::atoi(env_nw) == 12 || std::atoi(env_nw) == 10))
+    march_nw_ = std::atoi(env_nw);
   const char* env_dist = std::getenv("PHOTONFDTD_HIP_PREFETCH");
   if (env_dist && std::atoi(env_dist) == 1) march_dist_ = 1;
   const char* env_order = std::getenv("PHOTONFDTD_HIP_ORDER");
   band_order_ = env_order && env_order[0] == '1';
+  // Separate psi-free launch for the tiles off the CPML slabs: the default
+  // with padded rows (the psi-free kernel, 72 VGPRs, runs two workgroups
+  // per CU there).
   const char* env_split = std::getenv("PHOTONFDTD_HIP_SPLIT");
-  split_ = env_split && env_split[0] == '1';
+  split_ = env_split ? env_split[0] == '1' : zpad_;
+  pml_nw_ = march_nw_;
+  const char* env_pnw = std::getenv("PHOTONFDTD_HIP_PML_NW");
+  if (split_ && env_pnw && (std::atoi(env_pnw) == 10 || std::atoi(env_pnw) == 12 || std::atoi(env_pnw) == 16))
+    pml_nw_ = std::atoi(env_pnw);
+  const char* env_side = std::getenv("PHOTONFDTD_HIP_OVERLAP");
+  if (fused_ && split_ && !(env_side && env_side[0] == '0')) {
+    HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    for (hipEvent_t& e : fork_ev_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   if (fused_ && fa_[0].variant == 2) build_march_tiles(std::min(fa_[0].xchunk, pfhip::MARCH_MAX_XCHUNK));
   if (fused_ && fa_[0].variant == 2) setup_pair(std::min(fa_[0].xchunk, pfhip::MARCH_MAX_XCHUNK));
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -758,7 +844,16 @@ void Stepper<real>::build_march_tiles(int xchunk) {
   for (int a = 0; a < 3; ++a) {
     const int b0 = std::max(slab_e_[a][0], slab_h_[a][0] + 1);
     const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]);
-    seg[a] = split_ ? split_axis(n[a], b0, b1, len[a], a > 0) : split_axis(n[a], 0, 0, len[a], false);
+    if (a == 2 && zpad_) {
+      // padded rows: z tiles on the fixed 62-node grid (each owns one
+      // 64-slot block); a tile is psi-free when it lies inside [b0, b1)
+      for (int z = 0; z < n[2]; z += pfhip::ZPAD_OWN) {
+        const int c = std::min(pfhip::ZPAD_OWN, n[2] - z);
+        seg[2].push_back({z, c, z >= b0 && z + c <= b1});
+      }
+    } else {
+      seg[a] = split_ ? split_axis(n[a], b0, b1, len[a], a > 0) : split_axis(n[a], 0, 0, len[a], false);
+    }
   }
   std::vector<pfhip::MarchTile> bulk, pml;
   for (const Seg& sx : seg[0])
@@ -767,6 +862,26 @@ void Stepper<real>::build_march_tiles(int xchunk) {
         pfhip::MarchTile t{sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0};
         (split_ && sx.bulk && sy.bulk && sz.bulk ? bulk : pml).push_back(t);
       }
+  if (split_) {
+    // CPML tiles in shorter x chunks: that launch holds one workgroup per
+    // CU and few tiles, so it needs more of them to keep every CU busy
+    // (and rows for its own workgroup height)
+    const char* env_px = std::getenv("PHOTONFDTD_HIP_PML_XCHUNK");
+    const int px = std::max(1, env_px ? std::atoi(env_px) : 16);
+    const int py = pml_nw_ - 2;
+    std::vector<pfhip::MarchTile> cut;
+    for (const pfhip::MarchTile& t : pml)
+      for (int x = t.i0; x < t.i1; x += px)
+        for (int y = t.j0; y < t.j0 + t.jn; y += py) {
+          pfhip::MarchTile u = t;
+          u.i0 = x;
+          u.i1 = std::min(t.i1, x + px);
+          u.j0 = y;
+          u.jn = std::min(py, t.j0 + t.jn - y);
+          cut.push_back(u);
+        }
+    pml.swap(cut);
+  }
   const std::vector<pfhip::MarchTile>* lists[2] = {&bulk, &pml};
   for (int q = 0; q < 2; ++q) {
     std::vector<pfhip::MarchTile> ordered =
@@ -783,7 +898,7 @@ void Stepper<real>::build_march_tiles(int xchunk) {
 template <typename real>
 void Stepper<real>::setup_pair(int xchunk) {
   const char* env = std::getenv("PHOTONFDTD_HIP_PAIR");
-  if (!env || env[0] != '1') return;   // opt-in
+  if (!env || env[0] != '1' || zpad_) return;   // opt-in; dense rows only
   if (!flux_.empty() || !src_e_sorted_ || ade_.size() > (size_t)pfhip::MARCH2_MAX_ADE ||
       dft_.size() > (size_t)pfhip::MARCH2_MAX_DFT)
     return;
@@ -1040,11 +1155,24 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
     const int in = cur_;
     timed(PF_K_YEE_FUSED, [&] {
       if (fa_[in].variant == 2) {
-        for (int q = 0; q < 2; ++q) {
+        // CPML tiles on the side stream when both lists are in use, so
+        // the two launches overlap (they write disjoint tiles of level out
+        // and only read level in)
+        const bool fork = side_ && ntiles_[0] > 0 && ntiles_[1] > 0;
+        if (fork) {
+          HIP_CHECK(hipEventRecord(fork_ev_[0], stream_));
+          HIP_CHECK(hipStreamWaitEvent(side_, fork_ev_[0], 0));
+        }
+        for (int q = 1; q >= 0; --q) {
           pfhip::MarchArgs<real> ma = ma_[in];
           ma.tiles = tiles_[q];
           ma.ntiles = ntiles_[q];
-          pfhip::launch_march<real>(ma, exact_, q == 1, march_nw_, march_dist_, stream_);
+          pfhip::launch_march<real>(ma, exact_, q == 1, q == 1 ? pml_nw_ : march_nw_, march_dist_,
+                                    fork && q == 1 ? side_ : stream_);
+        }
+        if (fork) {
+          HIP_CHECK(hipEventRecord(fork_ev_[1], side_));
+          HIP_CHECK(hipStreamWaitEvent(stream_, fork_ev_[1], 0));
         }
       } else {
         pfhip::launch_fused<real>(fa_[in], exact_, stream_);
@@ -1067,7 +1195,7 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
       pfhip::launch_inject<real>(fields_dev_[0], src_e_, n_steps_, step, stream_);
       for (Flux& f : flux_)
         if (monitors)
-          pfhip::launch_flux<real>(args_.f, f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
+          pfhip::launch_flux<real>(args_.f, f.axis, f.index, args_.nx, args_.ny, args_.nz, args_.nz, 0, f.dA,
                                    f.dt, f.partial, f.n_part, f.total, stream_);
     });
   }
@@ -1086,7 +1214,8 @@ void Stepper<real>::after_march(int64_t step) {
     cur_ = out;
     pfhip::launch_inject<real>(fields_dev_[cur_], src_e_, n_steps_, step, stream_);
     for (Flux& f : flux_)
-      pfhip::launch_flux<real>(f_[cur_], f.axis, f.index, args_.nx, args_.ny, args_.nz, f.dA,
+      pfhip::launch_flux<real>(f_[cur_], f.axis, f.index, args_.nx, args_.ny, args_.nz, zrow_,
+                               zpad_ ? 1 : 0, f.dA,
                                f.dt, f.partial, f.n_part, f.total, stream_);
   });
   timed(PF_K_AUX, [&] { capture_dft(step); });
@@ -1271,7 +1400,7 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
 
 template <typename real>
 void Stepper<real>::reset(hipStream_t s) {
-  const size_t nf = 6 * (size_t)ncell();
+  const size_t nf = 6 * (size_t)fcell_;
   for (int p = 0; p < 2; ++p)
     if (lvl_[p]) HIP_CHECK(hipMemsetAsync(lvl_[p], 0, nf * sizeof(real), s));
   if (psi_len_e_) HIP_CHECK(hipMemsetAsync(psi_side_[0], 0, psi_len_e_ * sizeof(real), s));
@@ -1416,7 +1545,8 @@ void Stepper<real>::capture_dft(int64_t step) {
     if (sample >= m.n_samples) throw ArgError{"DFT sample beyond the phase table"};
     if (m.count == 0) m.sample0 = sample;
     else if (sample != m.sample0 + m.count) throw ArgError{"DFT samples must be consecutive"};
-    pfhip::launch_gather_box<real>(fields_dev_[cur_], m.comps_dev, m.n_comp, m.box, args_.nyz, args_.nz,
+    pfhip::launch_gather_box<real>(fields_dev_[cur_], m.comps_dev, m.n_comp, m.box,
+                                   (int64_t)args_.ny * zrow_, zrow_, zpad_ ? 1 : 0,
                                    m.ring + (int64_t)m.count * m.n_comp * m.nbox, stream_);
     if (++m.count == m.K) fold(m);
   }
@@ -1472,7 +1602,8 @@ void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
   const BoxDev b = check_box(box, shape);
   const int64_t nbox = b.count[0] * b.count[1] * b.count[2];
   real* out = staging(nbox);
-  pfhip::launch_gather_box<real>(fields_dev_[cur_], comp_index_dev_ + comp, 1, b, args_.nyz, args_.nz, out, stream_);
+  pfhip::launch_gather_box<real>(fields_dev_[cur_], comp_index_dev_ + comp, 1, b, (int64_t)args_.ny * zrow_,
+                                 zrow_, zpad_ ? 1 : 0, out, stream_);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(dst, out, nbox * sizeof(real), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1483,6 +1614,12 @@ void Stepper<real>::write_field(int comp, const void* src) {
   if (comp < 0 || comp > 5) throw ArgError{"component out of range"};
   if (!src) throw ArgError{"null source"};
   const int64_t ncell = shape[0] * shape[1] * shape[2];
+  if (zpad_) {
+    const std::vector<real> p = pad_rows(static_cast<const real*>(src), (int)shape[0], (int)shape[1],
+                                         (int)shape[2], zrow_);
+    HIP_CHECK(hipMemcpy(f_[cur_][comp], p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
+    return;
+  }
   HIP_CHECK(hipMemcpyAsync(f_[cur_][comp], src, ncell * sizeof(real), hipMemcpyHostToDevice, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
 }

@@ -63,6 +63,18 @@ struct FusedArgs {
   int tiles_z, tiles_y, chunks_x, grid_blocks;   // filled by the launcher
 };
 
+// Padded z rows of the march layout: node k of a row sits in slot
+// k + 2 * (k / 62), so each 62-node z tile of the march owns an aligned
+// 64-slot (256-byte in float32) block and its row stores are whole cache
+// lines; the two pad slots of a block hold no node. zpad = 0: dense rows.
+constexpr int ZPAD_OWN = 62;
+constexpr int ZPAD_BLOCK = 64;
+__host__ __device__ inline int64_t zslot(int64_t k, int zpad) {
+  return zpad ? k + 2 * (k / ZPAD_OWN) : k;
+}
+// Row length (slots) of the padded layout for nz nodes.
+inline int zpad_row(int nz) { return ZPAD_BLOCK * ((nz + ZPAD_OWN - 1) / ZPAD_OWN); }
+
 // Arguments of the march kernel: a few base pointers, everything else derived.
 constexpr int MARCH_MAX_XCHUNK = 128;
 template <typename real>
@@ -79,6 +91,7 @@ struct MarchArgs {
   int slab_e[3][2], slab_h[3][2];
   real d[3], inv_d[3], ch;
   int xchunk;
+  int zrow, zpad;         // field/ce row length in slots; 1: padded rows (zslot)
   const int* tiles;       // [ntiles][8]: i0, i1, j0, jn, k0, kn, sim, - (host-built)
   int ntiles;
   // batched launch: per-simulation arguments indexed by the tile's sim field
@@ -331,16 +344,18 @@ template <typename real>
 void launch_ade_apply(real* const* fields_dev, const int64_t* cells, int64_t n, const real* En,
                       real* Pcur, real* Pprev, const real* abc, int n_poles, real inv_eps,
                       hipStream_t s);
+// plane / row: field strides in elements (ny * row, row); zpad: padded rows.
 template <typename real>
 void launch_gather_box(real* const* fields_dev, const int32_t* comps, int n_comp, const BoxDev& box,
-                       int64_t nyz, int nz, real* out, hipStream_t s);
+                       int64_t plane, int row, int zpad, real* out, hipStream_t s);
 template <typename real>
 void launch_dft_fold(const real* ring, int64_t nbox, int n_comp, const int32_t* is_h, int count,
                      int64_t sample0, const double* phase_e, const double* phase_h, int n_freq,
                      double dt, double* acc, hipStream_t s);
 template <typename real>
 void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int ny, int nz,
-                 double dA, double dt, double* partial, int n_part, double* total, hipStream_t s);
+                 int zrow, int zpad, double dA, double dt, double* partial, int n_part,
+                 double* total, hipStream_t s);
 
 constexpr int MARCH_NW = 10;   // wavefronts per march workgroup (rows j0-1 .. j0+NW-2)
 template <typename real>

@@ -431,7 +431,11 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
   if (tile.i1 <= tile.i0) return;
   const MarchArgs<real> a = BATCH ? arg.sims[tile.pad0] : arg;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const size_t nyz = (size_t)ny * nz, ncell = (size_t)nx * nyz;
+  const size_t nyz = (size_t)ny * nz;
+  // Field and ce arrays use rows of a.zrow slots (padded rows when a.zpad:
+  // node k in slot zslot(k), each z tile owning one aligned 64-slot block);
+  // psi arrays keep dense rows of nz.
+  const size_t fplane = (size_t)ny * a.zrow, fcell = (size_t)nx * fplane;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int jn = tile.jn, kn = tile.kn;
   const int j = tile.j0 - 1 + w, k = tile.k0 - 1 + l;
@@ -446,17 +450,17 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
 
   // derived bases
   const real* Ein0 = a.Lin;
-  const real* Ein1 = a.Lin + ncell;
-  const real* Ein2 = a.Lin + 2 * ncell;
-  const real* Hin0 = a.Lin + 3 * ncell;
-  const real* Hin1 = a.Lin + 4 * ncell;
-  const real* Hin2 = a.Lin + 5 * ncell;
+  const real* Ein1 = a.Lin + fcell;
+  const real* Ein2 = a.Lin + 2 * fcell;
+  const real* Hin0 = a.Lin + 3 * fcell;
+  const real* Hin1 = a.Lin + 4 * fcell;
+  const real* Hin2 = a.Lin + 5 * fcell;
   real* Eo0 = a.Lout;
-  real* Eo1 = a.Lout + ncell;
-  real* Eo2 = a.Lout + 2 * ncell;
-  real* Ho0 = a.Lout + 3 * ncell;
-  real* Ho1 = a.Lout + 4 * ncell;
-  real* Ho2 = a.Lout + 5 * ncell;
+  real* Eo1 = a.Lout + fcell;
+  real* Eo2 = a.Lout + 2 * fcell;
+  real* Ho0 = a.Lout + 3 * fcell;
+  real* Ho1 = a.Lout + 4 * fcell;
+  real* Ho2 = a.Lout + 5 * fcell;
   const size_t sy_e = (size_t)nx * a.pe[1] * nz, sz_e = (size_t)nx * ny * a.pe[2],
                sx_e = (size_t)a.pe[0] * nyz;
   const size_t sy_h = (size_t)nx * a.ph[1] * nz, sz_h = (size_t)nx * ny * a.ph[2],
@@ -489,7 +493,18 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
   const real bze = pze ? coef_z[kc] : real(0), cze = pze ? coef_z[nz + kc] : real(0);
   const real bzh = pzh ? coef_z[2 * nz + kc] : real(0), czh = pzh ? coef_z[3 * nz + kc] : real(0);
   // byte offsets inside one plane of each layout
-  const uint32_t col = (uint32_t)(jc * nz + kc) * B;
+  const uint32_t col = (uint32_t)(jc * nz + kc) * B;   // psi (dense rows)
+  // fields and ce: load slot of this lane's node; store slot: the owned
+  // block of the tile (apron lanes 0 and 63 write its two pad slots, so a
+  // padded row store covers whole lines)
+  const uint32_t colf = (uint32_t)(jc * a.zrow + zslot(kc, a.zpad)) * B;
+  const uint32_t cols =
+      a.zpad ? (uint32_t)(jc * a.zrow + tile.k0 / ZPAD_OWN * ZPAD_BLOCK +
+                          (l == 0 ? ZPAD_OWN : l == 63 ? ZPAD_OWN + 1 : l - 1)) * B
+             : colf;
+  // lanes that store fields: the owned nodes; with padded rows every lane of
+  // an owned row (its slot is an owned node, a pad slot or past the row end)
+  const bool st_lane = a.zpad ? (j >= 0 && j < ny && w >= 1 && w <= jn) : own;
   const uint32_t cyh_o = (uint32_t)(yh.m(jc) * nz + kc) * B;
   const uint32_t czh_o = (uint32_t)(jc * a.ph[2] + zh.m(kc)) * B;
   const uint32_t cye_o = (uint32_t)(ye.m(jc) * nz + kc) * B;
@@ -541,31 +556,31 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
       Q.pe[4] = ld(pEzX + xo, col);
     }
   };  auto fetch = [&](Plane& P, int i) {
-    const size_t o = (size_t)i * nyz;
+    const size_t o = (size_t)i * fplane;
     if (valid && i + 1 < nx) {
-      const size_t on = o + nyz;
-      P.e[0] = ld(Ein0 + on, col);
-      P.e[1] = ld(Ein1 + on, col);
-      P.e[2] = ld(Ein2 + on, col);
+      const size_t on = o + fplane;
+      P.e[0] = ld(Ein0 + on, colf);
+      P.e[1] = ld(Ein1 + on, colf);
+      P.e[2] = ld(Ein2 + on, colf);
     } else {
       P.e[0] = P.e[1] = P.e[2] = real(0);
     }
     P.h[0] = P.h[1] = P.h[2] = P.ce = real(0);
     if (hrole) {
-      P.h[0] = ld(Hin0 + o, col);
-      P.h[1] = ld(Hin1 + o, col);
-      P.h[2] = ld(Hin2 + o, col);
+      P.h[0] = ld(Hin0 + o, colf);
+      P.h[1] = ld(Hin1 + o, colf);
+      P.h[2] = ld(Hin2 + o, colf);
     }
-    if (own) P.ce = ld(a.ce + o, col);
+    if (own) P.ce = ld(a.ce + o, colf);
     fetch_psi(P.q, i);
   };
 
   real ecur[3] = {0, 0, 0};
   if (valid) {
-    const size_t o = (size_t)ib * nyz;
-    ecur[0] = ld(Ein0 + o, col);
-    ecur[1] = ld(Ein1 + o, col);
-    ecur[2] = ld(Ein2 + o, col);
+    const size_t o = (size_t)ib * fplane;
+    ecur[0] = ld(Ein0 + o, colf);
+    ecur[1] = ld(Ein1 + o, colf);
+    ecur[2] = ld(Ein2 + o, colf);
   }
   real hy_prev = 0, hz_prev = 0;
   const real K_h = -a.ch;
@@ -605,10 +620,6 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
                          Q.ph[5], byh, cyh, &nps[5]);
       }
       if (own && live) {
-        const size_t o = (size_t)i * nyz;
-        st(Ho0 + o, col, hx);
-        st(Ho1 + o, col, hy);
-        st(Ho2 + o, col, hz);
         if (pyh && yh_in && zh_in) st(a.psiHout + hoff[0] + i * ys_h, cyh_o, nps[0]);
         if (pzh && yh_in && zh_in) st(a.psiHout + hoff[1] + i * zs_h, czh_o, nps[1]);
         if (pzh && zh_in && xh_in) st(a.psiHout + hoff[2] + i * zs_h, czh_o, nps[2]);
@@ -616,6 +627,12 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
         if (pxh && xh_in && yh_in) st(a.psiHout + hoff[4] + (size_t)xh.m(i) * nyz, col, nps[4]);
         if (pyh && xh_in && yh_in) st(a.psiHout + hoff[5] + i * ys_h, cyh_o, nps[5]);
       }
+    }
+    if (st_lane && live) {
+      const size_t o = (size_t)i * fplane;
+      st(Ho0 + o, cols, hx);
+      st(Ho1 + o, cols, hy);
+      st(Ho2 + o, cols, hz);
     }
     sHx[bi][tid] = hx;
     sHz[bi][tid] = hz;
@@ -627,7 +644,7 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
     const real hx_km = __shfl_up(hx, 1);
     const real hy_km = __shfl_up(hy, 1);
     // ---- E^{n+1}(i) ----
-    if (own && live) {
+    if (st_lane && live) {
       const bool xe_in = i >= 1, ye_in = j >= 1, ze_in = k >= 1;
       const bool pxe = PML && xe.in(i);
       const real bxe = sX[0][i - ib], cxe = sX[1][i - ib];
@@ -652,15 +669,15 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
         ez = curl_update(ez, K, true, dA, pxe, Q.pe[4], bxe, cxe, &npe[4], true, dB, pye,
                          Q.pe[5], bye, cye, &npe[5]);
       }
-      const size_t o = (size_t)i * nyz;
-      st(Eo0 + o, col, ex);
-      st(Eo1 + o, col, ey);
-      st(Eo2 + o, col, ez);
+      const size_t o = (size_t)i * fplane;
+      st(Eo0 + o, cols, ex);
+      st(Eo1 + o, cols, ey);
+      st(Eo2 + o, cols, ez);
       if (pye && ye_in && ze_in) st(pExY + i * ys_e, cye_o, npe[0]);
       if (pze && ye_in && ze_in) st(pExZ + i * zs_e, cze_o, npe[1]);
       if (pze && ze_in && xe_in) st(pEyZ + i * zs_e, cze_o, npe[2]);
-      if (pxe && ze_in && xe_in) st(pEyX + (size_t)xe.m(i) * nyz, col, npe[3]);
-      if (pxe && xe_in && ye_in) st(pEzX + (size_t)xe.m(i) * nyz, col, npe[4]);
+      if (own && pxe && ze_in && xe_in) st(pEyX + (size_t)xe.m(i) * nyz, col, npe[3]);
+      if (own && pxe && xe_in && ye_in) st(pEzX + (size_t)xe.m(i) * nyz, col, npe[4]);
       if (pye && xe_in && ye_in) st(pEzY + i * ys_e, cye_o, npe[5]);
     }
     hy_prev = hy;
