@@ -160,17 +160,18 @@ std::vector<Seg> split_even(int n, int b0, int b1, int len) {
   return out;
 }
 
-// Dense rows [nx][ny][nz] -> padded rows [nx][ny][zrow] (pad slots zero).
+// Dense rows [nx][ny][nz] -> padded rows [nx][ny][zrow], zpad nodes per
+// block (pad slots zero).
 template <typename real>
-std::vector<real> pad_rows(const real* src, int nx, int ny, int nz, int zrow) {
+std::vector<real> pad_rows(const real* src, int nx, int ny, int nz, int zrow, int zpad) {
   std::vector<real> out((size_t)nx * ny * zrow, real(0));
   for (int64_t r = 0; r < (int64_t)nx * ny; ++r)
-    for (int k = 0; k < nz; ++k) out[(size_t)(r * zrow + pfhip::zslot(k, 1))] = src[r * nz + k];
+    for (int k = 0; k < nz; ++k) out[(size_t)(r * zrow + pfhip::zslot(k, zpad))] = src[r * nz + k];
   return out;
 }
 // Dense cell index -> index in padded rows.
-int64_t pad_cell(int64_t c, int nz, int zrow) {
-  return (c / nz) * zrow + pfhip::zslot(c % nz, 1);
+int64_t pad_cell(int64_t c, int nz, int zrow, int zpad) {
+  return (c / nz) * zrow + pfhip::zslot(c % nz, zpad);
 }
 
 // Order tiles so that the blocks the hardware sends to one XCD (block ids
@@ -292,7 +293,7 @@ class Stepper final : public pf_handle {
     if (zpad_) {
       HIP_CHECK(hipStreamSynchronize(s));
       const std::vector<real> p = pad_rows(static_cast<const real*>(src), (int)shape[0],
-                                           (int)shape[1], (int)shape[2], zrow_);
+                                           (int)shape[1], (int)shape[2], zrow_, zpad_);
       HIP_CHECK(hipMemcpy(ce_f_, p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
     }
   }
@@ -377,7 +378,7 @@ class Stepper final : public pf_handle {
   pfhip::MarchArgs<real> ma_[2]; // march-kernel arguments reading level p
   // Field levels of the march use padded z rows (pfhip::zslot) when zpad_:
   // rows of zrow_ slots, fcell_ elements per component, ce in ce_f_.
-  bool zpad_ = false;
+  int zpad_ = 0;                 // nodes per padded block (62, or 60 with the two-step march); 0: dense
   int zrow_ = 0;
   int64_t fcell_ = 0;
   real* ce_f_ = nullptr;
@@ -694,13 +695,16 @@ Stepper<real>::Stepper(const pf_desc& d) {
     const char* env_var = std::getenv("PHOTONFDTD_HIP_VARIANT");
     int variant = env_var ? std::atoi(env_var) : 2;
     if (variant == 2 && !slab_ok_) variant = 1;
-    // Padded z rows for the march (whole-line row stores); the two-step
-    // march uses dense rows.
+    // Padded z rows for the marches (whole-line row stores).
     const char* env_zpad = std::getenv("PHOTONFDTD_HIP_ZPAD");
+    // 62-node blocks for the single-step march's tiles; 60 when the
+    // two-step march (two-node apron) shares the levels
     const char* env_pair = std::getenv("PHOTONFDTD_HIP_PAIR");
-    zpad_ = variant == 2 && !(env_zpad && env_zpad[0] == '0') && !(env_pair && env_pair[0] == '1');
+    zpad_ = variant == 2 && !(env_zpad && env_zpad[0] == '0')
+                ? (env_pair && env_pair[0] == '1' ? pfhip::MARCH2_OWNED_Z : 62)
+                : 0;
     if (zpad_) {
-      zrow_ = pfhip::zpad_row(nz);
+      zrow_ = pfhip::zpad_row(nz, zpad_);
       fcell_ = (int64_t)nx * ny * zrow_;
       arena_.free(lvl_[0]);
       lvl_[0] = arena_.zeros<real>(6 * (size_t)fcell_, stream_);
@@ -709,7 +713,7 @@ Stepper<real>::Stepper(const pf_desc& d) {
         args_.f[c] = nullptr;   // the generic kernels never run on this stepper
       }
       fields_dev_[0] = arena_.upload(f_[0], 6, stream_);
-      const std::vector<real> cep = pad_rows(static_cast<const real*>(d.ce), nx, ny, nz, zrow_);
+      const std::vector<real> cep = pad_rows(static_cast<const real*>(d.ce), nx, ny, nz, zrow_, zpad_);
       ce_f_ = arena_.upload(cep.data(), cep.size(), stream_);
       // cell indices of the ADE media and the E-side sources in padded rows
       size_t ia = 0;
@@ -717,12 +721,12 @@ Stepper<real>::Stepper(const pf_desc& d) {
         const pf_ade_desc& ad = d.ade[m];
         if (ad.n_cells <= 0) continue;
         std::vector<int64_t> cells(ad.cells, ad.cells + ad.n_cells);
-        for (int64_t& c : cells) c = pad_cell(c, nz, zrow_);
+        for (int64_t& c : cells) c = pad_cell(c, nz, zrow_, zpad_);
         ade_[ia++].cells = arena_.upload(cells.data(), cells.size(), stream_);
       }
       if (src_e_.n_groups > 0) {
         std::vector<int64_t> cells = gcell[0];
-        for (int64_t& c : cells) c = pad_cell(c, nz, zrow_);
+        for (int64_t& c : cells) c = pad_cell(c, nz, zrow_, zpad_);
         src_e_.cell = arena_.upload(cells.data(), cells.size(), stream_);
       }
     }
@@ -776,7 +780,7 @@ Stepper<real>::Stepper(const pf_desc& d) {
       ma.Lout = lvl_[p ^ 1];
       ma.ce = zpad_ ? ce_f_ : args_.ce;
       ma.zrow = zrow_;
-      ma.zpad = zpad_ ? 1 : 0;
+      ma.zpad = zpad_;
       ma.psiE = psi_side_[0];
       ma.psiHin = p == 0 ? psi_side_[1] : psi_h_lvl1_;
       ma.psiHout = p == 0 ? psi_h_lvl1_ : psi_side_[1];
@@ -818,7 +822,7 @@ Stepper<real>::Stepper(const pf_desc& d) {
   // with padded rows (the psi-free kernel, 72 VGPRs, runs two workgroups
   // per CU there).
   const char* env_split = std::getenv("PHOTONFDTD_HIP_SPLIT");
-  split_ = env_split ? env_split[0] == '1' : zpad_;
+  split_ = env_split ? env_split[0] == '1' : zpad_ != 0;
   pml_nw_ = march_nw_;
   const char* env_pnw = std::getenv("PHOTONFDTD_HIP_PML_NW");
   if (split_ && env_pnw && (std::atoi(env_pnw) == 10 || std::atoi(env_pnw) == 12 || std::atoi(env_pnw) == 16))
@@ -847,8 +851,8 @@ void Stepper<real>::build_march_tiles(int xchunk) {
     if (a == 2 && zpad_) {
       // padded rows: z tiles on the fixed 62-node grid (each owns one
       // 64-slot block); a tile is psi-free when it lies inside [b0, b1)
-      for (int z = 0; z < n[2]; z += pfhip::ZPAD_OWN) {
-        const int c = std::min(pfhip::ZPAD_OWN, n[2] - z);
+      for (int z = 0; z < n[2]; z += zpad_) {
+        const int c = std::min(zpad_, n[2] - z);
         seg[2].push_back({z, c, z >= b0 && z + c <= b1});
       }
     } else {
@@ -898,7 +902,8 @@ void Stepper<real>::build_march_tiles(int xchunk) {
 template <typename real>
 void Stepper<real>::setup_pair(int xchunk) {
   const char* env = std::getenv("PHOTONFDTD_HIP_PAIR");
-  if (!env || env[0] != '1' || zpad_) return;   // opt-in; dense rows only
+  if (!env || env[0] != '1') return;   // opt-in
+  if (zpad_ && zpad_ != pfhip::MARCH2_OWNED_Z) return;   // padded blocks of the pair's tile width
   if (!flux_.empty() || !src_e_sorted_ || ade_.size() > (size_t)pfhip::MARCH2_MAX_ADE ||
       dft_.size() > (size_t)pfhip::MARCH2_MAX_DFT)
     return;
@@ -935,6 +940,8 @@ void Stepper<real>::setup_pair(int xchunk) {
     }
   }
   m.ch = ma.ch;
+  m.zrow = ma.zrow;
+  m.zpad = ma.zpad;
   m.n_steps = n_steps_;
   for (int c = 0; c < 4; ++c) m.src_beg[c] = src_e_beg_[c];
   m.src_i_lo = src_e_ilo_;
@@ -1004,7 +1011,15 @@ void Stepper<real>::build_pair_tiles(int xchunk) {
   for (int a = 0; a < 3; ++a) {
     const int b0 = std::max(slab_e_[a][0], slab_h_[a][0]) + 2;
     const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]) - 2;
-    seg[a] = split_even(n[a], b0, b1, len[a]);
+    if (a == 2 && zpad_) {
+      // padded rows: z tiles on the fixed 60-node grid (one 64-slot block each)
+      for (int z = 0; z < n[2]; z += zpad_) {
+        const int c = std::min(zpad_, n[2] - z);
+        seg[2].push_back({z, c, z >= b0 && z + c <= b1});
+      }
+    } else {
+      seg[a] = split_even(n[a], b0, b1, len[a]);
+    }
   }
   std::vector<pfhip::MarchTile> bulk, pml;
   for (const Seg& sx : seg[0])
@@ -1014,11 +1029,26 @@ void Stepper<real>::build_pair_tiles(int xchunk) {
         if (sx.count < 1 || sx.count > pfhip::MARCH_MAX_XCHUNK || sy.count < 1 ||
             sy.count > pair_nw_ - 4 || sz.count < 1 || sz.count > pfhip::MARCH2_OWNED_Z ||
             sx.start < 0 || sx.start + sx.count > n[0] || sy.start < 0 || sy.start + sy.count > n[1] ||
-            sz.start < 0 || sz.start + sz.count > n[2])
+            sz.start < 0 || sz.start + sz.count > n[2] || (zpad_ && sz.start % zpad_ != 0))
           throw HipError{PF_E_STATE, "pair tile outside the kernel's bounds"};
         pfhip::MarchTile t{sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0};
         (sx.bulk && sy.bulk && sz.bulk ? bulk : pml).push_back(t);
       }
+  if (zpad_) {
+    // CPML tiles in shorter x chunks (their launch runs beside the psi-free
+    // one and needs more, shorter tiles to spread over the CUs)
+    const char* env_px = std::getenv("PHOTONFDTD_HIP_PAIR_PML_XCHUNK");
+    const int px = std::max(1, env_px ? std::atoi(env_px) : 32);
+    std::vector<pfhip::MarchTile> cut;
+    for (const pfhip::MarchTile& t : pml)
+      for (int x = t.i0; x < t.i1; x += px) {
+        pfhip::MarchTile u = t;
+        u.i0 = x;
+        u.i1 = std::min(t.i1, x + px);
+        cut.push_back(u);
+      }
+    pml.swap(cut);
+  }
   const std::vector<pfhip::MarchTile>* lists[2] = {&bulk, &pml};
   for (int q = 0; q < 2; ++q) {
     std::vector<pfhip::MarchTile> ordered = xcd_order(*lists[q]);
@@ -1073,10 +1103,21 @@ void Stepper<real>::step_pair(int64_t step) {
     ++d.count;
   }
   timed(PF_K_YEE_PAIR, [&] {
-    for (int q = 0; q < 2; ++q) {
+    // CPML tiles on the side stream when both lists are in use (disjoint
+    // outputs, shared read-only inputs), overlapping the psi-free launch
+    const bool fork = side_ && ntiles2_[0] > 0 && ntiles2_[1] > 0;
+    if (fork) {
+      HIP_CHECK(hipEventRecord(fork_ev_[0], stream_));
+      HIP_CHECK(hipStreamWaitEvent(side_, fork_ev_[0], 0));
+    }
+    for (int q = 1; q >= 0; --q) {
       a.tiles = tiles2_[q];
       a.ntiles = ntiles2_[q];
-      pfhip::launch_march2<real>(a, exact_, q == 1, pair_nw_, stream_);
+      pfhip::launch_march2<real>(a, exact_, q == 1, pair_nw_, fork && q == 1 ? side_ : stream_);
+    }
+    if (fork) {
+      HIP_CHECK(hipEventRecord(fork_ev_[1], side_));
+      HIP_CHECK(hipStreamWaitEvent(stream_, fork_ev_[1], 0));
     }
   });
   for (Ade& m : ade_) {
@@ -1215,7 +1256,7 @@ void Stepper<real>::after_march(int64_t step) {
     pfhip::launch_inject<real>(fields_dev_[cur_], src_e_, n_steps_, step, stream_);
     for (Flux& f : flux_)
       pfhip::launch_flux<real>(f_[cur_], f.axis, f.index, args_.nx, args_.ny, args_.nz, zrow_,
-                               zpad_ ? 1 : 0, f.dA,
+                               zpad_, f.dA,
                                f.dt, f.partial, f.n_part, f.total, stream_);
   });
   timed(PF_K_AUX, [&] { capture_dft(step); });
@@ -1239,7 +1280,7 @@ void Stepper<real>::check_march_batch() const {
 template <typename real>
 std::vector<pfhip::MarchTile> Stepper<real>::march_tile_list(int nw, int xchunk) const {
   const int n[3] = {args_.nx, args_.ny, args_.nz};
-  const int len[3] = {xchunk, nw - 2, 62};
+  const int len[3] = {xchunk, nw - 2, zpad_ ? zpad_ : 62};
   std::vector<Seg> seg[3];
   for (int a = 0; a < 3; ++a) seg[a] = split_axis(n[a], 0, 0, len[a], false);
   std::vector<pfhip::MarchTile> out;
@@ -1546,7 +1587,7 @@ void Stepper<real>::capture_dft(int64_t step) {
     if (m.count == 0) m.sample0 = sample;
     else if (sample != m.sample0 + m.count) throw ArgError{"DFT samples must be consecutive"};
     pfhip::launch_gather_box<real>(fields_dev_[cur_], m.comps_dev, m.n_comp, m.box,
-                                   (int64_t)args_.ny * zrow_, zrow_, zpad_ ? 1 : 0,
+                                   (int64_t)args_.ny * zrow_, zrow_, zpad_,
                                    m.ring + (int64_t)m.count * m.n_comp * m.nbox, stream_);
     if (++m.count == m.K) fold(m);
   }
@@ -1603,7 +1644,7 @@ void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
   const int64_t nbox = b.count[0] * b.count[1] * b.count[2];
   real* out = staging(nbox);
   pfhip::launch_gather_box<real>(fields_dev_[cur_], comp_index_dev_ + comp, 1, b, (int64_t)args_.ny * zrow_,
-                                 zrow_, zpad_ ? 1 : 0, out, stream_);
+                                 zrow_, zpad_, out, stream_);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(dst, out, nbox * sizeof(real), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1616,7 +1657,7 @@ void Stepper<real>::write_field(int comp, const void* src) {
   const int64_t ncell = shape[0] * shape[1] * shape[2];
   if (zpad_) {
     const std::vector<real> p = pad_rows(static_cast<const real*>(src), (int)shape[0], (int)shape[1],
-                                         (int)shape[2], zrow_);
+                                         (int)shape[2], zrow_, zpad_);
     HIP_CHECK(hipMemcpy(f_[cur_][comp], p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
     return;
   }

@@ -63,17 +63,18 @@ struct FusedArgs {
   int tiles_z, tiles_y, chunks_x, grid_blocks;   // filled by the launcher
 };
 
-// Padded z rows of the march layout: node k of a row sits in slot
-// k + 2 * (k / 62), so each 62-node z tile of the march owns an aligned
-// 64-slot (256-byte in float32) block and its row stores are whole cache
-// lines; the two pad slots of a block hold no node. zpad = 0: dense rows.
-constexpr int ZPAD_OWN = 62;
+// Padded z rows of the march layout: with zpad = P nodes per block (62 for
+// the single-step march, 60 when the two-step march shares the levels),
+// node k of a row sits in slot k + (64 - P) * (k / P), so each P-node z
+// tile of the marches owns an aligned 64-slot (256-byte in float32) block
+// and its row stores are whole cache lines; the pad slots of a block hold
+// no node. zpad = 0: dense rows.
 constexpr int ZPAD_BLOCK = 64;
 __host__ __device__ inline int64_t zslot(int64_t k, int zpad) {
-  return zpad ? k + 2 * (k / ZPAD_OWN) : k;
+  return zpad ? k + (ZPAD_BLOCK - zpad) * (k / zpad) : k;
 }
-// Row length (slots) of the padded layout for nz nodes.
-inline int zpad_row(int nz) { return ZPAD_BLOCK * ((nz + ZPAD_OWN - 1) / ZPAD_OWN); }
+// Row length (slots) of the padded layout for nz nodes, zpad nodes per block.
+inline int zpad_row(int nz, int zpad) { return ZPAD_BLOCK * ((nz + zpad - 1) / zpad); }
 
 // Arguments of the march kernel: a few base pointers, everything else derived.
 constexpr int MARCH_MAX_XCHUNK = 128;
@@ -91,7 +92,7 @@ struct MarchArgs {
   int slab_e[3][2], slab_h[3][2];
   real d[3], inv_d[3], ch;
   int xchunk;
-  int zrow, zpad;         // field/ce row length in slots; 1: padded rows (zslot)
+  int zrow, zpad;         // field/ce row length in slots; nodes per padded block (0: dense)
   const int* tiles;       // [ntiles][8]: i0, i1, j0, jn, k0, kn, sim, - (host-built)
   int ntiles;
   // batched launch: per-simulation arguments indexed by the tile's sim field
@@ -153,6 +154,7 @@ struct MarchArgs2 {
   int pe[3], ph[3];
   int slab_e[3][2], slab_h[3][2];
   real d[3], inv_d[3], ch;
+  int zrow, zpad;         // field/ce row length in slots; nodes per padded block (0: dense)
   const int* tiles;
   int ntiles;
   int64_t step;           // n: the pass advances n -> n + 2

@@ -433,8 +433,8 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nyz = (size_t)ny * nz;
   // Field and ce arrays use rows of a.zrow slots (padded rows when a.zpad:
-  // node k in slot zslot(k), each z tile owning one aligned 64-slot block);
-  // psi arrays keep dense rows of nz.
+  // node k in slot zslot(k), each z tile of a.zpad nodes owning one
+  // aligned 64-slot block); psi arrays keep dense rows of nz.
   const size_t fplane = (size_t)ny * a.zrow, fcell = (size_t)nx * fplane;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int jn = tile.jn, kn = tile.kn;
@@ -495,12 +495,13 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
   // byte offsets inside one plane of each layout
   const uint32_t col = (uint32_t)(jc * nz + kc) * B;   // psi (dense rows)
   // fields and ce: load slot of this lane's node; store slot: the owned
-  // block of the tile (apron lanes 0 and 63 write its two pad slots, so a
-  // padded row store covers whole lines)
+  // block of the tile (lane l >= 1 owns node slot l - 1; lane 0 and the
+  // lanes past the block's nodes write its pad slots, so a padded row store
+  // covers whole lines)
   const uint32_t colf = (uint32_t)(jc * a.zrow + zslot(kc, a.zpad)) * B;
   const uint32_t cols =
-      a.zpad ? (uint32_t)(jc * a.zrow + tile.k0 / ZPAD_OWN * ZPAD_BLOCK +
-                          (l == 0 ? ZPAD_OWN : l == 63 ? ZPAD_OWN + 1 : l - 1)) * B
+      a.zpad ? (uint32_t)(jc * a.zrow + tile.k0 / a.zpad * ZPAD_BLOCK +
+                          (l == 0 ? a.zpad : l <= a.zpad ? l - 1 : l)) * B
              : colf;
   // lanes that store fields: the owned nodes; with padded rows every lane of
   // an owned row (its slot is an owned node, a pad slot or past the row end)

@@ -49,7 +49,11 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
   const MarchTile tile = reinterpret_cast<const MarchTile*>(a.tiles)[blockIdx.x];
   if (tile.i1 <= tile.i0) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const size_t nyz = (size_t)ny * nz, ncell = (size_t)nx * nyz;
+  const size_t nyz = (size_t)ny * nz;
+  // Field and ce arrays use rows of a.zrow slots (padded rows when a.zpad:
+  // node k in slot zslot(k), each 60-node z tile (a.zpad = 60) owning one aligned 64-slot
+  // block); psi arrays keep dense rows of nz.
+  const size_t fplane = (size_t)ny * a.zrow, fcell = (size_t)nx * fplane;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int jn = tile.jn, kn = tile.kn;
   const int j = tile.j0 - 2 + w, k = tile.k0 - 2 + l;
@@ -67,17 +71,17 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
   const int64_t n = a.step;
 
   const real* Ein0 = a.Lin;
-  const real* Ein1 = a.Lin + ncell;
-  const real* Ein2 = a.Lin + 2 * ncell;
-  const real* Hin0 = a.Lin + 3 * ncell;
-  const real* Hin1 = a.Lin + 4 * ncell;
-  const real* Hin2 = a.Lin + 5 * ncell;
+  const real* Ein1 = a.Lin + fcell;
+  const real* Ein2 = a.Lin + 2 * fcell;
+  const real* Hin0 = a.Lin + 3 * fcell;
+  const real* Hin1 = a.Lin + 4 * fcell;
+  const real* Hin2 = a.Lin + 5 * fcell;
   real* Eo0 = a.Lout;
-  real* Eo1 = a.Lout + ncell;
-  real* Eo2 = a.Lout + 2 * ncell;
-  real* Ho0 = a.Lout + 3 * ncell;
-  real* Ho1 = a.Lout + 4 * ncell;
-  real* Ho2 = a.Lout + 5 * ncell;
+  real* Eo1 = a.Lout + fcell;
+  real* Eo2 = a.Lout + 2 * fcell;
+  real* Ho0 = a.Lout + 3 * fcell;
+  real* Ho1 = a.Lout + 4 * fcell;
+  real* Ho2 = a.Lout + 5 * fcell;
   const size_t sy_e = (size_t)nx * a.pe[1] * nz, sz_e = (size_t)nx * ny * a.pe[2],
                sx_e = (size_t)a.pe[0] * nyz;
   const size_t sy_h = (size_t)nx * a.ph[1] * nz, sz_h = (size_t)nx * ny * a.ph[2],
@@ -137,9 +141,22 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
   const uint32_t cze_o = (uint32_t)(jc * a.pe[2] + ze.m(kc)) * B;
   const size_t ys_h = (size_t)a.ph[1] * nz, zs_h = (size_t)ny * a.ph[2];
   const size_t ys_e = (size_t)a.pe[1] * nz, zs_e = (size_t)ny * a.pe[2];
-  // lane offsets per role (BUF_NONE where the role does not hold)
-  const uint32_t o_l = lrole ? col : BUF_NONE, o_h = hA ? col : BUF_NONE;
+  // lane offsets per role (BUF_NONE where the role does not hold); o_*
+  // address psi (dense rows), f_* the fields and ce
+  const uint32_t o_h = hA ? col : BUF_NONE;
   const uint32_t o_e = eA ? col : BUF_NONE, o_own = own ? col : BUF_NONE;
+  const uint32_t colf = (uint32_t)(jc * a.zrow + zslot(kc, a.zpad)) * B;
+  const uint32_t f_l = lrole ? colf : BUF_NONE, f_h = hA ? colf : BUF_NONE;
+  const uint32_t f_e = eA ? colf : BUF_NONE;
+  // field stores: the owned nodes; with padded rows every lane of an owned
+  // row stores into the tile's block (lanes 0, 1, 62, 63 and lanes past the
+  // grid's end into its pad slots), so a row store covers whole lines
+  const bool orow = j >= 0 && j < ny && w >= 2 && w <= jn + 1;
+  const uint32_t f_own =
+      a.zpad ? (orow ? (uint32_t)(jc * a.zrow + tile.k0 / a.zpad * ZPAD_BLOCK +
+                                  (l < 2 ? a.zpad + l : l <= a.zpad + 1 ? l - 2 : l)) * B
+                     : BUF_NONE)
+             : (own ? colf : BUF_NONE);
   const uint32_t o_pyh = pyh ? cyh_o : BUF_NONE, o_pzh = pzh ? czh_o : BUF_NONE;
   const uint32_t o_pye = pye ? cye_o : BUF_NONE, o_pze = pze ? cze_o : BUF_NONE;
   // psi stores of step B: owned nodes only
@@ -157,16 +174,16 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
   };
 
   auto fetch = [&](Plane& P, int i) __attribute__((always_inline)) {
-    const size_t o = (size_t)i * nyz;
-    const size_t on = (size_t)min(i + 1, nx - 1) * nyz;
-    const uint32_t oe = i + 1 < nx ? o_l : BUF_NONE;
+    const size_t o = (size_t)i * fplane;
+    const size_t on = (size_t)min(i + 1, nx - 1) * fplane;
+    const uint32_t oe = i + 1 < nx ? f_l : BUF_NONE;
     P.e[0] = bld(Ein0 + on, oe);
     P.e[1] = bld(Ein1 + on, oe);
     P.e[2] = bld(Ein2 + on, oe);
-    P.h[0] = bld(Hin0 + o, o_h);
-    P.h[1] = bld(Hin1 + o, o_h);
-    P.h[2] = bld(Hin2 + o, o_h);
-    P.ce = bld(a.ce + o, o_e);
+    P.h[0] = bld(Hin0 + o, f_h);
+    P.h[1] = bld(Hin1 + o, f_h);
+    P.h[2] = bld(Hin2 + o, f_h);
+    P.ce = bld(a.ce + o, f_e);
     Psi& Q = P.q;
     if (!PML) {
       for (int q = 0; q < 6; ++q) Q.ph[q] = Q.pe[q] = real(0);
@@ -289,10 +306,10 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
 
   real ecur[3];                    // E^n(p)
   {
-    const size_t o = (size_t)ib * nyz;
-    ecur[0] = bld(Ein0 + o, o_l);
-    ecur[1] = bld(Ein1 + o, o_l);
-    ecur[2] = bld(Ein2 + o, o_l);
+    const size_t o = (size_t)ib * fplane;
+    ecur[0] = bld(Ein0 + o, f_l);
+    ecur[1] = bld(Ein1 + o, f_l);
+    ecur[2] = bld(Ein2 + o, f_l);
   }
   real hA_y = 0, hA_z = 0;         // H^{n+1/2}(p-1): Hy, Hz
   real hB_y = 0, hB_z = 0;         // H^{n+3/2}(q-1): Hy, Hz
@@ -369,7 +386,7 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
       const bool owned_plane = p >= i0 && p < i1;
       if (ade_mask) ade_a(p, ecur, e, own && owned_plane);
       if (p >= a.src_i_lo && p <= a.src_i_hi) {
-        const int64_t cell = (int64_t)p * (int64_t)nyz + (int64_t)j * nz + k;
+        const int64_t cell = (int64_t)p * (int64_t)fplane + (int64_t)j * a.zrow + zslot(k, a.zpad);
         for (int c = 0; c < 3; ++c) e[c] = add_src(c, cell, n, e[c]);
       }
       if (dft_mask && own && owned_plane) capture(p, e[0], e[1], e[2], hx, hy, hz);
@@ -424,12 +441,13 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
     }
     const bool oq = q >= i0 && q < i1;           // owned plane (uniform)
     const bool store = own && oq;
-    const uint32_t os = oq ? o_own : BUF_NONE;
+    const uint32_t os = oq ? o_own : BUF_NONE;       // psi (dense rows)
+    const uint32_t fos = oq ? f_own : BUF_NONE;      // fields
     {
-      const size_t o = (size_t)q * nyz;
-      bst(Ho0 + o, os, hx);
-      bst(Ho1 + o, os, hy);
-      bst(Ho2 + o, os, hz);
+      const size_t o = (size_t)q * fplane;
+      bst(Ho0 + o, fos, hx);
+      bst(Ho1 + o, fos, hy);
+      bst(Ho2 + o, fos, hz);
       if (PML) {
         const bool yz = yh_in && zh_in, zx = zh_in && xh_in, xy = xh_in && yh_in;
         const bool xq = xh.in(q);
@@ -477,14 +495,14 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march2(MarchArgs2<real> a) {
     }
     if (store && ade_mask) ade_b(q, cE, e);
     if (store && q >= a.src_i_lo && q <= a.src_i_hi) {
-      const int64_t cell = (int64_t)q * (int64_t)nyz + (int64_t)j * nz + k;
+      const int64_t cell = (int64_t)q * (int64_t)fplane + (int64_t)j * a.zrow + zslot(k, a.zpad);
       for (int c = 0; c < 3; ++c) e[c] = add_src(c, cell, n + 1, e[c]);
     }
     {
-      const size_t o = (size_t)q * nyz;
-      bst(Eo0 + o, os, e[0]);
-      bst(Eo1 + o, os, e[1]);
-      bst(Eo2 + o, os, e[2]);
+      const size_t o = (size_t)q * fplane;
+      bst(Eo0 + o, fos, e[0]);
+      bst(Eo1 + o, fos, e[1]);
+      bst(Eo2 + o, fos, e[2]);
       if (PML) {
         const bool yz = ye_in && ze_in, zx = ze_in && xe_in, xy = xe_in && ye_in;
         const size_t xo = (size_t)(pxe ? xe.m(q) : 0) * nyz;
