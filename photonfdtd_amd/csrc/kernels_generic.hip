@@ -126,15 +126,34 @@ __global__ void k_inject(real* const* __restrict__ fields, const int32_t* __rest
                          int32_t n_groups) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_groups) return;
-  real* F = fields[g_comp[g]];
-  const int64_t cell = g_cell[g];
-  real v = F[cell];
-  for (int s = g_begin[g]; s < g_begin[g + 1]; ++s) {
-    const double add = amp[s] * wave[(int64_t)wave_row[s] * n_steps + step];
-    v = (real)((double)v + add);
-  }
-  F[cell] = v;
+  SourceGroupsDev G;
+  G.n_groups = n_groups;
+  G.comp = const_cast<int32_t*>(g_comp);
+  G.cell = const_cast<int64_t*>(g_cell);
+  G.begin = const_cast<int32_t*>(g_begin);
+  G.amp = const_cast<double*>(amp);
+  G.wave_row = const_cast<int32_t*>(wave_row);
+  G.wave = const_cast<double*>(wave);
+  inject_group<real>(fields, G, n_steps, step, g);
 }
+
+// Batched sources: job blockIdx.y, group blockIdx.x * blockDim.x + lane.
+template <typename real>
+__global__ void k_src_jobs(const SrcJob<real>* __restrict__ jobs, int64_t step) {
+  const SrcJob<real>& J = jobs[blockIdx.y];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= J.g.n_groups) return;
+  inject_group<real>(J.fields, J.g, J.n_steps, step, g);
+}
+
+template <typename real>
+void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64_t step, hipStream_t s) {
+  if (n_jobs <= 0 || max_groups <= 0) return;
+  dim3 grid((unsigned)((max_groups + 63) / 64), (unsigned)n_jobs);
+  k_src_jobs<real><<<grid, 64, 0, s>>>(jobs, step);
+}
+template void launch_src_jobs<float>(const SrcJob<float>*, int, int, int64_t, hipStream_t);
+template void launch_src_jobs<double>(const SrcJob<double>*, int, int, int64_t, hipStream_t);
 
 // E^n on a dispersive region, captured before the curl update
 // (simulation.py:638-643).

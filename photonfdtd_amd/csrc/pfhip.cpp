@@ -303,12 +303,18 @@ class Stepper final : public pf_handle {
   // batched 3-D stepping support: the batch launches the march kernel for
   // all its simulations, then each simulation runs the rest of its step.
   void check_march_batch() const;
-  std::vector<pfhip::MarchTile> march_tile_list(int nw, int xchunk) const;
+  std::vector<pfhip::MarchTile> march_tile_list(int nw, int xchunk, bool cpml) const;
   const pfhip::MarchArgs<real>& march_args(int p) const { return ma_[p]; }
   int level() const { return cur_; }
   int march_dist() const { return march_dist_; }
   void after_march(int64_t step);
   void adopt_stream(hipStream_t s);
+  // The batched ADE and source passes of a 3-D batch: this simulation's
+  // jobs for a step that reads level p; finish_step then flips the level
+  // and captures the DFT samples of that step.
+  void ade_jobs(int p, std::vector<pfhip::AdeJob<real>>& out) const;
+  bool src_job(int p, pfhip::SrcJob<real>& out) const;
+  void finish_step(int64_t step);
 
  private:
   struct Ade {
@@ -1275,20 +1281,78 @@ void Stepper<real>::check_march_batch() const {
   if (pair_) throw ArgError{"3-D batched stepping uses the single-step march (unset PHOTONFDTD_HIP_PAIR)"};
 }
 
-// Every march tile of this grid for workgroups of nw waves and x-chunks of
-// xchunk planes (one list, all tiles through the CPML variant).
+// March tiles of this grid for workgroups of nw waves and x-chunks of
+// xchunk planes, split as the single run splits them: cpml false gives the
+// psi-free tiles, true the tiles that touch a CPML slab (in x chunks of at
+// most 16 planes). With dense rows every tile is a CPML tile.
 template <typename real>
-std::vector<pfhip::MarchTile> Stepper<real>::march_tile_list(int nw, int xchunk) const {
+std::vector<pfhip::MarchTile> Stepper<real>::march_tile_list(int nw, int xchunk, bool cpml) const {
   const int n[3] = {args_.nx, args_.ny, args_.nz};
   const int len[3] = {xchunk, nw - 2, zpad_ ? zpad_ : 62};
   std::vector<Seg> seg[3];
-  for (int a = 0; a < 3; ++a) seg[a] = split_axis(n[a], 0, 0, len[a], false);
+  for (int a = 0; a < 3; ++a) {
+    const int b0 = std::max(slab_e_[a][0], slab_h_[a][0] + 1);
+    const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]);
+    if (a == 2 && zpad_) {
+      for (int z = 0; z < n[2]; z += zpad_) {
+        const int c = std::min(zpad_, n[2] - z);
+        seg[2].push_back({z, c, z >= b0 && z + c <= b1});
+      }
+    } else {
+      seg[a] = zpad_ ? split_axis(n[a], b0, b1, len[a], a > 0) : split_axis(n[a], 0, 0, len[a], false);
+    }
+  }
   std::vector<pfhip::MarchTile> out;
   for (const Seg& sx : seg[0])
     for (const Seg& sy : seg[1])
-      for (const Seg& sz : seg[2])
-        out.push_back({sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0});
+      for (const Seg& sz : seg[2]) {
+        const bool bulk = zpad_ && sx.bulk && sy.bulk && sz.bulk;
+        if (bulk == cpml) continue;
+        if (!cpml) {
+          out.push_back({sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0});
+          continue;
+        }
+        const int px = zpad_ ? 16 : sx.count;
+        for (int x = sx.start; x < sx.start + sx.count; x += px)
+          out.push_back({x, std::min(sx.start + sx.count, x + px), sy.start, sy.count, sz.start,
+                         sz.count, 0, 0});
+      }
   return out;
+}
+
+template <typename real>
+void Stepper<real>::ade_jobs(int p, std::vector<pfhip::AdeJob<real>>& out) const {
+  for (const Ade& a : ade_) {
+    if (a.n <= 0 || a.n_poles == 0) continue;
+    pfhip::AdeJob<real> J;
+    for (int c = 0; c < 3; ++c) {
+      J.Ein[c] = f_[p][c];
+      J.Eout[c] = f_[p ^ 1][c];
+    }
+    J.cells = a.cells;
+    J.n = a.n;
+    J.Pcur = a.Pcur;
+    J.Pprev = a.Pprev;
+    J.abc = a.abc;
+    J.n_poles = a.n_poles;
+    J.inv_eps = a.inv_eps;
+    out.push_back(J);
+  }
+}
+
+template <typename real>
+bool Stepper<real>::src_job(int p, pfhip::SrcJob<real>& out) const {
+  if (src_e_.n_groups == 0) return false;
+  out.fields = fields_dev_[p ^ 1];
+  out.g = src_e_;
+  out.n_steps = n_steps_;
+  return true;
+}
+
+template <typename real>
+void Stepper<real>::finish_step(int64_t step) {
+  cur_ ^= 1;
+  timed(PF_K_AUX, [&] { capture_dft(step); });
 }
 
 // Run on the stream of a batch from now on (the batch owns and destroys it).
@@ -1959,13 +2023,15 @@ class Batch final : public BatchBase {
 };
 
 // ---------------------------------------------------------------------------
-// Batch of 3-D simulations: one march launch per step covers the tiles of
-// every simulation still running (the tile's pad0 field names its
-// simulation, whose march arguments come from a device table per time
-// level); each simulation then runs its own ADE, source and DFT launches on
-// the batch stream. Simulations are ordered by decreasing step count so the
-// running ones own a prefix of the tile list; each simulation's tiles are in
-// XCD order, padded to a multiple of 8.
+// Batch of 3-D simulations: per step, one march launch covers the psi-free
+// tiles and one the CPML tiles of every simulation still running (the tile's
+// pad0 field names its simulation, whose march arguments come from a device
+// table per time level); then one launch runs the ADE media and one the
+// E-side sources of all of them (job tables per time level), and each
+// simulation captures its DFT samples on the batch stream. Simulations are
+// ordered by decreasing step count so the running ones own a prefix of the
+// tile and job lists; each simulation's tiles are in XCD order, padded to a
+// multiple of 8.
 // ---------------------------------------------------------------------------
 template <typename real>
 class Batch3 final : public BatchBase {
@@ -1996,6 +2062,27 @@ class Batch3 final : public BatchBase {
     for (int i = 0; i < n; ++i) order_[i] = i;
     std::stable_sort(order_.begin(), order_.end(),
                      [&](int a, int b) { return n_steps_[a] > n_steps_[b]; });
+    // ADE and source jobs per time level, in order_ (prefix counts per
+    // number of running simulations)
+    for (int p = 0; p < 2; ++p) {
+      std::vector<pfhip::AdeJob<real>> ade;
+      std::vector<pfhip::SrcJob<real>> src;
+      ade_prefix_[p].assign(1, 0);
+      src_prefix_[p].assign(1, 0);
+      for (int i : order_) {
+        sims_[i]->ade_jobs(p, ade);
+        pfhip::SrcJob<real> J;
+        if (sims_[i]->src_job(p, J)) {
+          src.push_back(J);
+          max_groups_ = std::max(max_groups_, J.g.n_groups);
+        }
+        ade_prefix_[p].push_back((int)ade.size());
+        src_prefix_[p].push_back((int)src.size());
+      }
+      for (const auto& J : ade) max_ade_ = std::max(max_ade_, J.n);
+      ade_tab_[p] = ade.empty() ? nullptr : arena_.upload(ade.data(), ade.size(), stream_);
+      src_tab_[p] = src.empty() ? nullptr : arena_.upload(src.data(), src.size(), stream_);
+    }
     HIP_CHECK(hipEventCreate(&ev_[0]));
     HIP_CHECK(hipEventCreate(&ev_[1]));
     configure(0, pfhip::MARCH_NW, pfhip::MARCH_MAX_XCHUNK);
@@ -2032,18 +2119,21 @@ class Batch3 final : public BatchBase {
         rows > pfhip::MARCH_MAX_XCHUNK)
       throw ArgError{"bad 3-D batch configuration: waves 10, 12 or 16; rows 1 .. 128"};
     if (waves_per_block == nw_ && rows == xchunk_) return;
-    std::vector<pfhip::MarchTile> all;
-    prefix_.assign(1, 0);
-    for (int i : order_) {
-      std::vector<pfhip::MarchTile> t = xcd_order(sims_[i]->march_tile_list(waves_per_block, rows));
-      for (pfhip::MarchTile& x : t) x.pad0 = i;
-      all.insert(all.end(), t.begin(), t.end());
-      prefix_.push_back(all.size());
-    }
-    if (all.size() > (size_t)INT32_MAX) throw ArgError{"too many march tiles in a batch"};
     HIP_CHECK(hipStreamSynchronize(stream_));
-    if (tiles_) arena_.free(tiles_);
-    tiles_ = reinterpret_cast<int*>(arena_.upload(all.data(), all.size(), stream_));
+    for (int q = 0; q < 2; ++q) {   // q = 0: psi-free tiles, 1: CPML tiles
+      std::vector<pfhip::MarchTile> all;
+      prefix_[q].assign(1, 0);
+      for (int i : order_) {
+        std::vector<pfhip::MarchTile> t =
+            xcd_order(sims_[i]->march_tile_list(waves_per_block, rows, q == 1));
+        for (pfhip::MarchTile& x : t) x.pad0 = i;
+        all.insert(all.end(), t.begin(), t.end());
+        prefix_[q].push_back(all.size());
+      }
+      if (all.size() > (size_t)INT32_MAX) throw ArgError{"too many march tiles in a batch"};
+      if (tiles_[q]) arena_.free(tiles_[q]);
+      tiles_[q] = all.empty() ? nullptr : reinterpret_cast<int*>(arena_.upload(all.data(), all.size(), stream_));
+    }
     nw_ = waves_per_block;
     xchunk_ = rows;
   }
@@ -2065,16 +2155,22 @@ class Batch3 final : public BatchBase {
       size_t running = 0;   // simulations with steps left: a prefix of order_
       while (running < order_.size() && n_steps_[order_[running]] > s) ++running;
       const int p = (int)(s & 1);
-      pfhip::MarchArgs<real> L;
-      std::memset(&L, 0, sizeof(L));
-      L.tiles = tiles_;
-      L.ntiles = (int)prefix_[running];
-      L.sims = table_[p];
-      if (L.ntiles > 0) pfhip::launch_march<real>(L, exact_, true, nw_, dist_, stream_);
+      for (int q = 0; q < 2; ++q) {   // psi-free tiles, then CPML tiles
+        pfhip::MarchArgs<real> L;
+        std::memset(&L, 0, sizeof(L));
+        L.tiles = tiles_[q];
+        L.ntiles = (int)prefix_[q][running];
+        L.sims = table_[p];
+        if (L.ntiles > 0) pfhip::launch_march<real>(L, exact_, q == 1, nw_, dist_, stream_);
+      }
+      // ADE media and E-side sources of every running simulation, then
+      // each simulation's level flip and DFT capture (the single run's order)
+      pfhip::launch_ade_jobs<real>(ade_tab_[p], ade_prefix_[p][running], max_ade_, stream_);
+      pfhip::launch_src_jobs<real>(src_tab_[p], src_prefix_[p][running], max_groups_, s, stream_);
       for (size_t r = 0; r < running; ++r) {
         Stepper<real>& sim = *sims_[order_[r]];
         if (sim.level() != p) throw HipError{PF_E_STATE, "batch simulation out of step"};
-        sim.after_march(s);
+        sim.finish_step(s);
       }
     }
     HIP_CHECK(hipGetLastError());
@@ -2090,11 +2186,16 @@ class Batch3 final : public BatchBase {
   std::vector<std::unique_ptr<Stepper<real>>> sims_;
   std::vector<int64_t> n_steps_;
   std::vector<int> order_;          // simulations by decreasing step count
-  std::vector<size_t> prefix_;      // tiles of the first r simulations of order_
+  std::vector<size_t> prefix_[2];   // tiles of the first r simulations of order_ (per list)
   DeviceArena arena_;
   hipStream_t stream_ = nullptr;
   pfhip::MarchArgs<real>* table_[2] = {nullptr, nullptr};   // per-simulation args reading level p
-  int* tiles_ = nullptr;
+  pfhip::AdeJob<real>* ade_tab_[2] = {nullptr, nullptr};    // ADE jobs reading level p, in order_
+  pfhip::SrcJob<real>* src_tab_[2] = {nullptr, nullptr};    // source jobs writing level p ^ 1
+  std::vector<int> ade_prefix_[2], src_prefix_[2];          // jobs of the first r simulations
+  int64_t max_ade_ = 0;
+  int max_groups_ = 0;
+  int* tiles_[2] = {nullptr, nullptr};   // psi-free tiles, CPML tiles
   int nw_ = 0, xchunk_ = 0, dist_ = 2;
   int64_t step_ = 0, n_steps_max_ = 0;
   bool exact_;

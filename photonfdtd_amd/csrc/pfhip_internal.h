@@ -219,6 +219,74 @@ struct SourceGroupsDev {
   double* wave = nullptr;        // [n_wave][n_steps]
 };
 
+// Soft sources of group g: summed in source order, rounded to the field
+// dtype after every add (the per-dipole `F += amp * w(t)` of the reference
+// loop, simulation.py:1366-1390).
+template <typename real>
+__device__ inline void inject_group(real* const* fields, const SourceGroupsDev& G, int64_t n_steps,
+                                    int64_t step, int g) {
+  real* F = fields[G.comp[g]];
+  const int64_t cell = G.cell[g];
+  real v = F[cell];
+  for (int s = G.begin[g]; s < G.begin[g + 1]; ++s) {
+    const double add = G.amp[s] * G.wave[(int64_t)G.wave_row[s] * n_steps + step];
+    v = (real)((double)v + add);
+  }
+  F[cell] = v;
+}
+
+// ADE correction of node t of a dispersive medium for the ping-pong layout:
+// E^n from Ein, the curl-updated E^{n+1} corrected in place in Eout; P is
+// [pole][comp][t], updated in place (simulation.py:638-669).
+template <typename real>
+__device__ inline void ade_pp_node(const real* const* Ein, real* const* Eout, const int64_t* cells,
+                                   int64_t n, real* Pcur, real* Pprev, const real* abc, int n_poles,
+                                   real inv_eps, int64_t t) {
+  const int64_t cell = cells[t];
+  for (int c = 0; c < 3; ++c) {
+    const real e = Ein[c][cell];
+    real delta = 0;
+    for (int p = 0; p < n_poles; ++p) {
+      const int64_t off = ((int64_t)p * 3 + c) * n + t;
+      const real pc = Pcur[off];
+      const real pp = Pprev[off];
+      const real pn = abc[3 * p] * pc + abc[3 * p + 1] * pp + abc[3 * p + 2] * e;
+      const real inc = pn - pc;
+      delta = p == 0 ? inc : delta + inc;
+      Pprev[off] = pc;
+      Pcur[off] = pn;
+    }
+    Eout[c][cell] = Eout[c][cell] - inv_eps * delta;
+  }
+}
+
+// Batched 3-D stepping: the ADE media and E-side sources of all simulations
+// of a batch in one launch each. One job per (simulation, medium) and per
+// simulation, for one time level; a launch takes the jobs of the running
+// simulations (a prefix of the table).
+template <typename real>
+struct AdeJob {
+  const real* Ein[3];
+  real* Eout[3];
+  const int64_t* cells;
+  int64_t n;
+  real* Pcur;
+  real* Pprev;
+  const real* abc;
+  int n_poles;
+  real inv_eps;
+};
+template <typename real>
+struct SrcJob {
+  real* const* fields;    // device table of the output level's six components
+  SourceGroupsDev g;
+  int64_t n_steps;
+};
+template <typename real>
+void launch_ade_jobs(const AdeJob<real>* jobs, int n_jobs, int64_t max_n, hipStream_t s);
+template <typename real>
+void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64_t step, hipStream_t s);
+
 // ---- batched 2-D stepping -------------------------------------------------
 struct DftDev {
   int n_comp;

@@ -743,25 +743,29 @@ __global__ void k_ade_pp(const real* __restrict__ Ein0, const real* __restrict__
                          real inv_eps) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || n_poles == 0) return;
-  const int64_t cell = cells[t];
   const real* Ein[3] = {Ein0, Ein1, Ein2};
   real* Eout[3] = {Eout0, Eout1, Eout2};
-  for (int c = 0; c < 3; ++c) {
-    const real e = Ein[c][cell];
-    real delta = 0;
-    for (int p = 0; p < n_poles; ++p) {
-      const int64_t off = ((int64_t)p * 3 + c) * n + t;
-      const real pc = Pcur[off];
-      const real pp = Pprev[off];
-      const real pn = abc[3 * p] * pc + abc[3 * p + 1] * pp + abc[3 * p + 2] * e;
-      const real inc = pn - pc;
-      delta = p == 0 ? inc : delta + inc;
-      Pprev[off] = pc;
-      Pcur[off] = pn;
-    }
-    Eout[c][cell] = Eout[c][cell] - inv_eps * delta;
-  }
+  ade_pp_node<real>(Ein, Eout, cells, n, Pcur, Pprev, abc, n_poles, inv_eps, t);
 }
+
+// Batched ADE: job blockIdx.y, node blockIdx.x * blockDim.x + lane.
+template <typename real>
+__global__ void k_ade_jobs(const AdeJob<real>* __restrict__ jobs) {
+  const AdeJob<real>& J = jobs[blockIdx.y];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= J.n || J.n_poles == 0) return;
+  ade_pp_node<real>(J.Ein, J.Eout, J.cells, J.n, J.Pcur, J.Pprev, J.abc, J.n_poles, J.inv_eps, t);
+}
+
+template <typename real>
+void launch_ade_jobs(const AdeJob<real>* jobs, int n_jobs, int64_t max_n, hipStream_t s) {
+  if (n_jobs <= 0 || max_n <= 0) return;
+  dim3 grid((unsigned)((max_n + 255) / 256), (unsigned)n_jobs);
+  k_ade_jobs<real><<<grid, 256, 0, s>>>(jobs);
+}
+template void launch_ade_jobs<float>(const AdeJob<float>*, int, int64_t, hipStream_t);
+template void launch_ade_jobs<double>(const AdeJob<double>*, int, int64_t, hipStream_t);
+
 
 template <typename real>
 bool fused_supported(const StepArgs<real>& a) {
@@ -795,9 +799,14 @@ void launch_ade_pp(real* const* Ein, real* const* Eout, const int64_t* cells, in
 
 template <typename real, int NW, int DIST>
 void launch_march_nw(MarchArgs<real>& a, bool exact, bool pml, hipStream_t s) {
-  if (a.sims) {   // batch: CPML variant for every tile
-    if (exact) k_yee_march<real, true, NW, true, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
-    else k_yee_march<real, false, NW, true, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+  if (a.sims) {   // batch
+    if (pml) {
+      if (exact) k_yee_march<real, true, NW, true, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+      else k_yee_march<real, false, NW, true, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+    } else {
+      if (exact) k_yee_march<real, true, NW, false, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+      else k_yee_march<real, false, NW, false, DIST, true><<<a.ntiles, NW * 64, 0, s>>>(a);
+    }
     return;
   }
   if (pml) {
