@@ -88,6 +88,17 @@ struct RowTE {
   real pey, pex;     // Ex psi y term (EXY), Ey psi x term (EYX)
 };
 
+// Field rows of a batched simulation are padded: column j sits in slot
+// zslot(j, MARCH_OWNED), so each 62-column strip owns an aligned 64-slot
+// block. A lane loads its column's slot; it stores into the strip's block
+// (lanes 1..62 their columns, lanes 0 and 63 the block's two pad slots), so
+// a row store covers whole lines.
+__device__ __forceinline__ int strip_load_slot(int jc) { return (int)zslot(jc, MARCH_OWNED); }
+__device__ __forceinline__ int strip_store_slot(const MarchChunk& c, int lane) {
+  return (c.j0 + 1) / MARCH_OWNED * ZPAD_BLOCK +
+         (lane == 0 ? MARCH_OWNED : lane == 63 ? MARCH_OWNED + 1 : lane - 1);
+}
+
 template <typename real, bool EXACT, bool HSRC>
 __device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk& c, int p,
                                          int64_t step) {
@@ -119,13 +130,14 @@ __device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk
   const real d0 = s.d[0], d1 = s.d[1], r0 = s.inv_d[0], r1 = s.inv_d[1];
   const real Kh = -s.ch;
   const int i0 = c.i0, i1 = c.i1, ib = i0 > 0 ? i0 - 1 : 0;
+  const int nyr = s.nyr, sj = strip_load_slot(jc), ss = strip_store_slot(c, lane);
 
   auto fetch = [&](RowTE<real>& R, int i) {
-    const int o = i * ny + jc;
+    const int o = i * nyr + sj;
     R.ex = R.ey = real(0);
     if (col && i + 1 < nx) {
-      R.ex = Ex[o + ny];
-      R.ey = Ey[o + ny];
+      R.ex = Ex[o + nyr];
+      R.ey = Ey[o + nyr];
     }
     R.hz = col ? Hz[o] : real(0);
     R.ce = own ? s.ce[o] : real(0);
@@ -137,8 +149,8 @@ __device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk
 
   real ex_c = real(0), ey_c = real(0);   // E^n(i)
   if (col) {
-    ex_c = Ex[ib * ny + jc];
-    ey_c = Ey[ib * ny + jc];
+    ex_c = Ex[ib * nyr + sj];
+    ey_c = Ey[ib * nyr + sj];
   }
   real hz_prev = real(0);                // H^{n+1/2}(i - 1)
   auto row = [&](int i, const RowTE<real>& R, RowTE<real>& N) {
@@ -157,19 +169,18 @@ __device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk
                        cyh, &nhy);
     }
     if (HSRC && col) hz = add_h_sources(s, 5, (int64_t)i * ny + j, step, hz);
+    if (live) Hzo[i * nyr + ss] = hz;   // every lane: the row store covers whole lines
     if (own && live) {
-      const int o = i * ny + j;
-      Hzo[o] = hz;
       if (hupd && pxh) hout_x[xh.m(i) * ny + j] = nhx;
       if (hupd && pyh) hout_y[i * ph1 + myh] = nhy;
     }
     const real hz_l = __shfl_up(hz, 1);
     // E^{n+1}(i)
+    real ex = ex_c, ey = ey_c;
     if (own && live) {
-      const int o = i * ny + j;
       const real K = R.ce;
       const bool pxe = xe.in(i);
-      real ex = ex_c, ey = ey_c, nex = real(0), ney = real(0), unused = real(0);
+      real nex = real(0), ney = real(0), unused = real(0);
       if (j >= 1) {
         const real dA = diff<real, EXACT>(hz, hz_l, d1, r1);
         ex = curl_update(ex, K, true, dA, pye, R.pey, bye, cye, &nex, false, real(0), false,
@@ -181,10 +192,12 @@ __device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk
         ey = curl_update(ey, K, false, real(0), false, real(0), real(0), real(0), &unused, true,
                          dB, pxe, R.pex, bxe, cxe, &ney);
       }
-      Exo[o] = ex;
-      Eyo[o] = ey;
       if (j >= 1 && pye) pe_y[i * pe1 + mye] = nex;
       if (i >= 1 && pxe) pe_x[xe.m(i) * ny + j] = ney;
+    }
+    if (live) {
+      Exo[i * nyr + ss] = ex;
+      Eyo[i * nyr + ss] = ey;
     }
     hz_prev = hz;
   };
@@ -244,10 +257,11 @@ __device__ __forceinline__ void march_tm(const SimDev<real>& s, const MarchChunk
   const real d0 = s.d[0], d1 = s.d[1], r0 = s.inv_d[0], r1 = s.inv_d[1];
   const real Kh = -s.ch;
   const int i0 = c.i0, i1 = c.i1, ib = i0 > 0 ? i0 - 1 : 0;
+  const int nyr = s.nyr, sj = strip_load_slot(jc), ss = strip_store_slot(c, lane);
 
   auto fetch = [&](RowTM<real>& R, int i) {
-    const int o = i * ny + jc;
-    R.ez = (col && i + 1 < nx) ? Ez[o + ny] : real(0);
+    const int o = i * nyr + sj;
+    R.ez = (col && i + 1 < nx) ? Ez[o + nyr] : real(0);
     R.hx = col ? Hx[o] : real(0);
     R.hy = col ? Hy[o] : real(0);
     R.ce = own ? s.ce[o] : real(0);
@@ -257,7 +271,7 @@ __device__ __forceinline__ void march_tm(const SimDev<real>& s, const MarchChunk
     R.pey = pye ? pe_y[i * pe1 + mye] : real(0);
   };
 
-  real ez_c = col ? Ez[ib * ny + jc] : real(0);   // E^n(i)
+  real ez_c = col ? Ez[ib * nyr + sj] : real(0);   // E^n(i)
   real hy_prev = real(0);                          // Hy^{n+1/2}(i - 1)
   auto row = [&](int i, const RowTM<real>& R, RowTM<real>& N) {
     if (i + 1 < i1) fetch(N, i + 1);
@@ -282,19 +296,20 @@ __device__ __forceinline__ void march_tm(const SimDev<real>& s, const MarchChunk
       hx = add_h_sources(s, 3, cell, step, hx);
       hy = add_h_sources(s, 4, cell, step, hy);
     }
+    if (live) {   // every lane: the row store covers whole lines
+      Hxo[i * nyr + ss] = hx;
+      Hyo[i * nyr + ss] = hy;
+    }
     if (own && live) {
-      const int o = i * ny + j;
-      Hxo[o] = hx;
-      Hyo[o] = hy;
       if (xupd && pyh) hout_y[i * ph1 + myh] = nhy;
       if (yupd && pxh) hout_x[xh.m(i) * ny + j] = nhx;
     }
     const real hx_l = __shfl_up(hx, 1);
+    real ez = ez_c;
     if (own && live) {
-      const int o = i * ny + j;
       const real K = R.ce;
       const bool pxe = xe.in(i);
-      real ez = ez_c, nzx = real(0), nzy = real(0);
+      real nzx = real(0), nzy = real(0);
       const bool eupd = i >= 1 && j >= 1;
       if (eupd) {
         const real bxe = pxe ? s.b_e[0][i] : real(0), cxe = pxe ? s.c_e[0][i] : real(0);
@@ -303,10 +318,10 @@ __device__ __forceinline__ void march_tm(const SimDev<real>& s, const MarchChunk
         ez = curl_update(ez, K, true, dA, pxe, R.pex, bxe, cxe, &nzx, true, dB, pye, R.pey, bye,
                          cye, &nzy);
       }
-      Ezo[o] = ez;
       if (eupd && pxe) pe_x[xe.m(i) * ny + j] = nzx;
       if (eupd && pye) pe_y[i * pe1 + mye] = nzy;
     }
+    if (live) Ezo[i * nyr + ss] = ez;
     hy_prev = hy;
   };
   RowTM<real> A, B;

@@ -290,10 +290,12 @@ class Stepper final : public pf_handle {
   void write_ce(const void* src, hipStream_t s) {
     HIP_CHECK(hipMemcpyAsync(const_cast<real*>(args_.ce), src, (size_t)ncell() * sizeof(real),
                              hipMemcpyHostToDevice, s));
-    if (zpad_) {
+    if (zpad_ || ypad_) {
       HIP_CHECK(hipStreamSynchronize(s));
-      const std::vector<real> p = pad_rows(static_cast<const real*>(src), (int)shape[0],
-                                           (int)shape[1], (int)shape[2], zrow_, zpad_);
+      const std::vector<real> p =
+          zpad_ ? pad_rows(static_cast<const real*>(src), (int)shape[0], (int)shape[1], (int)shape[2],
+                           zrow_, zpad_)
+                : pad_rows(static_cast<const real*>(src), (int)shape[0], 1, (int)shape[1], yrow_, ypad_);
       HIP_CHECK(hipMemcpy(ce_f_, p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
     }
   }
@@ -385,6 +387,9 @@ class Stepper final : public pf_handle {
   // Field levels of the march use padded z rows (pfhip::zslot) when zpad_:
   // rows of zrow_ slots, fcell_ elements per component, ce in ce_f_.
   int zpad_ = 0;                 // nodes per padded block (62, or 60 with the two-step march); 0: dense
+  // 2-D batch members: padded y rows (ypad_ columns per 64-slot block) of yrow_ slots
+  int ypad_ = 0;
+  int yrow_ = 0;
   int zrow_ = 0;
   int64_t fcell_ = 0;
   real* ce_f_ = nullptr;
@@ -1399,14 +1404,29 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
   if (dft_.size() > (size_t)pfhip::BATCH_MAX_DFT) throw ArgError{"too many DFT monitors for a batch"};
   if (ade_.size() > (size_t)pfhip::BATCH_MAX_ADE) throw ArgError{"too many dispersive media for a batch"};
   if (!flux_.empty()) throw ArgError{"flux monitors are not supported in a batch"};
-  const int64_t n = ncell();
   const int ny = (int)shape[1];
-  // second time level of the fields and of the H-side psi
-  if (!lvl_[1]) {
-    lvl_[1] = arena_.zeros<real>(6 * (size_t)n, stream_);
-    for (int c = 0; c < 6; ++c) f_[1][c] = lvl_[1] + (size_t)c * n;
-    fields_dev_[1] = arena_.upload(f_[1], 6, stream_);
+  // Two time levels with padded y rows for the batch march (each 62-column
+  // strip owns an aligned 64-slot block); this stepper's own generic
+  // kernels never run on them.
+  if (!ypad_) {
+    ypad_ = pfhip::MARCH_OWNED;
+    yrow_ = pfhip::zpad_row(ny, ypad_);
+    fcell_ = shape[0] * yrow_;
+    if (fcell_ >= (int64_t(1) << 31)) throw ArgError{"grid too large for a batch"};
+    for (int p = 0; p < 2; ++p) {
+      if (lvl_[p]) arena_.free(lvl_[p]);
+      lvl_[p] = arena_.zeros<real>(6 * (size_t)fcell_, stream_);
+      for (int c = 0; c < 6; ++c) f_[p][c] = lvl_[p] + (size_t)c * fcell_;
+      fields_dev_[p] = arena_.upload(f_[p], 6, stream_);
+    }
+    for (int c = 0; c < 6; ++c) args_.f[c] = nullptr;
+    std::vector<real> ce((size_t)ncell());
+    HIP_CHECK(hipMemcpy(ce.data(), args_.ce, ce.size() * sizeof(real), hipMemcpyDeviceToHost));
+    const std::vector<real> cep = pad_rows(ce.data(), (int)shape[0], 1, ny, yrow_, ypad_);
+    ce_f_ = arena_.upload(cep.data(), cep.size(), stream_);
   }
+  // dense node index i * ny + j -> its slot in the padded rows
+  auto slot = [&](int64_t c) { return (int)((c / ny) * yrow_ + pfhip::zslot(c % ny, ypad_)); };
   if (!psi_h_lvl1_ && psi_len_h_) psi_h_lvl1_ = arena_.zeros<real>(psi_len_h_, stream_);
   std::memset(&s, 0, sizeof(s));
   for (int c = 0; c < 6; ++c) {
@@ -1419,7 +1439,8 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
     s.psiH[0][q] = h0;
     s.psiH[1][q] = h0 ? psi_h_lvl1_ + (h0 - psi_side_[1]) : nullptr;
   }
-  s.ce = args_.ce;
+  s.ce = ce_f_;
+  s.nyr = yrow_;
   for (int a = 0; a < 2; ++a) {
     s.b_e[a] = args_.b_e[a];
     s.c_e[a] = args_.c_e[a];
@@ -1454,7 +1475,7 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
       const bool active = pol == 0 ? (comps[q] == 0 || comps[q] == 1 || comps[q] == 5)
                                    : (comps[q] == 2 || comps[q] == 3 || comps[q] == 4);
       if (!active) throw ArgError{"a source excites the idle polarisation group"};
-      if (side == 0) post.esrc.push_back({(int)cells[q], q});
+      if (side == 0) post.esrc.push_back({slot(cells[q]), q});
     }
     if (side == 1) {
       s.src_lo = *std::min_element(cells.begin(), cells.end());
@@ -1466,7 +1487,7 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
     std::vector<int64_t> cells(ade_[m].n);
     HIP_CHECK(hipMemcpy(cells.data(), ade_[m].cells, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
     for (int64_t t = 0; t < ade_[m].n; ++t)
-      post.ade.push_back({(int)cells[t], (int)(m << 26) | (int)t});
+      post.ade.push_back({slot(cells[t]), (int)(m << 26) | (int)t});
     pfhip::AdeDev<real>& A = s.ade[m];
     A.n = ade_[m].n;
     A.n_poles = ade_[m].n_poles;
@@ -1496,7 +1517,7 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
         for (int64_t bj = 0; bj < M.box.count[1]; ++bj) {
           const int i = (int)(M.box.start[0] + bi * M.box.stride[0]);
           const int j = (int)(M.box.start[1] + bj * M.box.stride[1]);
-          post.dft.push_back({i * ny + j, (int)m, c, comp, bi * M.box.count[1] + bj});
+          post.dft.push_back({slot((int64_t)i * ny + j), (int)m, c, comp, bi * M.box.count[1] + bj});
         }
     }
   }
@@ -1707,8 +1728,21 @@ void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
   const BoxDev b = check_box(box, shape);
   const int64_t nbox = b.count[0] * b.count[1] * b.count[2];
   real* out = staging(nbox);
-  pfhip::launch_gather_box<real>(fields_dev_[cur_], comp_index_dev_ + comp, 1, b, (int64_t)args_.ny * zrow_,
-                                 zrow_, zpad_, out, stream_);
+  if (ypad_) {
+    // padded y rows of a 2-D batch member: gather (x, 1, y) with y as the padded axis
+    BoxDev t = b;
+    t.start[1] = 0;
+    t.count[1] = 1;
+    t.stride[1] = 1;
+    t.start[2] = b.start[1];
+    t.count[2] = b.count[1];
+    t.stride[2] = b.stride[1];
+    pfhip::launch_gather_box<real>(fields_dev_[cur_], comp_index_dev_ + comp, 1, t, yrow_, yrow_, ypad_,
+                                   out, stream_);
+  } else {
+    pfhip::launch_gather_box<real>(fields_dev_[cur_], comp_index_dev_ + comp, 1, b,
+                                   (int64_t)args_.ny * zrow_, zrow_, zpad_, out, stream_);
+  }
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(dst, out, nbox * sizeof(real), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1719,9 +1753,11 @@ void Stepper<real>::write_field(int comp, const void* src) {
   if (comp < 0 || comp > 5) throw ArgError{"component out of range"};
   if (!src) throw ArgError{"null source"};
   const int64_t ncell = shape[0] * shape[1] * shape[2];
-  if (zpad_) {
-    const std::vector<real> p = pad_rows(static_cast<const real*>(src), (int)shape[0], (int)shape[1],
-                                         (int)shape[2], zrow_, zpad_);
+  if (zpad_ || ypad_) {
+    const std::vector<real> p =
+        zpad_ ? pad_rows(static_cast<const real*>(src), (int)shape[0], (int)shape[1], (int)shape[2],
+                         zrow_, zpad_)
+              : pad_rows(static_cast<const real*>(src), (int)shape[0], 1, (int)shape[1], yrow_, ypad_);
     HIP_CHECK(hipMemcpy(f_[cur_][comp], p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
     return;
   }

@@ -328,6 +328,7 @@ struct SimDev {
   int slab_e[2][2], slab_h[2][2];   // CPML nodes of x / y: [0, lo) U [hi, n)
   int pe[2], ph[2];                 // compact CPML extent per axis
   int nx, ny;
+  int nyr;                // field / ce row length in slots (padded rows, zslot(j, MARCH_OWNED))
   real d[2], inv_d[2], ch;
   int pol;                // 0: TE (Ex, Ey, Hz), 1: TM (Ez, Hx, Hy)
   int64_t n_steps;
