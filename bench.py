@@ -643,12 +643,17 @@ def main():
             import photonfdtd_amd as pf
             sims = batched_2d_scenes(pf, n_sims=64, steps=10, seed=0)
             cells = [s.grid.n_cells for s in sims]
+            big = sims[cells.index(max(cells))].grid.shape
+            # batched fields use padded y rows (64 slots per 62 columns): the
+            # calibrating gather of the largest scene reads whole padded rows
+            yrow = 64 * ((big[1] + 61) // 62)
             # one launch of the TE march variant covers every scene of this workload
             traffic_from_counters(*args.traffic, kernel="k_batch_march", copy_cells=max(cells),
                                   alg_bytes=sum(cells) * 7 * item, itemsize=item,
                                   name=f"batched2d_te_{args.precision}",
                                   workload=f"batched 2-D TE, 64 scenes, {sum(cells)} cells, "
-                                           f"{args.precision}")
+                                           f"{args.precision}",
+                                  copy_fetch_cells=big[0] * yrow)
         else:
             n = args.n
             # the march layout pads z rows to 64 slots per 62 nodes: the
