@@ -666,8 +666,9 @@ def main():
                                   copy_fetch_cells=n * n * zrow)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this stepper runs on one GPU; multi-GPU runs are not "
-                                   "part of this build (see the sharding design notes)",
+        print(json.dumps({"error": "this stepper runs on one GPU; runs on several GPUs are "
+                                   "not supported (how the grid would shard is described in "
+                                   "the design notes)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "batched2d":
