@@ -183,11 +183,12 @@ int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* gra
  * candidate settings from a reset state). pf_batch_persistent_grid reports
  * the occupancy and the workgroup count mode 1 would use, without
  * launching.
- * A batch of 3-D descriptors (every shape[2] >= 2; pols ignored) runs the
- * single-run march kernel once per step over the tiles of all simulations
- * still running, then each simulation's ADE, E-source and DFT launches on
+ * A batch of 3-D descriptors (every shape[2] >= 2; pols ignored) runs, per
+ * step, the single-run march kernels over the tiles of all simulations
+ * still running (one psi-free and one CPML launch), one ADE launch and one
+ * E-source launch for all of them, then each simulation's DFT capture on
  * the batch stream: mode 0 only, waves_per_block 10, 12 or 16 (march
- * wavefronts per workgroup), rows 1..128 (x planes per tile). Each 3-D
+ * wavefronts per workgroup), rows 1..128 (x planes per psi-free tile). Each 3-D
  * simulation needs the march kernel's conditions (E-side sources only,
  * CPML slabs) and no flux planes. 2-D and 3-D descriptors do not mix. */
 typedef struct pf_batch pf_batch;

@@ -302,8 +302,9 @@ class Stepper final : public pf_handle {
   hipStream_t stream() const { return stream_; }
   int64_t ncell() const { return shape[0] * shape[1] * shape[2]; }
   void timing_read(double* ms, int64_t* launches) override;
-  // batched 3-D stepping support: the batch launches the march kernel for
-  // all its simulations, then each simulation runs the rest of its step.
+  // batched 3-D stepping support: the batch launches the march kernels for
+  // all its simulations (tiles from march_tile_list, arguments from
+  // march_args) on the stream it lends (adopt_stream).
   void check_march_batch() const;
   std::vector<pfhip::MarchTile> march_tile_list(int nw, int xchunk, bool cpml) const;
   const pfhip::MarchArgs<real>& march_args(int p) const { return ma_[p]; }
@@ -860,8 +861,8 @@ void Stepper<real>::build_march_tiles(int xchunk) {
     const int b0 = std::max(slab_e_[a][0], slab_h_[a][0] + 1);
     const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]);
     if (a == 2 && zpad_) {
-      // padded rows: z tiles on the fixed 62-node grid (each owns one
-      // 64-slot block); a tile is psi-free when it lies inside [b0, b1)
+      // padded rows: z tiles on the fixed grid of zpad_ nodes (each owns
+      // one 64-slot block); a tile is psi-free when it lies inside [b0, b1)
       for (int z = 0; z < n[2]; z += zpad_) {
         const int c = std::min(zpad_, n[2] - z);
         seg[2].push_back({z, c, z >= b0 && z + c <= b1});
@@ -2146,9 +2147,9 @@ class Batch3 final : public BatchBase {
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
 
-  // mode: 0 only (one march launch per step); waves_per_block: wavefronts
-  // per march workgroup (10, 12 or 16); rows: x planes per tile (1 ..
-  // MARCH_MAX_XCHUNK).
+  // mode: 0 only (launches per step, no graph or persistent mode);
+  // waves_per_block: wavefronts per march workgroup (10, 12 or 16); rows: x
+  // planes per psi-free tile (1 .. MARCH_MAX_XCHUNK).
   void configure(int mode, int waves_per_block, int rows) override {
     if (mode != 0) throw ArgError{"3-D batches run in stream mode (0)"};
     if ((waves_per_block != 10 && waves_per_block != 12 && waves_per_block != 16) || rows < 1 ||
