@@ -375,6 +375,100 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def batched_3d_scenes(pf, n_sims=16, lo=64, hi=128, steps=220, precision="float32", seed=16):
+    """Independent 3-D scenes of uneven size for batched stepping.
+
+    Node counts per axis uniform in [lo, hi], dx = 40 nm, CPML of 5 / 4 / 6
+    cells on x / y / z, Courant number 0.9. Each scene holds a dielectric box
+    (eps uniform in [2, 6]), a Drude box and a two-pole Lorentz box placed by
+    the seed, an Ez and an Ex dipole (Gaussian pulse at 1.2 um, 4 fs FWHM),
+    and DFT planes on x (Ex, Ez, Hy at 2 frequencies) and y (Hz, Ey at 1).
+    """
+    rng = np.random.default_rng(seed)
+    dx = 40e-9
+    out = []
+    for _ in range(n_sims):
+        shape = rng.integers(lo, hi + 1, 3)
+        L = shape * dx
+        grid = pf.Grid(size=tuple(L.tolist()), cell_size=dx, pml_layers=(5, 4, 6))
+
+        def box(frac, medium):
+            c = (rng.uniform(-0.15, 0.15, 3) * L).tolist()
+            return pf.Box(center=tuple(c), size=tuple((np.asarray(frac) * L).tolist()),
+                          medium=medium)
+
+        st = [box((0.5, 0.4, 0.5), pf.Medium(float(rng.uniform(2.0, 6.0)))),
+              box((0.2, 0.2, 0.15),
+                  pf.DispersiveMedium.drude(1.0, f_plasma_Hz=80e12, f_collision_Hz=5e12)),
+              box((0.15, 0.2, 0.2),
+                  pf.DispersiveMedium.lorentz(2.0, [(1.5, 150e12, 5e12), (0.5, 300e12, 10e12)]))]
+        wave = pf.GaussianPulse(freq0=pf.C_0 / 1.2e-6, fwhm=4e-15)
+        p = (rng.uniform(-0.2, 0.2, 3) * L).tolist()
+        src = [pf.PointDipole(position=tuple(p), component="Ez", waveform=wave),
+               pf.PointDipole(position=(-p[0], p[1], -p[2]), component="Ex", waveform=wave,
+                              amplitude=0.5)]
+        mon = [pf.DFTMonitor(name="d", components=("Ex", "Ez", "Hy"),
+                             freqs=[pf.C_0 / 1.2e-6, pf.C_0 / 1.0e-6], plane_axis="x",
+                             plane_position=0.1 * L[0]),
+               pf.DFTMonitor(name="y", components=("Hz", "Ey"), freqs=[pf.C_0 / 1.1e-6],
+                             plane_axis="y", plane_position=-0.1 * L[1])]
+        dt = pf.Simulation(grid, run_time=1e-15, precision=precision, backend="hip",
+                           courant=0.9).dt
+        out.append(pf.Simulation(grid, structures=st, sources=src, monitors=mon,
+                                 run_time=(steps - 0.5) * dt, precision=precision, backend="hip",
+                                 courant=0.9))   # exactly `steps` steps
+    return out
+
+
+def bench_batched3d(args):
+    """Batched 3-D workload: 16 uneven scenes of 64-128 nodes per axis in one
+    batch, against the same scenes stepped one after another."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.batch import SimBatch
+    from photonfdtd_amd.hip_backend import HipEngine, device_info, exact_mode
+
+    total = args.warmup + args.steps
+    sims = batched_3d_scenes(pf, steps=total, precision=args.precision)
+    cells = sum(int(np.prod(s.grid.shape)) for s in sims)
+    npdt = np.float32 if args.precision == "float32" else np.float64
+    # the same scenes, one stepper each, run one after another
+    single = 0.0
+    for s in sims:
+        eng = HipEngine(s, npdt, exact_mode(npdt))
+        eng.advance(0, args.warmup)
+        t0 = time.perf_counter()
+        eng.advance(args.warmup, args.steps)
+        single += time.perf_counter() - t0
+        eng.close()
+    b = SimBatch(sims)
+    tuned = b.tune(trial_steps=min(50, total))
+    trials = tuned.pop("trials")
+    b.advance(args.warmup)
+    t0 = time.perf_counter()
+    b.advance(args.steps)
+    wall = time.perf_counter() - t0
+    dev_ms = b.last_ms()
+    finite = all(np.isfinite(v).all() for i in range(len(sims)) for v in b.dft(i, "d").values())
+    b.close()
+    out = {
+        "metric": METRIC, "value": round(cells * args.steps / wall / 1e9, 3),
+        "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if npdt == np.float32 else "f64", "data": "synthetic",
+        "config": {"workload": f"batched 3-D: {len(sims)} scenes, 64-128 nodes per axis, dx 40 nm, "
+                               "CPML 5/4/6, dielectric + Drude + Lorentz boxes, Ez + Ex dipoles, "
+                               "DFT planes on x and y",
+                   "cells": cells, "launch": tuned, "tune_trials": trials,
+                   "device": device_info()[0]},
+        "batch_device_ms_per_step": round(dev_ms / args.steps, 4),
+        "single_runs_ms_per_step": round(single / args.steps * 1e3, 4),
+        "speedup_vs_single_runs": round(single / wall, 3),
+        "dft_finite": bool(finite),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def adjoint3d_scene(pf, precision="float32", steps=3000, rho=None):
     """The 3-D adjoint inverse-design scene: 256 x 256 x 128 cells, dx 20 nm,
     12-cell CPML, silica background, a 500 x 220 nm Si core along x, a
@@ -620,9 +714,10 @@ def main():
     ap.add_argument("--rounds", type=int, default=6, help="surrogate design rounds")
     ap.add_argument("--fd-voxels", type=int, default=20, help="adjoint3d: central-difference voxels")
     ap.add_argument("--workload", default="slab3d",
-                    choices=("slab3d", "batched2d", "surrogate", "adjoint3d"),
+                    choices=("slab3d", "batched2d", "batched3d", "surrogate", "adjoint3d"),
                     help="slab3d: the 3-D dispersive slab (headline); batched2d: 64 uneven "
-                         "2-D TE scenes in one launch")
+                         "2-D TE scenes in one launch; batched3d: 16 uneven 3-D scenes in one "
+                         "batch")
     ap.add_argument("--batch-config", default=None, metavar="MODE,WAVES,ROWS",
                     help="fixed batched launch setting instead of the autotuner")
     ap.add_argument("--batch-modes", default="graph,stream,persistent",
@@ -673,6 +768,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched2d":
         bench_batched(args)
+        return
+    if args.workload == "batched3d":
+        bench_batched3d(args)
         return
     if args.workload == "surrogate":
         bench_surrogate(args)
