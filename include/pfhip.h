@@ -185,9 +185,8 @@ int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* gra
  * launching.
  * A batch of 3-D descriptors (every shape[2] >= 2; pols ignored) runs, per
  * step, the single-run march kernels over the tiles of all simulations
- * still running (one psi-free and one CPML launch), one ADE launch and one
- * E-source launch for all of them, then each simulation's DFT capture on
- * the batch stream: mode 0 only, waves_per_block 10, 12 or 16 (march
+ * still running (one psi-free and one CPML launch), then one ADE, one
+ * E-source and one DFT-capture launch for all of them: mode 0 only, waves_per_block 10, 12 or 16 (march
  * wavefronts per workgroup), rows 1..128 (x planes per psi-free tile). Each 3-D
  * simulation needs the march kernel's conditions (E-side sources only,
  * CPML slabs) and no flux planes. 2-D and 3-D descriptors do not mix. */

@@ -19,9 +19,8 @@ FluxMonitor need per-simulation host stops and are rejected.
 
 A batch of 3-D grids (every nz >= 2) runs the single-run march kernels each
 step over the tiles of all its simulations (each tile names its
-simulation), then one ADE launch and one source launch for all of them and
-each simulation's DFT capture on the batch stream; only the per-step stream
-mode exists for 3-D. The same
+simulation), then one ADE, one source and one DFT-capture launch for all of
+them; only the per-step stream mode exists for 3-D. The same
 bit-for-bit parity with single runs holds. 3-D batches need the march
 kernel's conditions: E-side sources only and CPML on slabs.
 """

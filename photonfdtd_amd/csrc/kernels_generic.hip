@@ -152,6 +152,38 @@ void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64
   dim3 grid((unsigned)((max_groups + 63) / 64), (unsigned)n_jobs);
   k_src_jobs<real><<<grid, 64, 0, s>>>(jobs, step);
 }
+// Batched DFT capture: job blockIdx.y, element blockIdx.x * blockDim.x +
+// lane of [comp][box node], the element order of k_gather_box.
+template <typename real>
+__global__ void k_dft_jobs(const DftJob<real>* __restrict__ jobs, int64_t step) {
+  const DftJob<real>& J = jobs[blockIdx.y];
+  if (step % J.interval != 0) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= J.nbox * J.n_comp) return;
+  const int64_t q = step / J.interval;
+  const int64_t slot = (J.count0 + (q - J.q0)) % J.K;
+  const BoxDev& box = J.box;
+  const int c = (int)(t / J.nbox);
+  const int64_t r = t - (int64_t)c * J.nbox;
+  const int64_t bk = r % box.count[2];
+  const int64_t bj = (r / box.count[2]) % box.count[1];
+  const int64_t bi = r / (box.count[2] * box.count[1]);
+  const int64_t node = (box.start[0] + bi * box.stride[0]) * J.plane +
+                       (box.start[1] + bj * box.stride[1]) * J.row +
+                       zslot(box.start[2] + bk * box.stride[2], J.zpad);
+  J.ring[slot * J.n_comp * J.nbox + t] = J.fields[J.comps[c]][node];
+}
+
+template <typename real>
+void launch_dft_jobs(const DftJob<real>* jobs, int n_jobs, int64_t max_elems, int64_t step,
+                     hipStream_t s) {
+  if (n_jobs <= 0 || max_elems <= 0) return;
+  dim3 grid((unsigned)((max_elems + 255) / 256), (unsigned)n_jobs);
+  k_dft_jobs<real><<<grid, 256, 0, s>>>(jobs, step);
+}
+template void launch_dft_jobs<float>(const DftJob<float>*, int, int64_t, int64_t, hipStream_t);
+template void launch_dft_jobs<double>(const DftJob<double>*, int, int64_t, int64_t, hipStream_t);
+
 template void launch_src_jobs<float>(const SrcJob<float>*, int, int, int64_t, hipStream_t);
 template void launch_src_jobs<double>(const SrcJob<double>*, int, int, int64_t, hipStream_t);
 

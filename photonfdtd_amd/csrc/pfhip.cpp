@@ -318,6 +318,12 @@ class Stepper final : public pf_handle {
   void ade_jobs(int p, std::vector<pfhip::AdeJob<real>>& out) const;
   bool src_job(int p, pfhip::SrcJob<real>& out) const;
   void finish_step(int64_t step);
+  // Batched DFT capture: this simulation's monitor jobs for samples written
+  // from level p from step0 on; finish_step_dft then flips the level and
+  // advances the rings' fill (folding a full ring) for a step whose samples
+  // the batched launch wrote.
+  void dft_jobs(int p, int64_t step0, std::vector<pfhip::DftJob<real>>& out) const;
+  void finish_step_dft(int64_t step);
 
  private:
   struct Ade {
@@ -1361,6 +1367,41 @@ void Stepper<real>::finish_step(int64_t step) {
   timed(PF_K_AUX, [&] { capture_dft(step); });
 }
 
+template <typename real>
+void Stepper<real>::dft_jobs(int p, int64_t step0, std::vector<pfhip::DftJob<real>>& out) const {
+  for (const Dft& m : dft_) {
+    pfhip::DftJob<real> J;
+    J.fields = fields_dev_[p];
+    J.comps = m.comps_dev;
+    J.n_comp = m.n_comp;
+    J.box = m.box;
+    J.plane = (int64_t)args_.ny * zrow_;
+    J.row = zrow_;
+    J.zpad = zpad_;
+    J.ring = m.ring;
+    J.nbox = m.nbox;
+    J.K = m.K;
+    J.interval = m.interval;
+    J.q0 = (step0 + m.interval - 1) / m.interval;
+    J.count0 = m.count;
+    out.push_back(J);
+  }
+}
+
+// The ring bookkeeping of capture_dft for a sample the batched launch wrote.
+template <typename real>
+void Stepper<real>::finish_step_dft(int64_t step) {
+  cur_ ^= 1;
+  for (Dft& m : dft_) {
+    if (step % m.interval != 0) continue;
+    const int64_t sample = step / m.interval;
+    if (sample >= m.n_samples) throw ArgError{"DFT sample beyond the phase table"};
+    if (m.count == 0) m.sample0 = sample;
+    else if (sample != m.sample0 + m.count) throw ArgError{"DFT samples must be consecutive"};
+    if (++m.count == m.K) fold(m);
+  }
+}
+
 // Run on the stream of a batch from now on (the batch owns and destroys it).
 template <typename real>
 void Stepper<real>::adopt_stream(hipStream_t s) {
@@ -2063,9 +2104,10 @@ class Batch final : public BatchBase {
 // Batch of 3-D simulations: per step, one march launch covers the psi-free
 // tiles and one the CPML tiles of every simulation still running (the tile's
 // pad0 field names its simulation, whose march arguments come from a device
-// table per time level); then one launch runs the ADE media and one the
-// E-side sources of all of them (job tables per time level), and each
-// simulation captures its DFT samples on the batch stream. Simulations are
+// table per time level); then one launch each runs the ADE media, the
+// E-side sources and the DFT captures of all of them (job tables; the DFT
+// jobs carry each ring's fill as the advance starts, and the host keeps
+// the rings' bookkeeping and folds). Simulations are
 // ordered by decreasing step count so the running ones own a prefix of the
 // tile and job lists; each simulation's tiles are in XCD order, padded to a
 // multiple of 8.
@@ -2119,6 +2161,20 @@ class Batch3 final : public BatchBase {
       for (const auto& J : ade) max_ade_ = std::max(max_ade_, J.n);
       ade_tab_[p] = ade.empty() ? nullptr : arena_.upload(ade.data(), ade.size(), stream_);
       src_tab_[p] = src.empty() ? nullptr : arena_.upload(src.data(), src.size(), stream_);
+    }
+    // DFT jobs: their ring state is refreshed at every advance
+    {
+      std::vector<pfhip::DftJob<real>> dft;
+      dft_prefix_.assign(1, 0);
+      for (int i : order_) {
+        sims_[i]->dft_jobs(1, 0, dft);
+        dft_prefix_.push_back((int)dft.size());
+      }
+      for (const auto& J : dft) max_dft_ = std::max<int64_t>(max_dft_, J.nbox * J.n_comp);
+      for (int p = 0; p < 2; ++p)
+        dft_tab_[p] = dft.empty() ? nullptr
+                                  : static_cast<pfhip::DftJob<real>*>(
+                                        arena_.alloc(dft.size() * sizeof(pfhip::DftJob<real>)));
     }
     HIP_CHECK(hipEventCreate(&ev_[0]));
     HIP_CHECK(hipEventCreate(&ev_[1]));
@@ -2187,6 +2243,16 @@ class Batch3 final : public BatchBase {
 
   void advance(int64_t n) override {
     if (n < 0 || step_ + n > n_steps_max_) throw ArgError{"step range exceeds the batch"};
+    if (dft_tab_[0]) {
+      // the rings' fill as this advance starts, for both sampled levels
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      for (int p = 0; p < 2; ++p) {
+        std::vector<pfhip::DftJob<real>> dft;
+        for (int i : order_) sims_[i]->dft_jobs(p ^ 1, step_, dft);
+        HIP_CHECK(hipMemcpy(dft_tab_[p], dft.data(), dft.size() * sizeof(pfhip::DftJob<real>),
+                            hipMemcpyHostToDevice));
+      }
+    }
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
     for (int64_t s = step_; s < step_ + n; ++s) {
       size_t running = 0;   // simulations with steps left: a prefix of order_
@@ -2204,10 +2270,11 @@ class Batch3 final : public BatchBase {
       // each simulation's level flip and DFT capture (the single run's order)
       pfhip::launch_ade_jobs<real>(ade_tab_[p], ade_prefix_[p][running], max_ade_, stream_);
       pfhip::launch_src_jobs<real>(src_tab_[p], src_prefix_[p][running], max_groups_, s, stream_);
+      pfhip::launch_dft_jobs<real>(dft_tab_[p], dft_prefix_[running], max_dft_, s, stream_);
       for (size_t r = 0; r < running; ++r) {
         Stepper<real>& sim = *sims_[order_[r]];
         if (sim.level() != p) throw HipError{PF_E_STATE, "batch simulation out of step"};
-        sim.finish_step(s);
+        sim.finish_step_dft(s);
       }
     }
     HIP_CHECK(hipGetLastError());
@@ -2229,6 +2296,9 @@ class Batch3 final : public BatchBase {
   pfhip::MarchArgs<real>* table_[2] = {nullptr, nullptr};   // per-simulation args reading level p
   pfhip::AdeJob<real>* ade_tab_[2] = {nullptr, nullptr};    // ADE jobs reading level p, in order_
   pfhip::SrcJob<real>* src_tab_[2] = {nullptr, nullptr};    // source jobs writing level p ^ 1
+  pfhip::DftJob<real>* dft_tab_[2] = {nullptr, nullptr};    // DFT jobs sampling level p ^ 1
+  std::vector<int> dft_prefix_;                             // jobs of the first r simulations
+  int64_t max_dft_ = 0;
   std::vector<int> ade_prefix_[2], src_prefix_[2];          // jobs of the first r simulations
   int64_t max_ade_ = 0;
   int max_groups_ = 0;

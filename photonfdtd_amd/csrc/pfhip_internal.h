@@ -282,8 +282,29 @@ struct SrcJob {
   SourceGroupsDev g;
   int64_t n_steps;
 };
+// One DFT monitor of one simulation: the step's sample goes to ring slot
+// (count0 + q - q0) mod K for sample q = step / interval (steps off the
+// interval take none); q0 / count0: the first sample of the advance and the
+// ring's fill when it started.
+template <typename real>
+struct DftJob {
+  real* const* fields;    // device table of the sampled level's six components
+  const int32_t* comps;
+  int n_comp;
+  BoxDev box;
+  int64_t plane;
+  int row, zpad;
+  real* ring;
+  int64_t nbox;
+  int K, interval;
+  int64_t q0;
+  int count0;
+};
 template <typename real>
 void launch_ade_jobs(const AdeJob<real>* jobs, int n_jobs, int64_t max_n, hipStream_t s);
+template <typename real>
+void launch_dft_jobs(const DftJob<real>* jobs, int n_jobs, int64_t max_elems, int64_t step,
+                     hipStream_t s);
 template <typename real>
 void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64_t step, hipStream_t s);
 
