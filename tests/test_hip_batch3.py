@@ -1,8 +1,9 @@
 """GPU: batched 3-D stepping reproduces single runs bit for bit.
 
-One march launch per step covers the tiles of every 3-D simulation in the
-batch; each simulation's ADE, source and DFT launches follow on the batch
-stream. Each batched simulation must equal its own ``backend="hip"`` run
+Per step, the march launches (psi-free and CPML tiles) cover the tiles of
+every 3-D simulation in the batch, then one launch each runs the ADE media,
+the E-side sources and the DFT captures of all of them. Each batched
+simulation must equal its own ``backend="hip"`` run
 (fields and DFT planes) for uneven grid shapes, uneven step counts, Drude
 and Lorentz boxes, several march shapes (wavefronts per workgroup, planes
 per tile) and both operation orders; float64 must also match the NumPy
