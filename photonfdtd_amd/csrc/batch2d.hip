@@ -27,6 +27,14 @@
 
 namespace pfhip {
 
+// Field rows are stored with the streaming hint: a row is read again only in
+// the next step, and the lines it would push out of the caches hold the
+// apron columns that the neighbouring strips read.
+template <typename real>
+__device__ __forceinline__ void st_row(real* p, real v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 namespace {
 
 // CPML membership of an axis: node n is in the slab set iff n < lo or
@@ -169,7 +177,7 @@ __device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk
                        cyh, &nhy);
     }
     if (HSRC && col) hz = add_h_sources(s, 5, (int64_t)i * ny + j, step, hz);
-    if (live) Hzo[i * nyr + ss] = hz;   // every lane: the row store covers whole lines
+    if (live) st_row(&Hzo[i * nyr + ss], hz);   // every lane: the row store covers whole lines
     if (own && live) {
       if (hupd && pxh) hout_x[xh.m(i) * ny + j] = nhx;
       if (hupd && pyh) hout_y[i * ph1 + myh] = nhy;
@@ -196,8 +204,8 @@ __device__ __forceinline__ void march_te(const SimDev<real>& s, const MarchChunk
       if (i >= 1 && pxe) pe_x[xe.m(i) * ny + j] = ney;
     }
     if (live) {
-      Exo[i * nyr + ss] = ex;
-      Eyo[i * nyr + ss] = ey;
+      st_row(&Exo[i * nyr + ss], ex);
+      st_row(&Eyo[i * nyr + ss], ey);
     }
     hz_prev = hz;
   };
@@ -297,8 +305,8 @@ __device__ __forceinline__ void march_tm(const SimDev<real>& s, const MarchChunk
       hy = add_h_sources(s, 4, cell, step, hy);
     }
     if (live) {   // every lane: the row store covers whole lines
-      Hxo[i * nyr + ss] = hx;
-      Hyo[i * nyr + ss] = hy;
+      st_row(&Hxo[i * nyr + ss], hx);
+      st_row(&Hyo[i * nyr + ss], hy);
     }
     if (own && live) {
       if (xupd && pyh) hout_y[i * ph1 + myh] = nhy;
@@ -321,7 +329,7 @@ __device__ __forceinline__ void march_tm(const SimDev<real>& s, const MarchChunk
       if (eupd && pxe) pe_x[xe.m(i) * ny + j] = nzx;
       if (eupd && pye) pe_y[i * pe1 + mye] = nzy;
     }
-    if (live) Ezo[i * nyr + ss] = ez;
+    if (live) st_row(&Ezo[i * nyr + ss], ez);
     hy_prev = hy;
   };
   RowTM<real> A, B;

@@ -629,11 +629,14 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
         if (pyh && xh_in && yh_in) st(a.psiHout + hoff[5] + i * ys_h, cyh_o, nps[5]);
       }
     }
+    // Field rows are stored with the streaming hint: the next read of a row
+    // is the next step's, long after it has left the caches, and the lines
+    // it would push out hold the y-apron rows the neighbouring tiles read.
     if (st_lane && live) {
       const size_t o = (size_t)i * fplane;
-      st(Ho0 + o, cols, hx);
-      st(Ho1 + o, cols, hy);
-      st(Ho2 + o, cols, hz);
+      st_nt(Ho0 + o, cols, hx);
+      st_nt(Ho1 + o, cols, hy);
+      st_nt(Ho2 + o, cols, hz);
     }
     sHx[bi][tid] = hx;
     sHz[bi][tid] = hz;
@@ -671,9 +674,9 @@ __global__ __launch_bounds__(NW * 64) void k_yee_march(MarchArgs<real> arg) {
                          Q.pe[5], bye, cye, &npe[5]);
       }
       const size_t o = (size_t)i * fplane;
-      st(Eo0 + o, cols, ex);
-      st(Eo1 + o, cols, ey);
-      st(Eo2 + o, cols, ez);
+      st_nt(Eo0 + o, cols, ex);
+      st_nt(Eo1 + o, cols, ey);
+      st_nt(Eo2 + o, cols, ez);
       if (pye && ye_in && ze_in) st(pExY + i * ys_e, cye_o, npe[0]);
       if (pze && ye_in && ze_in) st(pExZ + i * zs_e, cze_o, npe[1]);
       if (pze && ze_in && xe_in) st(pEyZ + i * zs_e, cze_o, npe[2]);

@@ -116,6 +116,12 @@ template <typename real>
 __device__ __forceinline__ void st(real* base, uint32_t boff, real v) {
   *reinterpret_cast<real*>(reinterpret_cast<char*>(base) + boff) = v;
 }
+// Streaming store (nontemporal hint): for data not read again before it
+// leaves the cache anyway, so it does not push out lines that are.
+template <typename real>
+__device__ __forceinline__ void st_nt(real* base, uint32_t boff, real v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<real*>(reinterpret_cast<char*>(base) + boff));
+}
 
 // Pointer in the global address space: accesses through it compile to
 // global_* instructions. Generic (flat) accesses count against both the
