@@ -50,6 +50,17 @@ struct ArgError {
   std::string msg;
 };
 
+// Host <-> device copy ordered on stream s (after the work already queued
+// there, such as the clearing of a fresh allocation) that returns once it is
+// complete, so the host buffer may be a temporary. The handles' streams are
+// non-blocking, so a plain hipMemcpy on the null stream is not ordered
+// against them.
+void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  if (bytes == 0) return;
+  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
 // Device allocations of one handle, freed together.
 class DeviceArena {
  public:
@@ -65,9 +76,7 @@ class DeviceArena {
   T* upload(const T* host, size_t n, hipStream_t s) {
     if (n == 0) return nullptr;
     T* d = static_cast<T*>(alloc(n * sizeof(T)));
-    (void)s;
-    // synchronous: host tables may be temporaries of the caller
-    HIP_CHECK(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
+    copy_sync(d, host, n * sizeof(T), hipMemcpyHostToDevice, s);
     return d;
   }
   template <typename T>
@@ -296,7 +305,7 @@ class Stepper final : public pf_handle {
           zpad_ ? pad_rows(static_cast<const real*>(src), (int)shape[0], (int)shape[1], (int)shape[2],
                            zrow_, zpad_)
                 : pad_rows(static_cast<const real*>(src), (int)shape[0], 1, (int)shape[1], yrow_, ypad_);
-      HIP_CHECK(hipMemcpy(ce_f_, p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
+      copy_sync(ce_f_, p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice, s);
     }
   }
   hipStream_t stream() const { return stream_; }
@@ -518,10 +527,10 @@ Stepper<real>::Stepper(const pf_desc& d) {
       if (d.c_h[a][i] != 0.0) mh[i] = ph++;
     }
     real* tbl = coef_ + coef_off;
-    HIP_CHECK(hipMemcpy(tbl, be.data(), n * sizeof(real), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(tbl + n, ce.data(), n * sizeof(real), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(tbl + 2 * n, bh.data(), n * sizeof(real), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(tbl + 3 * n, ch.data(), n * sizeof(real), hipMemcpyHostToDevice));
+    copy_sync(tbl, be.data(), n * sizeof(real), hipMemcpyHostToDevice, stream_);
+    copy_sync(tbl + n, ce.data(), n * sizeof(real), hipMemcpyHostToDevice, stream_);
+    copy_sync(tbl + 2 * n, bh.data(), n * sizeof(real), hipMemcpyHostToDevice, stream_);
+    copy_sync(tbl + 3 * n, ch.data(), n * sizeof(real), hipMemcpyHostToDevice, stream_);
     args_.b_e[a] = tbl;
     args_.c_e[a] = tbl + n;
     args_.b_h[a] = tbl + 2 * n;
@@ -1462,7 +1471,7 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
     }
     for (int c = 0; c < 6; ++c) args_.f[c] = nullptr;
     std::vector<real> ce((size_t)ncell());
-    HIP_CHECK(hipMemcpy(ce.data(), args_.ce, ce.size() * sizeof(real), hipMemcpyDeviceToHost));
+    copy_sync(ce.data(), args_.ce, ce.size() * sizeof(real), hipMemcpyDeviceToHost, stream_);
     const std::vector<real> cep = pad_rows(ce.data(), (int)shape[0], 1, ny, yrow_, ypad_);
     ce_f_ = arena_.upload(cep.data(), cep.size(), stream_);
   }
@@ -1510,8 +1519,8 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
     if (g.n_groups == 0) continue;
     std::vector<int64_t> cells(g.n_groups);
     std::vector<int32_t> comps(g.n_groups);
-    HIP_CHECK(hipMemcpy(cells.data(), g.cell, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(comps.data(), g.comp, comps.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    copy_sync(cells.data(), g.cell, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream_);
+    copy_sync(comps.data(), g.comp, comps.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
     for (int q = 0; q < g.n_groups; ++q) {
       const bool active = pol == 0 ? (comps[q] == 0 || comps[q] == 1 || comps[q] == 5)
                                    : (comps[q] == 2 || comps[q] == 3 || comps[q] == 4);
@@ -1526,7 +1535,7 @@ void Stepper<real>::describe(pfhip::SimDev<real>& s, int pol, PostNodes& post) {
   for (size_t m = 0; m < ade_.size(); ++m) {
     if (ade_[m].n >= (int64_t(1) << 26)) throw ArgError{"dispersive region too large for a batch"};
     std::vector<int64_t> cells(ade_[m].n);
-    HIP_CHECK(hipMemcpy(cells.data(), ade_[m].cells, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    copy_sync(cells.data(), ade_[m].cells, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream_);
     for (int64_t t = 0; t < ade_[m].n; ++t)
       post.ade.push_back({slot(cells[t]), (int)(m << 26) | (int)t});
     pfhip::AdeDev<real>& A = s.ade[m];
@@ -1629,7 +1638,8 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_hos
       throw ArgError{"bad cotangent"};
     const Dft& m = dft_[g.mon];
     const size_t n = (size_t)m.n_freq * m.nbox * 2;
-    HIP_CHECK(hipMemcpy(cot_dev[g.mon] + (size_t)g.slot * n, g.g, n * sizeof(double), hipMemcpyHostToDevice));
+    copy_sync(cot_dev[g.mon] + (size_t)g.slot * n, g.g, n * sizeof(double), hipMemcpyHostToDevice,
+              stream_);
   }
   real* lam = scratch.zeros<real>(elems, stream_);
   real* lam_d = scratch.zeros<real>(6 * (size_t)ncell, stream_);
@@ -1799,7 +1809,7 @@ void Stepper<real>::write_field(int comp, const void* src) {
         zpad_ ? pad_rows(static_cast<const real*>(src), (int)shape[0], (int)shape[1], (int)shape[2],
                          zrow_, zpad_)
               : pad_rows(static_cast<const real*>(src), (int)shape[0], 1, (int)shape[1], yrow_, ypad_);
-    HIP_CHECK(hipMemcpy(f_[cur_][comp], p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice));
+    copy_sync(f_[cur_][comp], p.data(), p.size() * sizeof(real), hipMemcpyHostToDevice, stream_);
     return;
   }
   HIP_CHECK(hipMemcpyAsync(f_[cur_][comp], src, ncell * sizeof(real), hipMemcpyHostToDevice, stream_));
@@ -2033,7 +2043,7 @@ class Batch final : public BatchBase {
     last_ms_ = t;
     if (mode_ == 1) {
       int failed = 0;
-      HIP_CHECK(hipMemcpy(&failed, fail_, sizeof(int), hipMemcpyDeviceToHost));
+      copy_sync(&failed, fail_, sizeof(int), hipMemcpyDeviceToHost, stream_);
       if (failed)
         throw HipError{PF_E_STATE, "persistent launch: a grid barrier wait expired (workgroups "
                                    "were not all resident); the batch state is undefined, reset it"};
@@ -2248,8 +2258,8 @@ class Batch3 final : public BatchBase {
       for (int p = 0; p < 2; ++p) {
         std::vector<pfhip::DftJob<real>> dft;
         for (int i : order_) sims_[i]->dft_jobs(p ^ 1, step_, dft);
-        HIP_CHECK(hipMemcpy(dft_tab_[p], dft.data(), dft.size() * sizeof(pfhip::DftJob<real>),
-                            hipMemcpyHostToDevice));
+        copy_sync(dft_tab_[p], dft.data(), dft.size() * sizeof(pfhip::DftJob<real>),
+                  hipMemcpyHostToDevice, stream_);
       }
     }
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
