@@ -13,16 +13,16 @@
 
 namespace pfhip {
 
+// PML state of one derivative term; m is the node's compact psi index along
+// axis `ax` (negative outside the slab).
 template <typename real>
 __device__ __forceinline__ void pml_term(Term<real>& t, const StepArgs<real>& a, bool e_side,
-                                         int ax, int psi_slot, int i, int j, int k) {
-  const int node = ax == 0 ? i : (ax == 1 ? j : k);
-  const int* map = e_side ? a.map_e[ax] : a.map_h[ax];
-  const int m = map[node];
+                                         int ax, int psi_slot, int i, int j, int k, int m) {
   if (m < 0) {
     t.psi = nullptr;
     return;
   }
+  const int node = ax == 0 ? i : (ax == 1 ? j : k);
   const int pc = e_side ? a.pe[ax] : a.ph[ax];
   t.psi = a.psi[psi_slot];
   t.pid = psi_index(ax, m, i, j, k, a.ny, a.nz, pc);
@@ -30,45 +30,100 @@ __device__ __forceinline__ void pml_term(Term<real>& t, const StepArgs<real>& a,
   t.c = e_side ? a.c_e[ax][node] : a.c_h[ax][node];
 }
 
+// F + K (dA - dB), then F += K psiA', F -= K psiB' on PML nodes, where
+// psi' = b psi + c d: the reference's bulk-then-slab order
+// (simulation.py:1263-1309).
+template <typename real>
+__device__ __forceinline__ real apply_terms_v(real f, real K, const Term<real>& A,
+                                              const Term<real>& B) {
+  real cb;
+  if (A.on) {
+    cb = A.d;
+    if (B.on) cb = cb - B.d;
+  } else {
+    cb = -B.d;
+  }
+  f = f + cb * K;
+  if (A.on && A.psi) {
+    real p = A.b * A.psi[A.pid] + A.c * A.d;
+    A.psi[A.pid] = p;
+    f = f + K * p;
+  }
+  if (B.on && B.psi) {
+    real p = B.b * B.psi[B.pid] + B.c * B.d;
+    B.psi[B.pid] = p;
+    f = f - K * p;
+  }
+  return f;
+}
+
+// (j, k) of flat in-plane index q; 32-bit division when the plane allows it.
+__device__ __forceinline__ void plane_jk(int64_t q, int64_t nyz, int nz, int& j, int& k) {
+  if (nyz <= 0x7FFFFFFF) {
+    const uint32_t u = (uint32_t)q, z = (uint32_t)nz;
+    const uint32_t jj = u / z;
+    j = (int)jj;
+    k = (int)(u - jj * z);
+  } else {
+    j = (int)(q / nz);
+    k = (int)(q - (int64_t)j * nz);
+  }
+}
+
+// Every field operand of a node is loaded up front (neighbour indices clamped
+// to the node itself where that difference is not taken, so each load is in
+// bounds) and the PML maps of the three axes once, so a wavefront keeps all
+// of its loads in flight together; the arithmetic is that of the reference
+// update, term by term.
 template <typename real, bool EXACT>
 __global__ __launch_bounds__(256) void k_update_h(StepArgs<real> a) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= a.nyz) return;
   const int i = blockIdx.y;
-  const int j = (int)(q / a.nz);
-  const int k = (int)(q - (int64_t)j * a.nz);
+  int j, k;
+  plane_jk(q, a.nyz, a.nz, j, k);
   const int64_t id = (int64_t)i * a.nyz + q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sx = a.nyz, sy = nz;
   const real K = -a.ch;
-  real* Ex = a.f[0]; real* Ey = a.f[1]; real* Ez = a.f[2];
+  const real* Ex = a.f[0]; const real* Ey = a.f[1]; const real* Ez = a.f[2];
   const bool xi = nx == 1 || i < nx - 1;
   const bool yj = ny == 1 || j < ny - 1;
   const bool zk = nz == 1 || k < nz - 1;
+  const int64_t ixp = nx > 1 && i < nx - 1 ? id + sx : id;
+  const int64_t iyp = ny > 1 && j < ny - 1 ? id + sy : id;
+  const int64_t izp = nz > 1 && k < nz - 1 ? id + 1 : id;
+  const real ex = Ex[id], ey = Ey[id], ez = Ez[id];
+  const real ez_y = Ez[iyp], ey_z = Ey[izp], ex_z = Ex[izp];
+  const real ez_x = Ez[ixp], ey_x = Ey[ixp], ex_y = Ex[iyp];
+  const real hx = a.f[3][id], hy = a.f[4][id], hz = a.f[5][id];
+  const int mx = nx > 1 ? a.map_h[0][i] : -1;
+  const int my = ny > 1 ? a.map_h[1][j] : -1;
+  const int mz = nz > 1 ? a.map_h[2][k] : -1;
   Term<real> A, B;
   // Hx -= ch (d_y Ez - d_z Ey)
   if (yj && zk) {
     A.on = ny > 1;
     B.on = nz > 1;
-    if (A.on) { A.d = diff<real, EXACT>(Ez[id + sy], Ez[id], a.d[1], a.inv_d[1]); pml_term(A, a, false, 1, PSI_HXY, i, j, k); }
-    if (B.on) { B.d = diff<real, EXACT>(Ey[id + 1], Ey[id], a.d[2], a.inv_d[2]); pml_term(B, a, false, 2, PSI_HXZ, i, j, k); }
-    apply_terms(a.f[3], id, K, A, B);
+    if (A.on) { A.d = diff<real, EXACT>(ez_y, ez, a.d[1], a.inv_d[1]); pml_term(A, a, false, 1, PSI_HXY, i, j, k, my); }
+    if (B.on) { B.d = diff<real, EXACT>(ey_z, ey, a.d[2], a.inv_d[2]); pml_term(B, a, false, 2, PSI_HXZ, i, j, k, mz); }
+    if (A.on || B.on) a.f[3][id] = apply_terms_v(hx, K, A, B);
   }
   // Hy -= ch (d_z Ex - d_x Ez)
   if (zk && xi) {
     A.on = nz > 1;
     B.on = nx > 1;
-    if (A.on) { A.d = diff<real, EXACT>(Ex[id + 1], Ex[id], a.d[2], a.inv_d[2]); pml_term(A, a, false, 2, PSI_HYZ, i, j, k); }
-    if (B.on) { B.d = diff<real, EXACT>(Ez[id + sx], Ez[id], a.d[0], a.inv_d[0]); pml_term(B, a, false, 0, PSI_HYX, i, j, k); }
-    apply_terms(a.f[4], id, K, A, B);
+    if (A.on) { A.d = diff<real, EXACT>(ex_z, ex, a.d[2], a.inv_d[2]); pml_term(A, a, false, 2, PSI_HYZ, i, j, k, mz); }
+    if (B.on) { B.d = diff<real, EXACT>(ez_x, ez, a.d[0], a.inv_d[0]); pml_term(B, a, false, 0, PSI_HYX, i, j, k, mx); }
+    if (A.on || B.on) a.f[4][id] = apply_terms_v(hy, K, A, B);
   }
   // Hz -= ch (d_x Ey - d_y Ex)
   if (xi && yj) {
     A.on = nx > 1;
     B.on = ny > 1;
-    if (A.on) { A.d = diff<real, EXACT>(Ey[id + sx], Ey[id], a.d[0], a.inv_d[0]); pml_term(A, a, false, 0, PSI_HZX, i, j, k); }
-    if (B.on) { B.d = diff<real, EXACT>(Ex[id + sy], Ex[id], a.d[1], a.inv_d[1]); pml_term(B, a, false, 1, PSI_HZY, i, j, k); }
-    apply_terms(a.f[5], id, K, A, B);
+    if (A.on) { A.d = diff<real, EXACT>(ey_x, ey, a.d[0], a.inv_d[0]); pml_term(A, a, false, 0, PSI_HZX, i, j, k, mx); }
+    if (B.on) { B.d = diff<real, EXACT>(ex_y, ex, a.d[1], a.inv_d[1]); pml_term(B, a, false, 1, PSI_HZY, i, j, k, my); }
+    if (A.on || B.on) a.f[5][id] = apply_terms_v(hz, K, A, B);
   }
 }
 
@@ -77,40 +132,50 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= a.nyz) return;
   const int i = blockIdx.y;
-  const int j = (int)(q / a.nz);
-  const int k = (int)(q - (int64_t)j * a.nz);
+  int j, k;
+  plane_jk(q, a.nyz, a.nz, j, k);
   const int64_t id = (int64_t)i * a.nyz + q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sx = a.nyz, sy = nz;
-  const real K = a.ce[id];
-  real* Hx = a.f[3]; real* Hy = a.f[4]; real* Hz = a.f[5];
+  const real* Hx = a.f[3]; const real* Hy = a.f[4]; const real* Hz = a.f[5];
   const bool xi = nx == 1 || i >= 1;
   const bool yj = ny == 1 || j >= 1;
   const bool zk = nz == 1 || k >= 1;
+  const int64_t ixm = nx > 1 && i >= 1 ? id - sx : id;
+  const int64_t iym = ny > 1 && j >= 1 ? id - sy : id;
+  const int64_t izm = nz > 1 && k >= 1 ? id - 1 : id;
+  const real hx = Hx[id], hy = Hy[id], hz = Hz[id];
+  const real hz_y = Hz[iym], hy_z = Hy[izm], hx_z = Hx[izm];
+  const real hz_x = Hz[ixm], hy_x = Hy[ixm], hx_y = Hx[iym];
+  const real ex = a.f[0][id], ey = a.f[1][id], ez = a.f[2][id];
+  const real K = a.ce[id];
+  const int mx = nx > 1 ? a.map_e[0][i] : -1;
+  const int my = ny > 1 ? a.map_e[1][j] : -1;
+  const int mz = nz > 1 ? a.map_e[2][k] : -1;
   Term<real> A, B;
   // Ex += ce (d_y Hz - d_z Hy)
   if (yj && zk) {
     A.on = ny > 1;
     B.on = nz > 1;
-    if (A.on) { A.d = diff<real, EXACT>(Hz[id], Hz[id - sy], a.d[1], a.inv_d[1]); pml_term(A, a, true, 1, PSI_EXY, i, j, k); }
-    if (B.on) { B.d = diff<real, EXACT>(Hy[id], Hy[id - 1], a.d[2], a.inv_d[2]); pml_term(B, a, true, 2, PSI_EXZ, i, j, k); }
-    apply_terms(a.f[0], id, K, A, B);
+    if (A.on) { A.d = diff<real, EXACT>(hz, hz_y, a.d[1], a.inv_d[1]); pml_term(A, a, true, 1, PSI_EXY, i, j, k, my); }
+    if (B.on) { B.d = diff<real, EXACT>(hy, hy_z, a.d[2], a.inv_d[2]); pml_term(B, a, true, 2, PSI_EXZ, i, j, k, mz); }
+    if (A.on || B.on) a.f[0][id] = apply_terms_v(ex, K, A, B);
   }
   // Ey += ce (d_z Hx - d_x Hz)
   if (zk && xi) {
     A.on = nz > 1;
     B.on = nx > 1;
-    if (A.on) { A.d = diff<real, EXACT>(Hx[id], Hx[id - 1], a.d[2], a.inv_d[2]); pml_term(A, a, true, 2, PSI_EYZ, i, j, k); }
-    if (B.on) { B.d = diff<real, EXACT>(Hz[id], Hz[id - sx], a.d[0], a.inv_d[0]); pml_term(B, a, true, 0, PSI_EYX, i, j, k); }
-    apply_terms(a.f[1], id, K, A, B);
+    if (A.on) { A.d = diff<real, EXACT>(hx, hx_z, a.d[2], a.inv_d[2]); pml_term(A, a, true, 2, PSI_EYZ, i, j, k, mz); }
+    if (B.on) { B.d = diff<real, EXACT>(hz, hz_x, a.d[0], a.inv_d[0]); pml_term(B, a, true, 0, PSI_EYX, i, j, k, mx); }
+    if (A.on || B.on) a.f[1][id] = apply_terms_v(ey, K, A, B);
   }
   // Ez += ce (d_x Hy - d_y Hx)
   if (xi && yj) {
     A.on = nx > 1;
     B.on = ny > 1;
-    if (A.on) { A.d = diff<real, EXACT>(Hy[id], Hy[id - sx], a.d[0], a.inv_d[0]); pml_term(A, a, true, 0, PSI_EZX, i, j, k); }
-    if (B.on) { B.d = diff<real, EXACT>(Hx[id], Hx[id - sy], a.d[1], a.inv_d[1]); pml_term(B, a, true, 1, PSI_EZY, i, j, k); }
-    apply_terms(a.f[2], id, K, A, B);
+    if (A.on) { A.d = diff<real, EXACT>(hy, hy_x, a.d[0], a.inv_d[0]); pml_term(A, a, true, 0, PSI_EZX, i, j, k, mx); }
+    if (B.on) { B.d = diff<real, EXACT>(hx, hx_y, a.d[1], a.inv_d[1]); pml_term(B, a, true, 1, PSI_EZY, i, j, k, my); }
+    if (A.on || B.on) a.f[2][id] = apply_terms_v(ez, K, A, B);
   }
 }
 

@@ -30,35 +30,6 @@ struct Term {
   real b, c;
 };
 
-// F += K * (dA - dB), then F += K psiA, F -= K psiB on PML nodes, where
-// psi <- b psi + c d first. Same order as the reference's bulk-then-slab
-// update (simulation.py:1263-1309).
-template <typename real>
-__device__ __forceinline__ void apply_terms(real* __restrict__ F, int64_t id, real K,
-                                            const Term<real>& A, const Term<real>& B) {
-  if (!A.on && !B.on) return;
-  real cb;
-  if (A.on) {
-    cb = A.d;
-    if (B.on) cb = cb - B.d;
-  } else {
-    cb = -B.d;
-  }
-  real f = F[id];
-  f = f + cb * K;
-  if (A.on && A.psi) {
-    real p = A.b * A.psi[A.pid] + A.c * A.d;
-    A.psi[A.pid] = p;
-    f = f + K * p;
-  }
-  if (B.on && B.psi) {
-    real p = B.b * B.psi[B.pid] + B.c * B.d;
-    B.psi[B.pid] = p;
-    f = f - K * p;
-  }
-  F[id] = f;
-}
-
 // One curl component: f + K (dA - dB) with the PML slab terms after the bulk
 // term (same order as the generic kernels).
 template <typename real>
