@@ -27,15 +27,15 @@ __device__ __forceinline__ int64_t adj_psi_index(int ax, int m, int i, int j, in
 
 // One term's transpose. Returns lam_d and adds the forward term (d + psi')
 // times `sign` to `curl`.
+// m is the node's compact psi index along axis ax (negative outside the slab).
 template <typename real>
 __device__ __forceinline__ real term_t(const AdjArgs<real>& a, bool e_side, int ax, int slot, int i,
-                                       int j, int k, real g, real sign, real d, real& curl) {
-  const int node = ax == 0 ? i : (ax == 1 ? j : k);
-  const int m = (e_side ? a.map_e[ax] : a.map_h[ax])[node];
+                                       int j, int k, int m, real g, real sign, real d, real& curl) {
   if (m < 0) {
     curl += sign * d;
     return sign * g;
   }
+  const int node = ax == 0 ? i : (ax == 1 ? j : k);
   const int pc = e_side ? a.pe[ax] : a.ph[ax];
   const int64_t pid = adj_psi_index(ax, m, i, j, k, a.ny, a.nz, pc);
   const real b = (e_side ? a.b_e[ax] : a.b_h[ax])[node];
@@ -49,6 +49,11 @@ __device__ __forceinline__ real term_t(const AdjArgs<real>& a, bool e_side, int 
 
 }  // namespace
 
+// The four per-node kernels below load every operand of a node up front:
+// neighbour indices are clamped to the node itself where a difference is not
+// taken (so each load is in bounds) and the PML maps of the three axes are
+// read once, so a wavefront keeps all of its loads in flight together.
+
 // Transpose of the E update. Forward inputs: H^{n+1/2} (a.f[3..5]) and the
 // E-side psi before the update (a.psi_fwd[0..5]).
 template <typename real>
@@ -56,21 +61,33 @@ __global__ __launch_bounds__(256) void k_adj_e(AdjArgs<real> a) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= a.nyz) return;
   const int i = blockIdx.y;
-  const int j = (int)(q / a.nz), k = (int)(q - (q / a.nz) * a.nz);
+  int j, k;
+  plane_jk(q, a.nyz, a.nz, j, k);
   const int64_t id = (int64_t)i * a.nyz + q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sx = a.nyz, sy = nz;
   const real* Hx = a.f[3]; const real* Hy = a.f[4]; const real* Hz = a.f[5];
-  const real K = a.ce[id];
   const bool xi = nx == 1 || i >= 1, yj = ny == 1 || j >= 1, zk = nz == 1 || k >= 1;
+  const int64_t ixm = nx > 1 && i >= 1 ? id - sx : id;
+  const int64_t iym = ny > 1 && j >= 1 ? id - sy : id;
+  const int64_t izm = nz > 1 && k >= 1 ? id - 1 : id;
+  const real hx = Hx[id], hy = Hy[id], hz = Hz[id];
+  const real hz_y = Hz[iym], hy_z = Hy[izm], hx_z = Hx[izm];
+  const real hz_x = Hz[ixm], hy_x = Hy[ixm], hx_y = Hx[iym];
+  const real l0 = a.lam_f[0][id], l1 = a.lam_f[1][id], l2 = a.lam_f[2][id];
+  const real K = a.ce[id];
+  const double g0 = a.grad[id];
+  const int mx = nx > 1 ? a.map_e[0][i] : -1;
+  const int my = ny > 1 ? a.map_e[1][j] : -1;
+  const int mz = nz > 1 ? a.map_e[2][k] : -1;
   double gsum = 0.0;
   // slots of lam_d: 0 ExY 1 ExZ 2 EyZ 3 EyX 4 EzX 5 EzY
   {  // Ex: +d_y Hz, -d_z Hy
     real dA_l = 0, dB_l = 0, curl = 0;
     if (yj && zk) {
-      const real lam = a.lam_f[0][id], g = K * lam;
-      if (ny > 1) dA_l = term_t(a, true, 1, PSI_EXY, i, j, k, g, real(1), (Hz[id] - Hz[id - sy]) * a.inv_d[1], curl);
-      if (nz > 1) dB_l = term_t(a, true, 2, PSI_EXZ, i, j, k, g, real(-1), (Hy[id] - Hy[id - 1]) * a.inv_d[2], curl);
+      const real lam = l0, g = K * lam;
+      if (ny > 1) dA_l = term_t(a, true, 1, PSI_EXY, i, j, k, my, g, real(1), (hz - hz_y) * a.inv_d[1], curl);
+      if (nz > 1) dB_l = term_t(a, true, 2, PSI_EXZ, i, j, k, mz, g, real(-1), (hy - hy_z) * a.inv_d[2], curl);
       gsum += (double)lam * (double)curl;
     }
     a.lam_d[0][id] = dA_l;
@@ -79,9 +96,9 @@ __global__ __launch_bounds__(256) void k_adj_e(AdjArgs<real> a) {
   {  // Ey: +d_z Hx, -d_x Hz
     real dA_l = 0, dB_l = 0, curl = 0;
     if (zk && xi) {
-      const real lam = a.lam_f[1][id], g = K * lam;
-      if (nz > 1) dA_l = term_t(a, true, 2, PSI_EYZ, i, j, k, g, real(1), (Hx[id] - Hx[id - 1]) * a.inv_d[2], curl);
-      if (nx > 1) dB_l = term_t(a, true, 0, PSI_EYX, i, j, k, g, real(-1), (Hz[id] - Hz[id - sx]) * a.inv_d[0], curl);
+      const real lam = l1, g = K * lam;
+      if (nz > 1) dA_l = term_t(a, true, 2, PSI_EYZ, i, j, k, mz, g, real(1), (hx - hx_z) * a.inv_d[2], curl);
+      if (nx > 1) dB_l = term_t(a, true, 0, PSI_EYX, i, j, k, mx, g, real(-1), (hz - hz_x) * a.inv_d[0], curl);
       gsum += (double)lam * (double)curl;
     }
     a.lam_d[2][id] = dA_l;
@@ -90,15 +107,15 @@ __global__ __launch_bounds__(256) void k_adj_e(AdjArgs<real> a) {
   {  // Ez: +d_x Hy, -d_y Hx
     real dA_l = 0, dB_l = 0, curl = 0;
     if (xi && yj) {
-      const real lam = a.lam_f[2][id], g = K * lam;
-      if (nx > 1) dA_l = term_t(a, true, 0, PSI_EZX, i, j, k, g, real(1), (Hy[id] - Hy[id - sx]) * a.inv_d[0], curl);
-      if (ny > 1) dB_l = term_t(a, true, 1, PSI_EZY, i, j, k, g, real(-1), (Hx[id] - Hx[id - sy]) * a.inv_d[1], curl);
+      const real lam = l2, g = K * lam;
+      if (nx > 1) dA_l = term_t(a, true, 0, PSI_EZX, i, j, k, mx, g, real(1), (hy - hy_x) * a.inv_d[0], curl);
+      if (ny > 1) dB_l = term_t(a, true, 1, PSI_EZY, i, j, k, my, g, real(-1), (hx - hx_y) * a.inv_d[1], curl);
       gsum += (double)lam * (double)curl;
     }
     a.lam_d[4][id] = dA_l;
     a.lam_d[5][id] = dB_l;
   }
-  a.grad[id] += gsum;
+  a.grad[id] = g0 + gsum;
 }
 
 // lam_H += D^T lam_d for the backward differences of the E update:
@@ -108,20 +125,29 @@ __global__ __launch_bounds__(256) void k_adj_e_gather(AdjArgs<real> a) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= a.nyz) return;
   const int i = blockIdx.y;
-  const int j = (int)(q / a.nz), k = (int)(q - (q / a.nz) * a.nz);
+  int j, k;
+  plane_jk(q, a.nyz, a.nz, j, k);
   const int64_t id = (int64_t)i * a.nyz + q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sx = a.nyz, sy = nz;
-  auto dt_ = [&](const real* v, int64_t s, bool has_next, real inv) {
-    return (v[id] - (has_next ? v[id + s] : real(0))) * inv;
+  const bool nxt_x = i + 1 < nx, nxt_y = j + 1 < ny, nxt_z = k + 1 < nz;
+  const int64_t ixp = nxt_x ? id + sx : id, iyp = nxt_y ? id + sy : id, izp = nxt_z ? id + 1 : id;
+  // (v[q] - v[q + s]) / d with v[q + s] = 0 past the last node
+  auto dt_ = [&](real v, real vn, bool has_next, real inv) {
+    return (v - (has_next ? vn : real(0))) * inv;
   };
+  const real *D0 = a.lam_d[0], *D1 = a.lam_d[1], *D2 = a.lam_d[2];
+  const real *D3 = a.lam_d[3], *D4 = a.lam_d[4], *D5 = a.lam_d[5];
+  const real d2 = D2[id], d2n = D2[izp], d5 = D5[id], d5n = D5[iyp];
+  const real d1 = D1[id], d1n = D1[izp], d4 = D4[id], d4n = D4[ixp];
+  const real d0 = D0[id], d0n = D0[iyp], d3 = D3[id], d3n = D3[ixp];
   real hx = a.lam_f[3][id], hy = a.lam_f[4][id], hz = a.lam_f[5][id];
-  if (nz > 1) hx += dt_(a.lam_d[2], 1, k + 1 < nz, a.inv_d[2]);      // Ey: +d_z Hx
-  if (ny > 1) hx += dt_(a.lam_d[5], sy, j + 1 < ny, a.inv_d[1]);     // Ez: -d_y Hx
-  if (nz > 1) hy += dt_(a.lam_d[1], 1, k + 1 < nz, a.inv_d[2]);      // Ex: -d_z Hy
-  if (nx > 1) hy += dt_(a.lam_d[4], sx, i + 1 < nx, a.inv_d[0]);     // Ez: +d_x Hy
-  if (ny > 1) hz += dt_(a.lam_d[0], sy, j + 1 < ny, a.inv_d[1]);     // Ex: +d_y Hz
-  if (nx > 1) hz += dt_(a.lam_d[3], sx, i + 1 < nx, a.inv_d[0]);     // Ey: -d_x Hz
+  if (nz > 1) hx += dt_(d2, d2n, nxt_z, a.inv_d[2]);      // Ey: +d_z Hx
+  if (ny > 1) hx += dt_(d5, d5n, nxt_y, a.inv_d[1]);      // Ez: -d_y Hx
+  if (nz > 1) hy += dt_(d1, d1n, nxt_z, a.inv_d[2]);      // Ex: -d_z Hy
+  if (nx > 1) hy += dt_(d4, d4n, nxt_x, a.inv_d[0]);      // Ez: +d_x Hy
+  if (ny > 1) hz += dt_(d0, d0n, nxt_y, a.inv_d[1]);      // Ex: +d_y Hz
+  if (nx > 1) hz += dt_(d3, d3n, nxt_x, a.inv_d[0]);      // Ey: -d_x Hz
   a.lam_f[3][id] = hx;
   a.lam_f[4][id] = hy;
   a.lam_f[5][id] = hz;
@@ -134,21 +160,32 @@ __global__ __launch_bounds__(256) void k_adj_h(AdjArgs<real> a) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= a.nyz) return;
   const int i = blockIdx.y;
-  const int j = (int)(q / a.nz), k = (int)(q - (q / a.nz) * a.nz);
+  int j, k;
+  plane_jk(q, a.nyz, a.nz, j, k);
   const int64_t id = (int64_t)i * a.nyz + q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sx = a.nyz, sy = nz;
   const real* Ex = a.f[0]; const real* Ey = a.f[1]; const real* Ez = a.f[2];
   const real K = -a.ch;
   const bool xi = nx == 1 || i < nx - 1, yj = ny == 1 || j < ny - 1, zk = nz == 1 || k < nz - 1;
+  const int64_t ixp = nx > 1 && i < nx - 1 ? id + sx : id;
+  const int64_t iyp = ny > 1 && j < ny - 1 ? id + sy : id;
+  const int64_t izp = nz > 1 && k < nz - 1 ? id + 1 : id;
+  const real ex = Ex[id], ey = Ey[id], ez = Ez[id];
+  const real ez_y = Ez[iyp], ey_z = Ey[izp], ex_z = Ex[izp];
+  const real ez_x = Ez[ixp], ey_x = Ey[ixp], ex_y = Ex[iyp];
+  const real l3 = a.lam_f[3][id], l4 = a.lam_f[4][id], l5 = a.lam_f[5][id];
+  const int mx = nx > 1 ? a.map_h[0][i] : -1;
+  const int my = ny > 1 ? a.map_h[1][j] : -1;
+  const int mz = nz > 1 ? a.map_h[2][k] : -1;
   real unused = 0;
   // slots of lam_d: 0 HxY 1 HxZ 2 HyZ 3 HyX 4 HzX 5 HzY
   {  // Hx: +d_y Ez, -d_z Ey
     real dA_l = 0, dB_l = 0;
     if (yj && zk) {
-      const real g = K * a.lam_f[3][id];
-      if (ny > 1) dA_l = term_t(a, false, 1, PSI_HXY, i, j, k, g, real(1), (Ez[id + sy] - Ez[id]) * a.inv_d[1], unused);
-      if (nz > 1) dB_l = term_t(a, false, 2, PSI_HXZ, i, j, k, g, real(-1), (Ey[id + 1] - Ey[id]) * a.inv_d[2], unused);
+      const real g = K * l3;
+      if (ny > 1) dA_l = term_t(a, false, 1, PSI_HXY, i, j, k, my, g, real(1), (ez_y - ez) * a.inv_d[1], unused);
+      if (nz > 1) dB_l = term_t(a, false, 2, PSI_HXZ, i, j, k, mz, g, real(-1), (ey_z - ey) * a.inv_d[2], unused);
     }
     a.lam_d[0][id] = dA_l;
     a.lam_d[1][id] = dB_l;
@@ -156,9 +193,9 @@ __global__ __launch_bounds__(256) void k_adj_h(AdjArgs<real> a) {
   {  // Hy: +d_z Ex, -d_x Ez
     real dA_l = 0, dB_l = 0;
     if (zk && xi) {
-      const real g = K * a.lam_f[4][id];
-      if (nz > 1) dA_l = term_t(a, false, 2, PSI_HYZ, i, j, k, g, real(1), (Ex[id + 1] - Ex[id]) * a.inv_d[2], unused);
-      if (nx > 1) dB_l = term_t(a, false, 0, PSI_HYX, i, j, k, g, real(-1), (Ez[id + sx] - Ez[id]) * a.inv_d[0], unused);
+      const real g = K * l4;
+      if (nz > 1) dA_l = term_t(a, false, 2, PSI_HYZ, i, j, k, mz, g, real(1), (ex_z - ex) * a.inv_d[2], unused);
+      if (nx > 1) dB_l = term_t(a, false, 0, PSI_HYX, i, j, k, mx, g, real(-1), (ez_x - ez) * a.inv_d[0], unused);
     }
     a.lam_d[2][id] = dA_l;
     a.lam_d[3][id] = dB_l;
@@ -166,9 +203,9 @@ __global__ __launch_bounds__(256) void k_adj_h(AdjArgs<real> a) {
   {  // Hz: +d_x Ey, -d_y Ex
     real dA_l = 0, dB_l = 0;
     if (xi && yj) {
-      const real g = K * a.lam_f[5][id];
-      if (nx > 1) dA_l = term_t(a, false, 0, PSI_HZX, i, j, k, g, real(1), (Ey[id + sx] - Ey[id]) * a.inv_d[0], unused);
-      if (ny > 1) dB_l = term_t(a, false, 1, PSI_HZY, i, j, k, g, real(-1), (Ex[id + sy] - Ex[id]) * a.inv_d[1], unused);
+      const real g = K * l5;
+      if (nx > 1) dA_l = term_t(a, false, 0, PSI_HZX, i, j, k, mx, g, real(1), (ey_x - ey) * a.inv_d[0], unused);
+      if (ny > 1) dB_l = term_t(a, false, 1, PSI_HZY, i, j, k, my, g, real(-1), (ex_y - ex) * a.inv_d[1], unused);
     }
     a.lam_d[4][id] = dA_l;
     a.lam_d[5][id] = dB_l;
@@ -182,20 +219,29 @@ __global__ __launch_bounds__(256) void k_adj_h_gather(AdjArgs<real> a) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= a.nyz) return;
   const int i = blockIdx.y;
-  const int j = (int)(q / a.nz), k = (int)(q - (q / a.nz) * a.nz);
+  int j, k;
+  plane_jk(q, a.nyz, a.nz, j, k);
   const int64_t id = (int64_t)i * a.nyz + q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sx = a.nyz, sy = nz;
-  auto dt_ = [&](const real* v, int64_t s, bool has_prev, real inv) {
-    return ((has_prev ? v[id - s] : real(0)) - v[id]) * inv;
+  const bool prv_x = i >= 1, prv_y = j >= 1, prv_z = k >= 1;
+  const int64_t ixm = prv_x ? id - sx : id, iym = prv_y ? id - sy : id, izm = prv_z ? id - 1 : id;
+  // (v[q - s] - v[q]) / d with v[q - s] = 0 before the first node
+  auto dt_ = [&](real v, real vp, bool has_prev, real inv) {
+    return ((has_prev ? vp : real(0)) - v) * inv;
   };
+  const real *D0 = a.lam_d[0], *D1 = a.lam_d[1], *D2 = a.lam_d[2];
+  const real *D3 = a.lam_d[3], *D4 = a.lam_d[4], *D5 = a.lam_d[5];
+  const real d2 = D2[id], d2p = D2[izm], d5 = D5[id], d5p = D5[iym];
+  const real d1 = D1[id], d1p = D1[izm], d4 = D4[id], d4p = D4[ixm];
+  const real d0 = D0[id], d0p = D0[iym], d3 = D3[id], d3p = D3[ixm];
   real ex = a.lam_f[0][id], ey = a.lam_f[1][id], ez = a.lam_f[2][id];
-  if (nz > 1) ex += dt_(a.lam_d[2], 1, k >= 1, a.inv_d[2]);      // Hy: +d_z Ex
-  if (ny > 1) ex += dt_(a.lam_d[5], sy, j >= 1, a.inv_d[1]);     // Hz: -d_y Ex
-  if (nz > 1) ey += dt_(a.lam_d[1], 1, k >= 1, a.inv_d[2]);      // Hx: -d_z Ey
-  if (nx > 1) ey += dt_(a.lam_d[4], sx, i >= 1, a.inv_d[0]);     // Hz: +d_x Ey
-  if (ny > 1) ez += dt_(a.lam_d[0], sy, j >= 1, a.inv_d[1]);     // Hx: +d_y Ez
-  if (nx > 1) ez += dt_(a.lam_d[3], sx, i >= 1, a.inv_d[0]);     // Hy: -d_x Ez
+  if (nz > 1) ex += dt_(d2, d2p, prv_z, a.inv_d[2]);      // Hy: +d_z Ex
+  if (ny > 1) ex += dt_(d5, d5p, prv_y, a.inv_d[1]);      // Hz: -d_y Ex
+  if (nz > 1) ey += dt_(d1, d1p, prv_z, a.inv_d[2]);      // Hx: -d_z Ey
+  if (nx > 1) ey += dt_(d4, d4p, prv_x, a.inv_d[0]);      // Hz: +d_x Ey
+  if (ny > 1) ez += dt_(d0, d0p, prv_y, a.inv_d[1]);      // Hx: +d_y Ez
+  if (nx > 1) ez += dt_(d3, d3p, prv_x, a.inv_d[0]);      // Hy: -d_x Ez
   a.lam_f[0][id] = ex;
   a.lam_f[1][id] = ey;
   a.lam_f[2][id] = ez;

@@ -57,19 +57,6 @@ __device__ __forceinline__ real apply_terms_v(real f, real K, const Term<real>& 
   return f;
 }
 
-// (j, k) of flat in-plane index q; 32-bit division when the plane allows it.
-__device__ __forceinline__ void plane_jk(int64_t q, int64_t nyz, int nz, int& j, int& k) {
-  if (nyz <= 0x7FFFFFFF) {
-    const uint32_t u = (uint32_t)q, z = (uint32_t)nz;
-    const uint32_t jj = u / z;
-    j = (int)jj;
-    k = (int)(u - jj * z);
-  } else {
-    j = (int)(q / nz);
-    k = (int)(q - (int64_t)j * nz);
-  }
-}
-
 // Every field operand of a node is loaded up front (neighbour indices clamped
 // to the node itself where that difference is not taken, so each load is in
 // bounds) and the PML maps of the three axes once, so a wavefront keeps all

@@ -76,6 +76,21 @@ __host__ __device__ inline int64_t zslot(int64_t k, int zpad) {
 // Row length (slots) of the padded layout for nz nodes, zpad nodes per block.
 inline int zpad_row(int nz, int zpad) { return ZPAD_BLOCK * ((nz + zpad - 1) / zpad); }
 
+// (j, k) of the flat in-plane index q of a plane of nyz nodes with rows of nz
+// (the per-node kernels' thread mapping); 32-bit division when the plane
+// allows it.
+__device__ __forceinline__ void plane_jk(int64_t q, int64_t nyz, int nz, int& j, int& k) {
+  if (nyz <= 0x7FFFFFFF) {
+    const uint32_t u = (uint32_t)q, z = (uint32_t)nz;
+    const uint32_t jj = u / z;
+    j = (int)jj;
+    k = (int)(u - jj * z);
+  } else {
+    j = (int)(q / nz);
+    k = (int)(q - (int64_t)j * nz);
+  }
+}
+
 // Arguments of the march kernel: a few base pointers, everything else derived.
 constexpr int MARCH_MAX_XCHUNK = 128;
 template <typename real>
