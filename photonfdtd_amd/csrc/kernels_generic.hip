@@ -286,18 +286,30 @@ template <typename real>
 __global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __restrict__ comps,
                              int n_comp, BoxDev box, int64_t plane, int row, int zpad,
                              real* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // blockIdx.y = component (its field pointer is one scalar load per wave);
+  // x covers the box's nodes, z fastest
+  const int c = blockIdx.y;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
-  if (t >= nbox * n_comp) return;
-  const int c = (int)(t / nbox);
-  const int64_t r = t - (int64_t)c * nbox;
-  const int64_t bk = r % box.count[2];
-  const int64_t bj = (r / box.count[2]) % box.count[1];
-  const int64_t bi = r / (box.count[2] * box.count[1]);
+  if (r >= nbox) return;
+  int64_t bi, bj, bk;
+  if (nbox <= 0x7FFFFFFF) {
+    const uint32_t u = (uint32_t)r, c2 = (uint32_t)box.count[2], c1 = (uint32_t)box.count[1];
+    const uint32_t q = u / c2;
+    bk = u - q * c2;
+    bi = q / c1;
+    bj = q - (uint32_t)bi * c1;
+  } else {
+    const int64_t q = r / box.count[2];
+    bk = r - q * box.count[2];
+    bi = q / box.count[1];
+    bj = q - bi * box.count[1];
+  }
+  const real* F = fields[comps[c]];
   const int64_t node = (box.start[0] + bi * box.stride[0]) * plane +
                        (box.start[1] + bj * box.stride[1]) * row +
                        zslot(box.start[2] + bk * box.stride[2], zpad);
-  out[t] = fields[comps[c]][node];
+  out[(int64_t)c * nbox + r] = F[node];
 }
 
 // Fold `count` buffered samples into the complex128 accumulators, one sample
@@ -414,8 +426,9 @@ template <typename real>
 void launch_gather_box(real* const* fields_dev, const int32_t* comps, int n_comp, const BoxDev& box,
                        int64_t plane, int row, int zpad, real* out, hipStream_t s) {
   const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
-  k_gather_box<real><<<blocks_for(nbox * n_comp, 256), 256, 0, s>>>(fields_dev, comps, n_comp, box,
-                                                                     plane, row, zpad, out);
+  if (nbox <= 0 || n_comp <= 0) return;
+  const dim3 grid(blocks_for(nbox, 256), (unsigned)n_comp);
+  k_gather_box<real><<<grid, 256, 0, s>>>(fields_dev, comps, n_comp, box, plane, row, zpad, out);
 }
 
 template <typename real>
