@@ -320,6 +320,13 @@ class Stepper final : public pf_handle {
   int level() const { return cur_; }
   int march_dist() const { return march_dist_; }
   void after_march(int64_t step);
+  // Device tables of the ADE media per time level read, for one launch of
+  // all media per step (built on first use when there are several media;
+  // their cell lists are disjoint, each cell having one owner).
+  bool ade_tab1_built_ = false;
+  pfhip::AdeJob<real>* ade_tab1_[2] = {nullptr, nullptr};
+  int ade_n_jobs1_ = 0;
+  int64_t ade_max_n1_ = 0;
   void adopt_stream(hipStream_t s);
   // The batched ADE and source passes of a 3-D batch: this simulation's
   // jobs for a step that reads level p; finish_step then flips the level
@@ -1275,10 +1282,25 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
 template <typename real>
 void Stepper<real>::after_march(int64_t step) {
   const int in = cur_, out = cur_ ^ 1;
+  if (ade_.size() > 1 && !ade_tab1_built_) {
+    ade_tab1_built_ = true;
+    for (int p = 0; p < 2; ++p) {
+      std::vector<pfhip::AdeJob<real>> jobs;
+      ade_jobs(p, jobs);
+      ade_n_jobs1_ = (int)jobs.size();
+      for (const pfhip::AdeJob<real>& J : jobs) ade_max_n1_ = std::max(ade_max_n1_, J.n);
+      ade_tab1_[p] = arena_.upload(jobs.data(), jobs.size(), stream_);
+    }
+  }
   timed(PF_K_AUX, [&] {
-    for (Ade& a : ade_)
-      pfhip::launch_ade_pp<real>(f_[in], f_[out], a.cells, a.n, a.Pcur, a.Pprev, a.abc,
-                                 a.n_poles, a.inv_eps, stream_);
+    if (ade_tab1_[in] && ade_n_jobs1_ > 0) {
+      // every medium in one launch (the same per-node code as k_ade_pp)
+      pfhip::launch_ade_jobs<real>(ade_tab1_[in], ade_n_jobs1_, ade_max_n1_, stream_);
+    } else {
+      for (Ade& a : ade_)
+        pfhip::launch_ade_pp<real>(f_[in], f_[out], a.cells, a.n, a.Pcur, a.Pprev, a.abc,
+                                   a.n_poles, a.inv_eps, stream_);
+    }
     cur_ = out;
     pfhip::launch_inject<real>(fields_dev_[cur_], src_e_, n_steps_, step, stream_);
     for (Flux& f : flux_)
