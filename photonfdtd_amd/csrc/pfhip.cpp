@@ -1683,7 +1683,7 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_hos
     const size_t off = sl < 6 ? (size_t)(args_.psi[sl] - args_.psi[0])
                               : psi_len_e_ + (size_t)(args_.psi[sl] - args_.psi[6]);
     a.lam_psi[sl] = lam + nf + off;
-    a.psi_fwd[sl] = args_.psi[sl];   // E side: the live state; H side set per step
+    a.psi_fwd[sl] = args_.psi[sl];   // both sides reset per reverse step (trajectory)
   }
   a.grad = grad;
   for (int ax = 0; ax < 3; ++ax) {
@@ -1721,12 +1721,23 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_hos
                                            args_.nyz, args_.nz, cot_dev[m], d.n_freq, d.phase_e,
                                            d.phase_h, st / d.interval, d.dt, stream_);
       }
-      // rebuild H^{n+1/2} from the stored pre-step state
-      load_state(T);
-      pfhip::launch_generic_h<real>(args_, exact_, stream_);
+      // rebuild H^{n+1/2} from the stored pre-step state: H and its psi are
+      // copied back into the live arrays (the H update works in place), E^n
+      // and the E-side psi are read where the trajectory holds them
+      HIP_CHECK(hipMemcpyAsync(f_[0][3], T + 3 * (size_t)ncell, 3 * (size_t)ncell * sizeof(real),
+                               hipMemcpyDeviceToDevice, stream_));
+      if (psi_len_h_)
+        HIP_CHECK(hipMemcpyAsync(psi_side_[1], T + nf + psi_len_e_, psi_len_h_ * sizeof(real),
+                                 hipMemcpyDeviceToDevice, stream_));
+      pfhip::StepArgs<real> hs = args_;
+      for (int c = 0; c < 3; ++c) hs.f[c] = const_cast<real*>(T) + (size_t)c * ncell;
+      pfhip::launch_generic_h<real>(hs, exact_, stream_);
       pfhip::launch_inject<real>(fields_dev_[0], src_h_, n_steps_, st, stream_);
-      for (int sl = 0; sl < 6; ++sl)
+      for (int c = 0; c < 3; ++c) a.f[c] = T + (size_t)c * ncell;
+      for (int sl = 0; sl < 6; ++sl) {
+        a.psi_fwd[sl] = T + nf + (size_t)(args_.psi[sl] - args_.psi[0]);
         a.psi_fwd[6 + sl] = T + nf + psi_len_e_ + (size_t)(args_.psi[6 + sl] - args_.psi[6]);
+      }
       for (int phase = 0; phase < 4; ++phase) pfhip::launch_adjoint_step<real>(a, stream_, phase);
     }
   }
