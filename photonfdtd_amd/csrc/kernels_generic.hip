@@ -25,6 +25,7 @@ __device__ __forceinline__ void pml_term(Term<real>& t, const StepArgs<real>& a,
   const int node = ax == 0 ? i : (ax == 1 ? j : k);
   const int pc = e_side ? a.pe[ax] : a.ph[ax];
   t.psi = a.psi[psi_slot];
+  t.psw = a.psio[psi_slot] ? a.psio[psi_slot] : t.psi;
   t.pid = psi_index(ax, m, i, j, k, a.ny, a.nz, pc);
   t.b = e_side ? a.b_e[ax][node] : a.b_h[ax][node];
   t.c = e_side ? a.c_e[ax][node] : a.c_h[ax][node];
@@ -46,12 +47,12 @@ __device__ __forceinline__ real apply_terms_v(real f, real K, const Term<real>& 
   f = f + cb * K;
   if (A.on && A.psi) {
     real p = A.b * A.psi[A.pid] + A.c * A.d;
-    A.psi[A.pid] = p;
+    A.psw[A.pid] = p;
     f = f + K * p;
   }
   if (B.on && B.psi) {
     real p = B.b * B.psi[B.pid] + B.c * B.d;
-    B.psi[B.pid] = p;
+    B.psw[B.pid] = p;
     f = f - K * p;
   }
   return f;
@@ -84,6 +85,13 @@ __global__ __launch_bounds__(256) void k_update_h(StepArgs<real> a) {
   const real ez_y = Ez[iyp], ey_z = Ey[izp], ex_z = Ex[izp];
   const real ez_x = Ez[ixp], ey_x = Ey[ixp], ex_y = Ex[iyp];
   const real hx = a.f[3][id], hy = a.f[4][id], hz = a.f[5][id];
+  // out of place: every node is written (unchanged where no term applies)
+  const bool oop = a.fo[3] != nullptr;
+  real* Hxo = oop ? a.fo[3] : a.f[3];
+  real* Hyo = oop ? a.fo[4] : a.f[4];
+  real* Hzo = oop ? a.fo[5] : a.f[5];
+  real nhx = hx, nhy = hy, nhz = hz;
+  bool ux = false, uy = false, uz = false;
   const int mx = nx > 1 ? a.map_h[0][i] : -1;
   const int my = ny > 1 ? a.map_h[1][j] : -1;
   const int mz = nz > 1 ? a.map_h[2][k] : -1;
@@ -94,7 +102,10 @@ __global__ __launch_bounds__(256) void k_update_h(StepArgs<real> a) {
     B.on = nz > 1;
     if (A.on) { A.d = diff<real, EXACT>(ez_y, ez, a.d[1], a.inv_d[1]); pml_term(A, a, false, 1, PSI_HXY, i, j, k, my); }
     if (B.on) { B.d = diff<real, EXACT>(ey_z, ey, a.d[2], a.inv_d[2]); pml_term(B, a, false, 2, PSI_HXZ, i, j, k, mz); }
-    if (A.on || B.on) a.f[3][id] = apply_terms_v(hx, K, A, B);
+    if (A.on || B.on) {
+      nhx = apply_terms_v(hx, K, A, B);
+      ux = true;
+    }
   }
   // Hy -= ch (d_z Ex - d_x Ez)
   if (zk && xi) {
@@ -102,7 +113,10 @@ __global__ __launch_bounds__(256) void k_update_h(StepArgs<real> a) {
     B.on = nx > 1;
     if (A.on) { A.d = diff<real, EXACT>(ex_z, ex, a.d[2], a.inv_d[2]); pml_term(A, a, false, 2, PSI_HYZ, i, j, k, mz); }
     if (B.on) { B.d = diff<real, EXACT>(ez_x, ez, a.d[0], a.inv_d[0]); pml_term(B, a, false, 0, PSI_HYX, i, j, k, mx); }
-    if (A.on || B.on) a.f[4][id] = apply_terms_v(hy, K, A, B);
+    if (A.on || B.on) {
+      nhy = apply_terms_v(hy, K, A, B);
+      uy = true;
+    }
   }
   // Hz -= ch (d_x Ey - d_y Ex)
   if (xi && yj) {
@@ -110,8 +124,14 @@ __global__ __launch_bounds__(256) void k_update_h(StepArgs<real> a) {
     B.on = ny > 1;
     if (A.on) { A.d = diff<real, EXACT>(ey_x, ey, a.d[0], a.inv_d[0]); pml_term(A, a, false, 0, PSI_HZX, i, j, k, mx); }
     if (B.on) { B.d = diff<real, EXACT>(ex_y, ex, a.d[1], a.inv_d[1]); pml_term(B, a, false, 1, PSI_HZY, i, j, k, my); }
-    if (A.on || B.on) a.f[5][id] = apply_terms_v(hz, K, A, B);
+    if (A.on || B.on) {
+      nhz = apply_terms_v(hz, K, A, B);
+      uz = true;
+    }
   }
+  if (ux || oop) Hxo[id] = nhx;
+  if (uy || oop) Hyo[id] = nhy;
+  if (uz || oop) Hzo[id] = nhz;
 }
 
 template <typename real, bool EXACT>
@@ -124,7 +144,14 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
   const int64_t id = (int64_t)i * a.nyz + q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sx = a.nyz, sy = nz;
-  const real* Hx = a.f[3]; const real* Hy = a.f[4]; const real* Hz = a.f[5];
+  const bool oop = a.fo[0] != nullptr;
+  // H^{n+1/2}: written by the H pass (to fo when out of place)
+  const real* Hx = oop ? a.fo[3] : a.f[3];
+  const real* Hy = oop ? a.fo[4] : a.f[4];
+  const real* Hz = oop ? a.fo[5] : a.f[5];
+  real* Exo = oop ? a.fo[0] : a.f[0];
+  real* Eyo = oop ? a.fo[1] : a.f[1];
+  real* Ezo = oop ? a.fo[2] : a.f[2];
   const bool xi = nx == 1 || i >= 1;
   const bool yj = ny == 1 || j >= 1;
   const bool zk = nz == 1 || k >= 1;
@@ -136,6 +163,8 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
   const real hz_x = Hz[ixm], hy_x = Hy[ixm], hx_y = Hx[iym];
   const real ex = a.f[0][id], ey = a.f[1][id], ez = a.f[2][id];
   const real K = a.ce[id];
+  real nex = ex, ney = ey, nez = ez;
+  bool ux = false, uy = false, uz = false;
   const int mx = nx > 1 ? a.map_e[0][i] : -1;
   const int my = ny > 1 ? a.map_e[1][j] : -1;
   const int mz = nz > 1 ? a.map_e[2][k] : -1;
@@ -146,7 +175,10 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
     B.on = nz > 1;
     if (A.on) { A.d = diff<real, EXACT>(hz, hz_y, a.d[1], a.inv_d[1]); pml_term(A, a, true, 1, PSI_EXY, i, j, k, my); }
     if (B.on) { B.d = diff<real, EXACT>(hy, hy_z, a.d[2], a.inv_d[2]); pml_term(B, a, true, 2, PSI_EXZ, i, j, k, mz); }
-    if (A.on || B.on) a.f[0][id] = apply_terms_v(ex, K, A, B);
+    if (A.on || B.on) {
+      nex = apply_terms_v(ex, K, A, B);
+      ux = true;
+    }
   }
   // Ey += ce (d_z Hx - d_x Hz)
   if (zk && xi) {
@@ -154,7 +186,10 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
     B.on = nx > 1;
     if (A.on) { A.d = diff<real, EXACT>(hx, hx_z, a.d[2], a.inv_d[2]); pml_term(A, a, true, 2, PSI_EYZ, i, j, k, mz); }
     if (B.on) { B.d = diff<real, EXACT>(hz, hz_x, a.d[0], a.inv_d[0]); pml_term(B, a, true, 0, PSI_EYX, i, j, k, mx); }
-    if (A.on || B.on) a.f[1][id] = apply_terms_v(ey, K, A, B);
+    if (A.on || B.on) {
+      ney = apply_terms_v(ey, K, A, B);
+      uy = true;
+    }
   }
   // Ez += ce (d_x Hy - d_y Hx)
   if (xi && yj) {
@@ -162,8 +197,14 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
     B.on = ny > 1;
     if (A.on) { A.d = diff<real, EXACT>(hy, hy_x, a.d[0], a.inv_d[0]); pml_term(A, a, true, 0, PSI_EZX, i, j, k, mx); }
     if (B.on) { B.d = diff<real, EXACT>(hx, hx_y, a.d[1], a.inv_d[1]); pml_term(B, a, true, 1, PSI_EZY, i, j, k, my); }
-    if (A.on || B.on) a.f[2][id] = apply_terms_v(ez, K, A, B);
+    if (A.on || B.on) {
+      nez = apply_terms_v(ez, K, A, B);
+      uz = true;
+    }
   }
+  if (ux || oop) Exo[id] = nex;
+  if (uy || oop) Eyo[id] = ney;
+  if (uz || oop) Ezo[id] = nez;
 }
 
 // Soft sources: one thread per distinct target node; equal targets are

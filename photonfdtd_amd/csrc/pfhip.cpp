@@ -1703,13 +1703,43 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_hos
   a.nyz = args_.nyz;
   a.ch = args_.ch;
 
+  // field tables of the trajectory slots (the source kernels take one)
+  std::vector<real*> slot_ptrs((size_t)C * 6);
+  for (int k = 0; k < C; ++k)
+    for (int c = 0; c < 6; ++c) slot_ptrs[(size_t)k * 6 + c] = traj + (size_t)k * elems + (size_t)c * ncell;
+  real** slot_tab = scratch.upload(slot_ptrs.data(), slot_ptrs.size(), stream_);
+  // psi slot offsets inside a stored state (E side, then H side)
+  size_t psi_off[pfhip::PSI_COUNT];
+  for (int sl = 0; sl < pfhip::PSI_COUNT; ++sl)
+    psi_off[sl] = sl < 6 ? (size_t)(args_.psi[sl] - args_.psi[0])
+                         : psi_len_e_ + (size_t)(args_.psi[sl] - args_.psi[6]);
+
   const int64_t nseg = (n_steps_ + C - 1) / C;
   for (int64_t sg = nseg - 1; sg >= 0; --sg) {
     const int64_t s0 = sg * C, s1 = std::min<int64_t>(s0 + C, n_steps_);
-    load_state(ckpts_[sg]);
-    for (int64_t st = s0; st < s1; ++st) {
-      save_state(traj + (size_t)(st - s0) * elems);
-      step_once(st, false);
+    // the segment's pre-step states: slot 0 is the checkpoint; each step then
+    // runs out of place from slot k into slot k + 1 (the state after the
+    // segment's last step is not needed)
+    HIP_CHECK(hipMemcpyAsync(traj, ckpts_[sg], elems * sizeof(real), hipMemcpyDeviceToDevice, stream_));
+    for (int64_t st = s0; st + 1 < s1; ++st) {
+      const int k = (int)(st - s0);
+      real* Tin = traj + (size_t)k * elems;
+      real* Tout = Tin + elems;
+      pfhip::StepArgs<real> sa = args_;
+      for (int c = 0; c < 6; ++c) {
+        sa.f[c] = Tin + (size_t)c * ncell;
+        sa.fo[c] = Tout + (size_t)c * ncell;
+      }
+      for (int sl = 0; sl < pfhip::PSI_COUNT; ++sl) {
+        sa.psi[sl] = args_.psi[sl] ? Tin + nf + psi_off[sl] : nullptr;
+        sa.psio[sl] = args_.psi[sl] ? Tout + nf + psi_off[sl] : nullptr;
+      }
+      real* const* out_tab = slot_tab + (size_t)(k + 1) * 6;
+      // the generic step's order: H, H sources, E, E sources (no dispersive media here)
+      pfhip::launch_generic_h<real>(sa, exact_, stream_);
+      pfhip::launch_inject<real>(out_tab, src_h_, n_steps_, st, stream_);
+      pfhip::launch_generic_e<real>(sa, exact_, stream_);
+      pfhip::launch_inject<real>(out_tab, src_e_, n_steps_, st, stream_);
     }
     for (int64_t st = s1 - 1; st >= s0; --st) {
       const real* T = traj + (size_t)(st - s0) * elems;

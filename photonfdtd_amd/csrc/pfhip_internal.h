@@ -19,6 +19,11 @@ struct StepArgs {
   real* f[6];            // Ex, Ey, Ez, Hx, Hy, Hz
   const real* ce;        // per-node E coefficient
   real* psi[PSI_COUNT];  // compact CPML state
+  // Out-of-place update (null: in place). The H pass reads E and H from f
+  // and writes H to fo[3..5]; the E pass reads the new H from fo[3..5] and E
+  // from f and writes E to fo[0..2]; psi is read from psi, written to psio.
+  real* fo[6];
+  real* psio[PSI_COUNT];
   const real* b_e[3];
   const real* c_e[3];
   const real* b_h[3];

@@ -26,6 +26,7 @@ struct Term {
   bool on;
   real d;
   real* psi;    // nullptr when the node is outside the PML slab
+  real* psw;    // where the updated psi is written (psi, or the out-of-place array)
   int64_t pid;
   real b, c;
 };
