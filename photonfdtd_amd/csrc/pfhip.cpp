@@ -386,7 +386,6 @@ class Stepper final : public pf_handle {
 
   void step_once(int64_t step, bool monitors = true);
   void save_state(real* dst);
-  void load_state(const real* src);
   void capture_dft(int64_t step);
   void fold(Dft& m);
   real* staging(size_t elems);
@@ -1450,18 +1449,6 @@ void Stepper<real>::save_state(real* dst) {
                              hipMemcpyDeviceToDevice, stream_));
   if (psi_len_h_)
     HIP_CHECK(hipMemcpyAsync(dst + nf + psi_len_e_, psi_side_[1], psi_len_h_ * sizeof(real),
-                             hipMemcpyDeviceToDevice, stream_));
-}
-
-template <typename real>
-void Stepper<real>::load_state(const real* src) {
-  const size_t nf = 6 * (size_t)(shape[0] * shape[1] * shape[2]);
-  HIP_CHECK(hipMemcpyAsync(lvl_[0], src, nf * sizeof(real), hipMemcpyDeviceToDevice, stream_));
-  if (psi_len_e_)
-    HIP_CHECK(hipMemcpyAsync(psi_side_[0], src + nf, psi_len_e_ * sizeof(real),
-                             hipMemcpyDeviceToDevice, stream_));
-  if (psi_len_h_)
-    HIP_CHECK(hipMemcpyAsync(psi_side_[1], src + nf + psi_len_e_, psi_len_h_ * sizeof(real),
                              hipMemcpyDeviceToDevice, stream_));
 }
 
