@@ -170,6 +170,43 @@ def parity_and_cpu(n=64, steps=1000):
                 ref_fp32_rel_l2=errs["ref_float32"])
 
 
+def fullsize_fp32_error(n, steps):
+    """Relative L2 of the float32 stepper (fast order) against the float64
+    stepper (reference order, bitwise equal to the oracle on every scene the
+    tests compare) on the full workload: final six fields and both DFT
+    planes, n^3 cells x ``steps`` steps, both on the GPU."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.hip_backend import HipEngine
+
+    t0 = time.perf_counter()
+    sim32 = dispersive_slab(pf, n, steps, "float32")
+    eng = HipEngine(sim32, np.float32, False)
+    eng.advance(0, sim32.n_steps)
+    f32 = {c: eng.get(c).copy() for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+    d32 = {nm: eng.dft(nm) for nm in ("in", "out")}
+    eng.close()
+    sim64 = dispersive_slab(pf, n, steps, "float64")
+    eng = HipEngine(sim64, np.float64, True)
+    eng.advance(0, sim64.n_steps)
+    per = {}
+    for c, a in f32.items():
+        b = eng.get(c)
+        num = den = 0.0
+        for i in range(0, b.shape[0], 32):          # bounded host temporaries
+            d = a[i:i + 32].astype(np.float64) - b[i:i + 32]
+            num += float(np.vdot(d, d).real)
+            den += float(np.vdot(b[i:i + 32], b[i:i + 32]).real)
+        per[c] = math.sqrt(num / max(den, 1e-300))
+    for nm in ("in", "out"):
+        got = eng.dft(nm)
+        for c, v in got.items():
+            per[f"dft_{nm}_{c}"] = _rel(d32[nm][c], v)
+    eng.close()
+    worst = max(per.values())
+    return dict(rel_l2=worst, steps=steps, cells=n ** 3, seconds=round(time.perf_counter() - t0, 1),
+                worst_array=max(per, key=per.get))
+
+
 PROFILES = ROOT / "profiles"
 
 
@@ -364,6 +401,9 @@ def bench_batched(args):
                                  if tuned["mode"] != "persistent" else None),
                      "bytes_per_cell_update": bytes_per_update,
                      "kernel_ms_per_step": round(dev_ms / args.steps, 4)},
+        "march": {"psi_free_cell_fraction": round(info["cells_bulk"] / cells, 4),
+                  "tiles_psi_free": info["tiles_bulk"], "tiles_cpml": info["tiles_cpml"],
+                  "dft_ring": info["dft_ring"]},
         "dft_finite": bool(finite),
         "setup_s": round(setup_s, 1),
     }
@@ -464,6 +504,9 @@ def bench_batched3d(args):
         "batch_device_ms_per_step": round(dev_ms / args.steps, 4),
         "single_runs_ms_per_step": round(single / args.steps * 1e3, 4),
         "speedup_vs_single_runs": round(single / wall, 3),
+        "march": {"psi_free_cell_fraction": round(info["cells_bulk"] / cells, 4),
+                  "tiles_psi_free": info["tiles_bulk"], "tiles_cpml": info["tiles_cpml"],
+                  "dft_ring": info["dft_ring"]},
         "dft_finite": bool(finite),
     }
     print(json.dumps(out), flush=True)
@@ -724,6 +767,8 @@ def main():
                     help="launch modes the batched autotuner may pick from")
     ap.add_argument("--precision", default="float32", choices=("float32", "float64"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg")
+    ap.add_argument("--check-steps", type=int, default=2000,
+                    help="steps of the full-size float32-vs-float64 check (0: skip)")
     ap.add_argument("--summarize", nargs=2, metavar=("DIR", "KIND"),
                     help="print a summary of rocprofv3 CSV output (KIND: stats | pmc)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -802,6 +847,9 @@ def main():
     N.check(lib, lib.pf_timing_enable(eng.h, 1), "pf_timing_enable")
     t0 = time.perf_counter()
     eng.advance(args.warmup, args.steps)
+    # fold the DFT samples still buffered, so the timed region holds the
+    # float64 accumulation of every timed step whatever --steps is
+    eng.dft_flush()
     wall = time.perf_counter() - t0
     ms = np.zeros(len(N.KERNEL_CLASSES), dtype=np.float64)
     cnt = np.zeros(len(N.KERNEL_CLASSES), dtype=np.int64)
@@ -812,7 +860,12 @@ def main():
     # one full-field gather: a plain copy kernel of known size (n cells read,
     # n cells written) that calibrates the FETCH_SIZE / WRITE_SIZE counters
     field_max = float(np.abs(eng.get("Ey")).max())
+    info = eng.info()
     eng.close()
+    full = None
+    if args.check_steps > 0 and args.precision == "float32":
+        full = fullsize_fp32_error(args.n, args.check_steps)
+        print(f"[bench] full-size float32 vs float64: {full}", file=sys.stderr, flush=True)
 
     cells = sim.grid.n_cells
     value = cells * args.steps / wall / 1e9
@@ -856,6 +909,9 @@ def main():
         "kernel_ms": {name: {"total_ms": round(t, 3), "launch_groups": int(c)}
                       for name, (t, c) in k.items()},
         "hbm_fraction_of_value": round(value * 1e9 * bytes_per_update / 1e9 / HBM_PEAK_GBS, 4),
+        "march": {"psi_free_cell_fraction": round(info["cells_bulk"] / cells, 4),
+                  "tiles_psi_free": info["tiles_bulk"], "tiles_cpml": info["tiles_cpml"],
+                  "dft_ring": info["dft_ring"]},
         "dft_finite": bool(finite),
         "ey_max_final": field_max,
         "setup_s": round(setup_s, 1),
@@ -866,6 +922,11 @@ def main():
         out["fp64_rel_l2_vs_oracle"] = check["fp64_rel_l2"]
         out["fp32_rel_l2_vs_oracle"] = check["fp32_rel_l2"]
         out["reference_order_fp32_rel_l2_vs_oracle"] = check["ref_fp32_rel_l2"]
+    if full is not None:
+        out["fullsize_fp32_vs_fp64_rel_l2"] = full["rel_l2"]
+        out["fullsize_check"] = (f"{full['cells']} cells x {full['steps']} steps, float32 fast order "
+                                 "vs float64 reference order (GPU), final fields and both DFT "
+                                 f"planes; worst array {full['worst_array']}")
     print(json.dumps(out), flush=True)
 
 

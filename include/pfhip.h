@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 5
+#define PFHIP_ABI_VERSION 6
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -131,9 +131,10 @@ enum pf_kernel_class {
   PF_K_YEE_H = 0,      /* H half-step (curl E + CPML)                       */
   PF_K_YEE_E = 1,      /* E half-step (curl H + CPML)                       */
   PF_K_YEE_FUSED = 2,  /* one launch doing a whole H+E step                 */
-  PF_K_AUX = 3,        /* per-step sources, ADE, flux and DFT capture/fold  */
+  PF_K_AUX = 3,        /* per-step sources, ADE, flux and DFT sample capture */
   PF_K_YEE_PAIR = 4,   /* one launch doing two whole steps (temporal block) */
-  PF_NKCLASS = 5
+  PF_K_DFT_FOLD = 5,   /* FP64 DFT accumulation of the buffered samples     */
+  PF_NKCLASS = 6
 };
 
 /* Replaces CudaStepper.new_f32/new_f64 (rust/src/cuda.rs:424-540). */
@@ -155,6 +156,36 @@ int pf_write_field(pf_handle* h, int32_t comp, const void* src);
  * comps[]): complex128 interleaved [n_freq][box cells]. Replaces the host
  * accumulation of rustbackend.py:384-397. */
 int pf_dft_read(pf_handle* h, int32_t mon, int32_t slot, double* dst);
+/* Folds the DFT samples the stepper still buffers into the complex128
+ * accumulators of every monitor and waits for it: afterwards the
+ * accumulators hold every step advanced so far (the reference accumulates
+ * each sampled step at once, simulation.py:1116-1133; pf_dft_read folds
+ * implicitly). */
+int pf_dft_flush(pf_handle* h);
+/* How the handle steps (no reference counterpart: the reference's CUDA
+ * stepper has one kernel pair). path: 0 generic two-pass kernels, 1 fused
+ * one-launch step on dense rows, 2 the 3-D march, 3 the march with the
+ * two-step launch. zpad: nodes per 64-slot padded z block (0: dense rows),
+ * zrow: slots per padded row. tiles_* / cells_*: march tiles (and the cells
+ * they own) that lie off every CPML slab (bulk, the psi-free kernel) and
+ * that touch one (cpml). dft_ring: samples a DFT monitor buffers before a
+ * fold (largest over monitors). */
+typedef struct pf_info {
+  int32_t path;
+  int32_t zpad;
+  int32_t zrow;
+  int32_t march_nw;
+  int32_t pml_nw;
+  int32_t reserved;
+  int64_t tiles_bulk;
+  int64_t tiles_cpml;
+  int64_t cells_bulk;
+  int64_t cells_cpml;
+  int64_t pair_tiles_bulk;
+  int64_t pair_tiles_cpml;
+  int64_t dft_ring;
+} pf_info;
+int pf_stepper_info(pf_handle* h, pf_info* out);
 /* Accumulated flux of every flux monitor, [n_flux] doubles. */
 int pf_flux_read(pf_handle* h, double* dst);
 /* Discrete adjoint. The reference differentiates its stepper through JAX

@@ -11,11 +11,11 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 PF_FLAG_GENERIC = 1
 
 PF_F32, PF_F64 = 0, 1
-KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux", "yee_pair")
+KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux", "yee_pair", "dft_fold")
 COMP_CODE = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 
 _i32, _i64, _f64, _vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p
@@ -58,6 +58,13 @@ class PfNnConv(C.Structure):
                 ("ksize", _i32), ("stride", _i32), ("relu", _i32), ("out_f32", _i32)]
 
 
+class PfInfo(C.Structure):
+    _fields_ = [("path", _i32), ("zpad", _i32), ("zrow", _i32), ("march_nw", _i32),
+                ("pml_nw", _i32), ("reserved", _i32), ("tiles_bulk", _i64), ("tiles_cpml", _i64),
+                ("cells_bulk", _i64), ("cells_cpml", _i64), ("pair_tiles_bulk", _i64),
+                ("pair_tiles_cpml", _i64), ("dft_ring", _i64)]
+
+
 class PfCotangent(C.Structure):
     _fields_ = [("mon", _i32), ("slot", _i32), ("g", _P(_f64))]
 
@@ -71,6 +78,8 @@ SIGNATURES = {
     "pf_read_box": (_i32, [_vp, _i32, _P(PfBox), _vp]),
     "pf_write_field": (_i32, [_vp, _i32, _vp]),
     "pf_dft_read": (_i32, [_vp, _i32, _i32, _P(_f64)]),
+    "pf_dft_flush": (_i32, [_vp]),
+    "pf_stepper_info": (_i32, [_vp, _P(PfInfo)]),
     "pf_flux_read": (_i32, [_vp, _P(_f64)]),
     "pf_checkpoint_enable": (_i32, [_vp, _i32]),
     "pf_adjoint": (_i32, [_vp, _i32, _P(PfCotangent), _P(_f64)]),

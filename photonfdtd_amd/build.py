@@ -33,18 +33,27 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the HIP library cannot be built")
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, jobs: int = 0) -> Path:
+    """Compile every translation unit (in parallel, ``jobs`` at a time; 0:
+    one per source up to the CPU count) and link the library."""
     cc = hipcc()
     BUILD.mkdir(exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         obj = BUILD / (Path(src).stem + ".o")
-        lang = ["-x", "hip"]
-        cmd = [cc, *COMMON, *lang, "-c", str(CSRC / src), "-o", str(obj)]
+        cmds.append([cc, *COMMON, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
+    workers = jobs or max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    running = []
+    for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+        running.append(subprocess.Popen(cmd))
+        while len([p for p in running if p.poll() is None]) >= workers:
+            running[[p.poll() is None for p in running].index(True)].wait()
+    failed = [c for p, c in zip(running, cmds) if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:

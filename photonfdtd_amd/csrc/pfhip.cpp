@@ -247,6 +247,8 @@ struct pf_handle {
   virtual void checkpoint_enable(int interval) = 0;
   virtual void adjoint(int n_cot, const pf_cotangent* cot, double* grad) = 0;
   virtual void timing_read(double* ms, int64_t* launches) = 0;
+  virtual void dft_flush() = 0;
+  virtual void info(pf_info* out) const = 0;
   int64_t shape[3];
 };
 
@@ -278,6 +280,8 @@ class Stepper final : public pf_handle {
   void read_box(int comp, const pf_box& box, void* dst) override;
   void write_field(int comp, const void* src) override;
   void dft_read(int mon, int slot, double* dst) override;
+  void dft_flush() override;
+  void info(pf_info* out) const override;
   void flux_read(double* dst) override;
   void timing_enable(bool on) override;
   void checkpoint_enable(int interval) override;
@@ -320,11 +324,13 @@ class Stepper final : public pf_handle {
   int level() const { return cur_; }
   int march_dist() const { return march_dist_; }
   void after_march(int64_t step);
-  // Device tables of the ADE media per time level read, for one launch of
-  // all media per step (built on first use when there are several media;
-  // their cell lists are disjoint, each cell having one owner).
-  bool ade_tab1_built_ = false;
-  pfhip::AdeJob<real>* ade_tab1_[2] = {nullptr, nullptr};
+  // Device tables of the ADE media for one launch of all media per step,
+  // indexed [field level read][polarisation level] (the two-step march
+  // swaps the polarisation buffers after every pair, so a single step can
+  // run at either P level). Built on first use of each pair of levels when
+  // there are several media; their cell lists are disjoint, each cell
+  // having one owner.
+  pfhip::AdeJob<real>* ade_tab1_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int ade_n_jobs1_ = 0;
   int64_t ade_max_n1_ = 0;
   void adopt_stream(hipStream_t s);
@@ -422,6 +428,7 @@ class Stepper final : public pf_handle {
   real* coef_ = nullptr;
   int* tiles_[2] = {nullptr, nullptr};   // march tiles: [0] psi-free, [1] with CPML
   int ntiles_[2] = {0, 0};
+  int64_t tile_cells_[2] = {0, 0};       // owned cells of each list
   int march_nw_ = pfhip::MARCH_NW;
   int pml_nw_ = pfhip::MARCH_NW;  // wavefronts per workgroup of the CPML-tile launch
   int march_dist_ = 2;            // planes of prefetch in the march kernel
@@ -469,8 +476,11 @@ class Stepper final : public pf_handle {
   bool timing_ = false;
   std::vector<Timed> pending_;
   std::vector<hipEvent_t> pool_;
-  double t_ms_[PF_NKCLASS] = {0, 0, 0, 0, 0};
-  int64_t t_count_[PF_NKCLASS] = {0, 0, 0, 0, 0};
+  int open_cls_ = -1;            // kernel class of the timed group being issued
+  hipEvent_t open_ev_ = nullptr; // its start event
+  hipEvent_t record_event();
+  double t_ms_[PF_NKCLASS] = {};
+  int64_t t_count_[PF_NKCLASS] = {};
 };
 
 template <typename real>
@@ -921,6 +931,8 @@ void Stepper<real>::build_march_tiles(int xchunk) {
   }
   const std::vector<pfhip::MarchTile>* lists[2] = {&bulk, &pml};
   for (int q = 0; q < 2; ++q) {
+    tile_cells_[q] = 0;
+    for (const pfhip::MarchTile& t : *lists[q]) tile_cells_[q] += (int64_t)(t.i1 - t.i0) * t.jn * t.kn;
     std::vector<pfhip::MarchTile> ordered =
         band_order_ ? xcd_band_order(*lists[q]) : xcd_order(*lists[q]);
     ntiles_[q] = (int)ordered.size();
@@ -1168,25 +1180,37 @@ void Stepper<real>::step_pair(int64_t step) {
 }
 
 template <typename real>
+hipEvent_t Stepper<real>::record_event() {
+  hipEvent_t e;
+  if (pool_.empty()) {
+    HIP_CHECK(hipEventCreate(&e));
+  } else {
+    e = pool_.back();
+    pool_.pop_back();
+  }
+  HIP_CHECK(hipEventRecord(e, stream_));
+  return e;
+}
+
+// Brackets the launches of `launch` with events on the handle's stream and
+// books the interval to kernel class `cls`. A group opened inside another
+// (a DFT fold issued by a capture in the aux group) closes the enclosing
+// group first and reopens it afterwards, so no interval is counted twice.
+template <typename real>
 template <typename F>
 void Stepper<real>::timed(int cls, F&& launch) {
   if (!timing_) {
     launch();
     return;
   }
-  hipEvent_t ev[2];
-  for (hipEvent_t& e : ev) {
-    if (pool_.empty()) {
-      HIP_CHECK(hipEventCreate(&e));
-    } else {
-      e = pool_.back();
-      pool_.pop_back();
-    }
-  }
-  HIP_CHECK(hipEventRecord(ev[0], stream_));
+  const int outer = open_cls_;
+  if (outer >= 0) pending_.push_back({open_ev_, record_event(), outer});
+  open_cls_ = cls;
+  open_ev_ = record_event();
   launch();
-  HIP_CHECK(hipEventRecord(ev[1], stream_));
-  pending_.push_back({ev[0], ev[1], cls});
+  pending_.push_back({open_ev_, record_event(), cls});
+  open_cls_ = outer;
+  if (outer >= 0) open_ev_ = record_event();
 }
 
 template <typename real>
@@ -1281,20 +1305,19 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
 template <typename real>
 void Stepper<real>::after_march(int64_t step) {
   const int in = cur_, out = cur_ ^ 1;
-  if (ade_.size() > 1 && !ade_tab1_built_) {
-    ade_tab1_built_ = true;
-    for (int p = 0; p < 2; ++p) {
-      std::vector<pfhip::AdeJob<real>> jobs;
-      ade_jobs(p, jobs);
-      ade_n_jobs1_ = (int)jobs.size();
-      for (const pfhip::AdeJob<real>& J : jobs) ade_max_n1_ = std::max(ade_max_n1_, J.n);
-      ade_tab1_[p] = arena_.upload(jobs.data(), jobs.size(), stream_);
-    }
+  pfhip::AdeJob<real>*& tab = ade_tab1_[in][p_lvl_];
+  if (ade_.size() > 1 && !tab) {
+    // ade_jobs reads the polarisation pointers of the current P level
+    std::vector<pfhip::AdeJob<real>> jobs;
+    ade_jobs(in, jobs);
+    ade_n_jobs1_ = (int)jobs.size();
+    for (const pfhip::AdeJob<real>& J : jobs) ade_max_n1_ = std::max(ade_max_n1_, J.n);
+    tab = arena_.upload(jobs.data(), jobs.size(), stream_);
   }
   timed(PF_K_AUX, [&] {
-    if (ade_tab1_[in] && ade_n_jobs1_ > 0) {
+    if (tab && ade_n_jobs1_ > 0) {
       // every medium in one launch (the same per-node code as k_ade_pp)
-      pfhip::launch_ade_jobs<real>(ade_tab1_[in], ade_n_jobs1_, ade_max_n1_, stream_);
+      pfhip::launch_ade_jobs<real>(tab, ade_n_jobs1_, ade_max_n1_, stream_);
     } else {
       for (Ade& a : ade_)
         pfhip::launch_ade_pp<real>(f_[in], f_[out], a.cells, a.n, a.Pcur, a.Pprev, a.abc,
@@ -1782,10 +1805,38 @@ void Stepper<real>::capture_dft(int64_t step) {
 template <typename real>
 void Stepper<real>::fold(Dft& m) {
   if (m.count == 0) return;
-  pfhip::launch_dft_fold<real>(m.ring, m.nbox, m.n_comp, m.is_h_dev, m.count, m.sample0, m.phase_e,
-                               m.phase_h, m.n_freq, m.dt, m.acc, stream_);
+  timed(PF_K_DFT_FOLD, [&] {
+    pfhip::launch_dft_fold<real>(m.ring, m.nbox, m.n_comp, m.is_h_dev, m.count, m.sample0,
+                                 m.phase_e, m.phase_h, m.n_freq, m.dt, m.acc, stream_);
+  });
   m.sample0 += m.count;
   m.count = 0;
+}
+
+template <typename real>
+void Stepper<real>::info(pf_info* out) const {
+  std::memset(out, 0, sizeof(*out));
+  out->path = !fused_ ? 0 : fa_[0].variant != 2 ? 1 : pair_ ? 3 : 2;
+  out->zpad = zpad_;
+  out->zrow = zrow_;
+  out->march_nw = march_nw_;
+  out->pml_nw = pml_nw_;
+  out->tiles_bulk = ntiles_[0];
+  out->tiles_cpml = ntiles_[1];
+  out->cells_bulk = tile_cells_[0];
+  out->cells_cpml = tile_cells_[1];
+  out->pair_tiles_bulk = ntiles2_[0];
+  out->pair_tiles_cpml = ntiles2_[1];
+  for (const Dft& m : dft_) out->dft_ring = std::max<int64_t>(out->dft_ring, m.K);
+}
+
+// Folds every buffered DFT sample into the complex128 accumulators and
+// waits, so the accumulators hold every step advanced so far.
+template <typename real>
+void Stepper<real>::dft_flush() {
+  for (Dft& m : dft_) fold(m);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 template <typename real>
@@ -2432,6 +2483,16 @@ int pf_write_field(pf_handle* h, int32_t comp, const void* src) {
 int pf_dft_read(pf_handle* h, int32_t mon, int32_t slot, double* dst) {
   if (!h) return fail(PF_E_ARG, "null handle");
   return guarded([&] { h->dft_read(mon, slot, dst); });
+}
+
+int pf_stepper_info(pf_handle* h, pf_info* out) {
+  if (!h || !out) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { h->info(out); });
+}
+
+int pf_dft_flush(pf_handle* h) {
+  if (!h) return fail(PF_E_ARG, "null handle");
+  return guarded([&] { h->dft_flush(); });
 }
 
 int pf_flux_read(pf_handle* h, double* dst) {

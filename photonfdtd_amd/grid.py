@@ -97,8 +97,3 @@ class Grid:
             c = self.coords[axis]
             out.append(0 if c.size == 1 else int(np.argmin(np.abs(c - pt[axis]))))
         return tuple(out)
-
-    def nearest_index(self, axis: int, position: float) -> int:
-        """Nearest node along one axis (``simulation.py:1013,1037``)."""
-        c = np.asarray(self.coords[axis])
-        return int(np.argmin(np.abs(c - position)))

@@ -16,6 +16,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import sys
+from dataclasses import dataclass
 from typing import Dict, List
 
 import numpy as np
@@ -23,14 +24,93 @@ import numpy as np
 from . import _native as N
 from .constants import EPS_0, MU_0
 from .monitors import DFTMonitor, FieldMonitor, FluxMonitor
-from .simulation import COMPUTE_KEYS, Result
+from .simulation import Result
+
+# Every name this module takes from its package (constants, monitors,
+# Simulation.Result) exists under the same name in the reference package, and
+# the only Simulation / Grid attributes it reads are the reference's own
+# (grid, structures, sources, monitors, dt, n_steps, dtypes, eps_r,
+# _b_e/_c_e/_b_h/_c_h, subpixel, particle_sources, progress_callback,
+# verbose; grid.shape/cell_size/coords/index_at), so the file can sit in the
+# reference package as its HIP driver beside _native.py.
 
 _AXIS = {"x": 0, "y": 1, "z": 2}
+# The stepping arrays whose precision must agree (simulation.py:274).
+_COMPUTE_KEYS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "eps_r", "psi")
+
+
+def nearest_node(grid, axis: int, position: float) -> int:
+    """Index of the node nearest ``position`` along ``axis``; the first wins
+    a tie (the argmin of ``rustbackend.py:350-352``, ``simulation.py:1013``)."""
+    return int(np.argmin(np.abs(np.asarray(grid.coords[axis]) - position)))
+
+
+@dataclass
+class AdeMedium:
+    """One dispersive material's cells and pole recursion coefficients.
+
+    ``P^{n+1} = a P^n + b P^{n-1} + c E^n`` with ``a = (2 - w0^2 dt^2)/(1+g dt)``,
+    ``b = (g dt - 1)/(1+g dt)``, ``c = eps0 S dt^2/(1+g dt)``; E is corrected by
+    ``-inv_eps * sum_p (P^{n+1} - P^n)``, ``inv_eps = 1/(eps0 eps_inf)``
+    (``simulation.py:572-669``).
+    """
+    cells: np.ndarray          # flat C-order indices, ascending
+    abc: np.ndarray            # (n_pole, 3) float64
+    inv_eps: float
+
+
+def ade_tables(sim) -> List[AdeMedium]:
+    """Dispersive regions after overlap resolution, in first-use order, from
+    ``sim.structures`` (the table half of ``_build_ade``, ``simulation.py:572-636``).
+
+    Structures are stamped in order; a dispersive medium is keyed by object
+    identity and a later non-dispersive structure releases the cells it
+    covers. Raises the reference's ``ValueError`` when a pole has
+    ``omega0 dt >= 2`` (``simulation.py:593-609``).
+    """
+    grid, dt = sim.grid, sim.dt
+    owner = np.full(grid.shape, -1, dtype=np.int32)
+    media: List = []
+    ids: Dict[int, int] = {}
+    for st in sim.structures:
+        med = getattr(st, "medium", None)
+        mask = st.region_mask(grid)
+        if getattr(med, "is_dispersive", False):
+            if id(med) not in ids:
+                ids[id(med)] = len(media)
+                media.append(med)
+            owner[mask] = ids[id(med)]
+        else:
+            owner[mask] = -1
+    if not media:
+        return []
+    worst = max(m.max_pole_omega() * dt for m in media)
+    if worst >= 2.0:
+        raise ValueError(
+            f"A dispersive medium has a pole with omega0*dt = {worst:.2f} >= 2, which makes "
+            "the explicit ADE update unstable at this resolution. Use a finer grid, drop "
+            "poles outside the band, or DispersiveMedium.at_wavelength() for a fixed-index "
+            "Medium.")
+    flat = owner.ravel()
+    out = []
+    for mid, med in enumerate(media):
+        cells = np.flatnonzero(flat == mid)
+        if cells.size == 0:
+            continue
+        rows = []
+        for p in med.poles:
+            den = 1.0 + p.gamma * dt
+            rows.append(((2.0 - p.omega0 ** 2 * dt ** 2) / den, (p.gamma * dt - 1.0) / den,
+                         (EPS_0 * p.strength * dt ** 2) / den))
+        out.append(AdeMedium(cells=cells.astype(np.int64),
+                             abc=np.asarray(rows, dtype=np.float64).reshape(-1, 3),
+                             inv_eps=1.0 / (EPS_0 * med.eps_inf)))
+    return out
 
 
 def validate(sim) -> np.dtype:
     """Feature gate; returns the single compute dtype."""
-    dts = {np.dtype(sim.dtypes[k]) for k in COMPUTE_KEYS}
+    dts = {np.dtype(sim.dtypes[k]) for k in _COMPUTE_KEYS}
     if dts not in ({np.dtype(np.float64)}, {np.dtype(np.float32)}):
         raise NotImplementedError("backend='hip' needs one uniform compute precision "
                                   "('float32' or 'float64'; monitor storage may differ).")
@@ -109,14 +189,14 @@ def build_tables(sim, npdt) -> Dict:
     t["wave"] = (np.ascontiguousarray(np.stack(rows)) if rows
                  else np.zeros((1, max(n_steps, 1)), dtype=np.float64))
 
-    t["ade"] = sim.ade_media()
+    t["ade"] = ade_tables(sim)
 
     dfts = []
     for m in sim.monitors:
         if not isinstance(m, DFTMonitor):
             continue
         pl = m.plane()
-        plane = None if pl is None else (pl[0], grid.nearest_index(pl[0], pl[1]))
+        plane = None if pl is None else (pl[0], nearest_node(grid, pl[0], pl[1]))
         omega = 2.0 * np.pi * np.asarray(m.freqs, dtype=np.float64)
         sample_steps = range(0, n_steps, m.interval)
         ph_e = np.empty((len(sample_steps), omega.size), dtype=np.complex128)
@@ -133,7 +213,7 @@ def build_tables(sim, npdt) -> Dict:
         if not isinstance(m, FluxMonitor):
             continue
         ax = _AXIS[m.plane_axis]
-        idx = 0 if grid.shape[ax] == 1 else grid.nearest_index(ax, m.plane_position)
+        idx = 0 if grid.shape[ax] == 1 else nearest_node(grid, ax, m.plane_position)
         ext = [grid.cell_size[a] if grid.shape[a] > 1 else 1.0 for a in range(3)]
         dA = float(np.prod([ext[a] for a in range(3) if a != ax]))
         fluxes.append(dict(mon=m, axis=ax, index=idx, dA=dA))
@@ -273,6 +353,16 @@ class HipEngine:
         self._cache.pop(comp, None)
         N.check(self.lib, self.lib.pf_write_field(self.h, N.COMP_CODE[comp], arr.ctypes.data),
                 "pf_write_field")
+
+    def info(self) -> Dict[str, int]:
+        """Stepping path, z-row padding and march tile counts (``pf_stepper_info``)."""
+        out = N.PfInfo()
+        N.check(self.lib, self.lib.pf_stepper_info(self.h, C.byref(out)), "pf_stepper_info")
+        return {name: int(getattr(out, name)) for name, _ in out._fields_ if name != "reserved"}
+
+    def dft_flush(self) -> None:
+        """Fold every buffered DFT sample into the accumulators (synchronous)."""
+        N.check(self.lib, self.lib.pf_dft_flush(self.h), "pf_dft_flush")
 
     def dft(self, name: str) -> Dict[str, np.ndarray]:
         mon, comps, count, n_freq = self.dft_slots[name]

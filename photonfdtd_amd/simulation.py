@@ -20,7 +20,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 import numpy as np
 
 from .boundaries import CPMLParams, cpml_coeffs_1axis
-from .constants import C_0, EPS_0
+from .constants import C_0
 from .grid import Grid
 from .sources import PointDipole
 
@@ -81,20 +81,6 @@ class Result:
     flux: Dict[str, float] = field(default_factory=dict)
     dft: Dict[str, Dict[str, np.ndarray]] = field(default_factory=dict)
     dft_freqs: Dict[str, np.ndarray] = field(default_factory=dict)
-
-
-@dataclass
-class AdeMedium:
-    """One dispersive material's cells and pole recursion coefficients.
-
-    ``P^{n+1} = a P^n + b P^{n-1} + c E^n`` with ``a = (2 - w0^2 dt^2)/(1+g dt)``,
-    ``b = (g dt - 1)/(1+g dt)``, ``c = eps0 S dt^2/(1+g dt)``; E is corrected by
-    ``-inv_eps * sum_p (P^{n+1} - P^n)``, ``inv_eps = 1/(eps0 eps_inf)``
-    (``simulation.py:572-669``).
-    """
-    cells: np.ndarray          # flat C-order indices, ascending
-    abc: np.ndarray            # (n_pole, 3) float64
-    inv_eps: float
 
 
 class Simulation:
@@ -192,51 +178,11 @@ class Simulation:
             self._b_h.append(bh)
             self._c_h.append(ch)
 
-    def ade_media(self) -> List[AdeMedium]:
-        """Dispersive regions after overlap resolution, in first-use order.
-
-        Media are keyed by object identity and a later non-dispersive structure
-        clears earlier dispersive cells (``simulation.py:577-590``). Raises the
-        reference's ``ValueError`` when a pole has ``omega0 dt >= 2``.
-        """
-        if not self._has_dispersion:
-            return []
-        owner = np.full(self.grid.shape, -1, dtype=np.int32)
-        media, ids = [], {}
-        for s in self.structures:
-            med = getattr(s, "medium", None)
-            mask = s.region_mask(self.grid)
-            if getattr(med, "is_dispersive", False):
-                if id(med) not in ids:
-                    ids[id(med)] = len(media)
-                    media.append(med)
-                owner[mask] = ids[id(med)]
-            else:
-                owner[mask] = -1
-        dt = self.dt
-        worst = max((m.max_pole_omega() * dt for m in media), default=0.0)
-        if worst >= 2.0:
-            raise ValueError(
-                f"A dispersive medium has a pole with omega0*dt = {worst:.2f} >= 2, which "
-                "makes the explicit ADE update unstable at this resolution. Use a finer "
-                "grid, drop poles outside the band, or DispersiveMedium.at_wavelength() "
-                "for a fixed-index Medium.")
-        out = []
-        flat_owner = owner.ravel()
-        for mid, med in enumerate(media):
-            cells = np.flatnonzero(flat_owner == mid)
-            if cells.size == 0:
-                continue
-            abc = []
-            for p in med.poles:
-                den = 1.0 + p.gamma * dt
-                abc.append(((2.0 - p.omega0 ** 2 * dt ** 2) / den,
-                            (p.gamma * dt - 1.0) / den,
-                            (EPS_0 * p.strength * dt ** 2) / den))
-            out.append(AdeMedium(cells=cells.astype(np.int64),
-                                 abc=np.asarray(abc, dtype=np.float64).reshape(-1, 3),
-                                 inv_eps=1.0 / (EPS_0 * med.eps_inf)))
-        return out
+    def ade_media(self):
+        """Dispersive regions after overlap resolution, in first-use order
+        (the HIP driver's ``ade_tables``, ``simulation.py:572-636``)."""
+        from .hip_backend import ade_tables
+        return ade_tables(self)
 
     # -- dispatch ------------------------------------------------------- #
     def run(self, out_of_core: bool = False, tile_cells: Optional[int] = None,

@@ -133,3 +133,37 @@ SCENES = {
 }
 # float32 trajectories pinned as well (the reference's NumPy float32 path)
 F32_SCENES = ("tm2d_box", "box3d", "metals_xz")
+
+
+def dispersive_slab(pf, backend, shape=(48, 72, 140), steps=300, precision="float64",
+                    pml=12, courant=0.9):
+    """The 3-D dispersive slab physics on a smaller grid: dx = 20 nm, CPML on
+    all six faces, silica cladding (eps 2.085), the silicon Sellmeier box
+    200 nm x 500 nm x 8 um (clipped by the domain in z), a 300 nm cube with
+    gold's Drude pole (Rakic row 0), an Ey point dipole at 1.55 um (10 fs)
+    and two x-plane DFT monitors (Ey, Ez, Hy, Hz) at 32 frequencies in
+    [c/1.65 um, c/1.45 um]. Positions scale per axis with the domain, as on
+    the 512^3 grid (source at -0.146 Lx, cube at +0.146 (Lx, Ly), planes at
+    +-0.2 Lx). Courant 0.9: the explicit ADE of silicon and gold grows without
+    bound at 0.99 on a 20 nm 3-D grid."""
+    dx = 20e-9
+    L = [n * dx for n in shape]
+    f = 1.5e-6 / (512 * dx)
+    gold_drude = pf.DispersiveMedium(eps_inf=1.0, poles=(pf.gold().poles[0],),
+                                     name="Au Drude term (Rakic row 0)")
+    structures = [
+        pf.Box(center=(0, 0, 0), size=tuple(3 * v for v in L), medium=pf.Medium(2.085)),
+        pf.Box(center=(0, 0, 0), size=(200e-9, 500e-9, 8e-6), medium=pf.silicon()),
+        pf.Box(center=(f * L[0], f * L[1], 0), size=(300e-9,) * 3, medium=gold_drude),
+    ]
+    src = pf.PointDipole(position=(-f * L[0], 0, 0), component="Ey",
+                         waveform=pf.GaussianPulse(freq0=pf.C_0 / 1.55e-6, fwhm=10e-15))
+    freqs = np.linspace(pf.C_0 / 1.65e-6, pf.C_0 / 1.45e-6, 32)
+    mons = [pf.DFTMonitor(name=nm, components=("Ey", "Ez", "Hy", "Hz"), freqs=freqs,
+                          plane_axis="x", plane_position=x)
+            for nm, x in (("in", -0.2 * L[0]), ("out", 0.2 * L[0]))]
+    grid = pf.Grid(size=tuple(L), cell_size=dx, pml_layers=(pml, pml, pml))
+    dt = courant / (pf.C_0 * np.sqrt(3.0 / dx ** 2))
+    return pf.Simulation(grid, structures=structures, sources=[src], monitors=mons,
+                         run_time=(steps - 0.5) * dt, courant=courant, precision=precision,
+                         backend=backend)

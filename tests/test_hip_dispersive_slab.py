@@ -1,0 +1,96 @@
+"""GPU parity of the 3-D dispersive slab scene (silicon Sellmeier box, gold
+Drude cube, silica cladding, Ey dipole, two 32-frequency x-plane DFT
+monitors, 12-cell CPML on six faces) against the float64 oracle.
+
+The grid (48 x 72 x 140) is sized so the march runs its psi-free kernel on
+padded rows: the z block of nodes [62, 124) lies inside the CPML-free range
+[12, 128), and so do x/y tiles, so the bulk tile list is non-empty and the
+silicon box crosses those tiles. Bars: float64 in the reference operation
+order within 1e-12 relative L2 of the oracle (fields and both DFT planes),
+float32 in the fast order within 1e-4; dense z rows (every tile on the CPML
+kernel) give the same bits as the padded split launch.
+"""
+import numpy as np
+import pytest
+
+from scenes import dispersive_slab
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def oracle_run():
+    import photonfdtd_amd as pf
+    from oracle.fdtd_numpy import run_oracle
+    print("\n[case] dispersive slab 48x72x140 oracle (float64, 300 steps)", flush=True)
+    return run_oracle(dispersive_slab(pf, "hip"), return_state=True)
+
+
+def _gpu(precision, exact, env, monkeypatch):
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.hip_backend import HipEngine
+    for k in ("PHOTONFDTD_HIP_ZPAD", "PHOTONFDTD_HIP_SPLIT", "PHOTONFDTD_HIP_PAIR"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    sim = dispersive_slab(pf, "hip", precision=precision)
+    eng = HipEngine(sim, np.float64 if precision == "float64" else np.float32, exact)
+    info = eng.info()
+    eng.advance(0, sim.n_steps)
+    out = {c: eng.get(c).astype(np.float64) for c in FIELDS}
+    dft = {nm: eng.dft(nm) for nm in ("in", "out")}
+    eng.close()
+    return info, out, dft
+
+
+def _worst(out, dft, ref, state):
+    w = 0.0
+    for c in FIELDS:
+        w = max(w, _rel(out[c], state[c]))
+    for nm in ("in", "out"):
+        for c, v in dft[nm].items():
+            w = max(w, _rel(v, ref["dft"][nm][c]))
+    return w
+
+
+def test_fp64_exact_psi_free_tiles_match_oracle(oracle_run, monkeypatch):
+    ref, state = oracle_run
+    print("[case] dispersive slab float64 exact, padded split march", flush=True)
+    info, out, dft = _gpu("float64", True, {}, monkeypatch)
+    assert info["path"] == 2 and info["zpad"] == 62, info
+    assert info["tiles_bulk"] > 0 and info["cells_bulk"] > 0, info
+    worst = _worst(out, dft, ref, state)
+    print(f"[fp64] dispersive slab worst rel L2 = {worst:.3e}; psi-free tiles "
+          f"{info['tiles_bulk']} ({info['cells_bulk']} cells), CPML tiles {info['tiles_cpml']}",
+          flush=True)
+    assert worst <= 1e-12
+    assert np.abs(state["Ey"]).max() > 0
+
+
+def test_fp32_fast_within_1e4_of_oracle(oracle_run, monkeypatch):
+    ref, state = oracle_run
+    print("[case] dispersive slab float32 fast, padded split march", flush=True)
+    info, out, dft = _gpu("float32", False, {}, monkeypatch)
+    assert info["tiles_bulk"] > 0, info
+    worst = _worst(out, dft, ref, state)
+    print(f"[fp32] dispersive slab worst rel L2 vs float64 oracle = {worst:.3e}", flush=True)
+    assert worst <= 1e-4
+
+
+@pytest.mark.parametrize("precision,exact", [("float64", True), ("float32", False)])
+def test_dense_rows_same_bits_as_padded_split(precision, exact, monkeypatch):
+    print(f"[case] dispersive slab padded vs dense rows {precision}", flush=True)
+    ia, fa, da = _gpu(precision, exact, {}, monkeypatch)
+    ib, fb, db = _gpu(precision, exact, {"PHOTONFDTD_HIP_ZPAD": "0"}, monkeypatch)
+    assert ia["tiles_bulk"] > 0 and ib["zpad"] == 0 and ib["tiles_bulk"] == 0, (ia, ib)
+    for c in FIELDS:
+        assert np.array_equal(fa[c], fb[c]), c
+    for nm in ("in", "out"):
+        for c in da[nm]:
+            assert np.array_equal(da[nm][c], db[nm][c]), (nm, c)

@@ -128,3 +128,34 @@ def test_pair_fast_order_bitwise(monkeypatch):
     b = _scene(pf, "float32").run()
     for c in a.fields["s"]:
         assert np.array_equal(a.fields["s"][c], b.fields["s"][c]), c
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_pair_single_steps_at_both_polarisation_levels(precision, monkeypatch):
+    """Uneven advances (1, 2, 1, 3, 2, 1, 4, 1 steps) put single steps after
+    both an even and an odd number of pairs, so the all-media ADE launch of a
+    single step runs at both polarisation levels of the two-step march (two
+    dispersive media: the Drude and the silicon box). Same bits as single
+    steps only."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.hip_backend import HipEngine
+    print(f"\n[case] pair single steps at both P levels {precision}", flush=True)
+    monkeypatch.setenv("PHOTONFDTD_HIP_EXACT", "1")
+    npdt = np.float32 if precision == "float32" else np.float64
+    out = []
+    for pair in ("1", "0"):
+        monkeypatch.setenv("PHOTONFDTD_HIP_PAIR", pair)
+        eng = HipEngine(_scene(pf, precision), npdt, True)
+        pos = 0
+        for n in (1, 2, 1, 3, 2, 1, 4, 1):
+            eng.advance(pos, n)
+            pos += n
+        out.append(({c: eng.get(c).copy() for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")},
+                    eng.dft("d")))
+        eng.close()
+    (fa, da), (fb, db) = out
+    for c in fa:
+        assert np.array_equal(fa[c], fb[c]), c
+    for c in da:
+        assert np.array_equal(da[c], db[c]), c
+    assert np.abs(fa["Ez"]).max() > 0
