@@ -22,6 +22,7 @@
 // j = 0..3, and receives D[4 (l / 16) + j][l % 16].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -134,13 +135,23 @@ struct WeightGradOp {   // M = cout, N = kdim, K = output pixels (split)
   const float* dy;
   const void* y;
   int relu, out_f32;
-  float* dw;
+  float* part;          // [split][cout][kdim]: the partial sum of each pixel split
   __device__ uint16_t a(int m, int r) const { return f2bf(dz(g, dy, y, relu, out_f32, r, m)); }
   __device__ uint16_t b(int r, int nn) const { return im2col(g, x, r, nn); }
   __device__ void store(int m, int nn, float acc) const {
-    atomicAdd(&dw[(size_t)m * g.kdim + nn], acc);
+    part[((size_t)blockIdx.z * g.cout + m) * g.kdim + nn] = acc;
   }
 };
+
+// dw[i] += sum over the splits of part[z][i], z = 0, 1, ... in order: the
+// weight gradient is the same bits on every run (no atomics).
+__global__ void k_reduce_splits(const float* part, int splits, int64_t count, float* dw) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = part[i];
+  for (int z = 1; z < splits; ++z) s += part[(size_t)z * count + i];
+  dw[i] += s;
+}
 
 constexpr int TM = 64, TN = 64, TK = 16, LDS_ROW = TK + 4;
 
@@ -240,12 +251,36 @@ __global__ void k_to_bf16(int64_t n, const float* x, uint16_t* y) {
   if (i < n) y[i] = f2bf(x[i]);
 }
 
+// Number of K ranges a GEMM of K = kk splits into: ceil(kk / kper) for
+// kper = ceil(kk / splits) rounded up to TK.
+inline int split_count(int kk, int splits) {
+  const int kper = ((kk + splits - 1) / splits + TK - 1) / TK * TK;
+  return (kk + kper - 1) / kper;
+}
+
 template <class Op>
 void launch_gemm(const Op& op, int M, int N, int K, int splits, hipStream_t s) {
   const int kper = ((K + splits - 1) / splits + TK - 1) / TK * TK;
   const int z = (K + kper - 1) / kper;
   dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, z);
   k_gemm<Op><<<grid, 256, 0, s>>>(op, M, N, K, kper);
+}
+
+// Largest weight-gradient workspace a conv backward may need from the
+// library's static scratch array, in floats (16 MiB); the pixel splits of a
+// layer are capped so that splits * cout * kdim stays within it.
+constexpr int64_t SCRATCH_FLOATS = int64_t(4) << 20;
+
+// Pixel splits of the weight-gradient GEMM of a layer: about 4096 output
+// pixels each, at most SCRATCH_FLOATS / (cout kdim) (at least 1). A function
+// of the shape only, so both backward entry points sum the same partials in
+// the same order.
+inline int weight_grad_splits(const Geom& g) {
+  const int64_t R = (int64_t)g.n * g.ho * g.wo;
+  const int64_t per = (int64_t)g.cout * g.kdim;
+  int64_t s = R > 4096 ? (R + 4095) / 4096 : 1;
+  s = std::min<int64_t>(s, std::max<int64_t>(1, SCRATCH_FLOATS / per));
+  return split_count((int)R, (int)s);
 }
 
 }  // namespace pfnn
@@ -288,23 +323,50 @@ int pf_nn_conv_forward(const pf_nn_conv* c, const void* x, const void* w, const 
   return nn_status("pf_nn_conv_forward");
 }
 
-int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void* y,
-                        const float* dy, float* dx, float* dw, float* dbias, void* stream) {
-  if (!valid(c) || !x || !w || !y || !dy) return nn_fail("pf_nn_conv_backward: bad arguments");
+}  // extern "C"
+
+namespace {
+
+// The backward kernels of one layer; `part` is the weight-gradient workspace
+// of weight_grad_splits(g) * cout * kdim floats (unused when dw is NULL).
+void conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void* y,
+                   const float* dy, float* dx, float* dw, float* dbias, float* part,
+                   hipStream_t s) {
   const pfnn::Geom g = pfnn::make_geom(*c);
-  hipStream_t s = static_cast<hipStream_t>(stream);
   if (dx) {
     pfnn::DataGradOp op{g, static_cast<const uint16_t*>(w), dy, y, c->relu, c->out_f32, dx};
     pfnn::launch_gemm(op, g.n * g.h * g.w, g.cin, g.k * g.k * g.cout, 1, s);
   }
   if (dw) {
-    pfnn::WeightGradOp op{g, static_cast<const uint16_t*>(x), dy, y, c->relu, c->out_f32, dw};
-    const int R = g.n * g.ho * g.wo;
-    const int splits = R > 4096 ? (R + 4095) / 4096 : 1;
-    pfnn::launch_gemm(op, g.cout, g.kdim, R, splits, s);
+    const int splits = pfnn::weight_grad_splits(g);
+    pfnn::WeightGradOp op{g, static_cast<const uint16_t*>(x), dy, y, c->relu, c->out_f32, part};
+    pfnn::launch_gemm(op, g.cout, g.kdim, g.n * g.ho * g.wo, splits, s);
+    const int64_t count = (int64_t)g.cout * g.kdim;
+    pfnn::k_reduce_splits<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(part, splits, count, dw);
   }
   if (dbias) pfnn::k_bias_grad<<<g.cout, 256, 0, s>>>(g, dy, y, c->relu, c->out_f32, dbias);
-  return nn_status("pf_nn_conv_backward");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void* y,
+                        const float* dy, float* dx, float* dw, float* dbias, void* stream) {
+  if (!valid(c) || !x || !w || !y || !dy) return nn_fail("pf_nn_conv_backward: bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float* part = nullptr;
+  if (dw) {
+    // stream-ordered workspace: allocated, used and released in order on s
+    const pfnn::Geom g = pfnn::make_geom(*c);
+    const size_t bytes = (size_t)pfnn::weight_grad_splits(g) * g.cout * g.kdim * sizeof(float);
+    if (hipMallocAsync(reinterpret_cast<void**>(&part), bytes, s) != hipSuccess)
+      return nn_status("pf_nn_conv_backward: workspace allocation");
+  }
+  conv_backward(c, x, w, y, dy, dx, dw, dbias, part, s);
+  const int st = nn_status("pf_nn_conv_backward");
+  if (part) (void)hipFreeAsync(part, s);
+  return st;
 }
 
 int pf_nn_mse(int32_t total, const float* pred, const float* target, float* dpred, float* loss,
