@@ -268,9 +268,23 @@ typedef struct pf_nn_conv {
 int pf_nn_conv_forward(const pf_nn_conv* c, const void* x, const void* w, const float* bias,
                        void* y, void* stream);
 /* dy: FP32 gradient w.r.t. the layer output y; dx (FP32, may be NULL) is
- * written; dw and dbias (FP32, may be NULL) are accumulated into. */
+ * written; dw and dbias (FP32, may be NULL) are accumulated into. The weight
+ * gradient sums fixed pixel splits in a fixed order (same bits every run);
+ * its partial sums use a workspace allocated and freed in order on the
+ * stream (hipMallocAsync / hipFreeAsync). */
 int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void* y,
                         const float* dy, float* dx, float* dw, float* dbias, void* stream);
+/* pf_nn_conv_backward without any allocation (no reference counterpart:
+ * the reference has no learned component). For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight training or
+ * serving loops. Same arguments, kernels and bits as pf_nn_conv_backward;
+ * the weight gradient's per-split partial sums go to a static 16 MiB device
+ * array per GPU instead of a stream-ordered workspace. PF_E_LIMIT when a
+ * layer needs more (cout * ksize^2 * cin above 4 Mi floats). Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads) race. The stream must belong to the current device. */
+int pf_nn_conv_backward_noalloc(const pf_nn_conv* c, const void* x, const void* w, const void* y,
+                                const float* dy, float* dx, float* dw, float* dbias, void* stream);
 /* loss (one FP32, zeroed by the caller) += mean (pred - target)^2; dpred written. */
 int pf_nn_mse(int32_t total, const float* pred, const float* target, float* dpred, float* loss,
               void* stream);

@@ -98,6 +98,8 @@ SIGNATURES = {
     "pf_batch_write_ce": (_i32, [_vp, _i32, _vp]),
     "pf_nn_conv_forward": (_i32, [_P(PfNnConv), _vp, _vp, _vp, _vp, _vp]),
     "pf_nn_conv_backward": (_i32, [_P(PfNnConv), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_nn_conv_backward_noalloc": (_i32, [_P(PfNnConv), _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           _vp]),
     "pf_nn_mse": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp]),
     "pf_nn_adam": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float,
                           C.c_float, _i32, _vp]),

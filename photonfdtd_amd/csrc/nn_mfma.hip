@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdint>
 #include <cstdio>
 
@@ -367,6 +368,69 @@ int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const
   const int st = nn_status("pf_nn_conv_backward");
   if (part) (void)hipFreeAsync(part, s);
   return st;
+}
+
+}  // extern "C"
+
+// Weight-gradient workspace of pf_nn_conv_backward_noalloc: one array per
+// GPU, loaded with the library's code object and held until the process
+// ends. In host code the name `scratch` is only a stand-in for the symbol;
+// its device address comes from hipGetSymbolAddress (scratch_on_device).
+static __device__ __align__(256) float scratch[pfnn::SCRATCH_FLOATS];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+float* scratch_dev[MAX_DEVICES] = {};   // device address of `scratch`, per device
+
+// The device address of this GPU's scratch array, looked up on the first
+// call made with that device current and kept per device.
+int scratch_on_device(float** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+    snprintf(nn_err, sizeof(nn_err), "pf_nn_conv_backward_noalloc: no usable current device");
+    return PF_E_HIP;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      snprintf(nn_err, sizeof(nn_err), "pf_nn_conv_backward_noalloc: hipGetSymbolAddress: %s",
+               hipGetErrorString(e));
+      return PF_E_HIP;
+    }
+    scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = scratch_dev[dev];
+  return PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_nn_conv_backward_noalloc(const pf_nn_conv* c, const void* x, const void* w, const void* y,
+                                const float* dy, float* dx, float* dw, float* dbias, void* stream) {
+  if (!valid(c) || !x || !w || !y || !dy)
+    return nn_fail("pf_nn_conv_backward_noalloc: bad arguments");
+  float* part = nullptr;
+  if (dw) {
+    const pfnn::Geom g = pfnn::make_geom(*c);
+    const int64_t need = (int64_t)pfnn::weight_grad_splits(g) * g.cout * g.kdim;
+    if (need > pfnn::SCRATCH_FLOATS) {
+      snprintf(nn_err, sizeof(nn_err),
+               "pf_nn_conv_backward_noalloc: the weight gradient needs %lld floats of "
+               "workspace, more than the %lld-float scratch array",
+               (long long)need, (long long)pfnn::SCRATCH_FLOATS);
+      return PF_E_LIMIT;
+    }
+    const int st = scratch_on_device(&part);
+    if (st != PF_OK) return st;
+  }
+  conv_backward(c, x, w, y, dy, dx, dw, dbias, part, static_cast<hipStream_t>(stream));
+  return nn_status("pf_nn_conv_backward_noalloc");
 }
 
 int pf_nn_mse(int32_t total, const float* pred, const float* target, float* dpred, float* loss,
