@@ -150,6 +150,14 @@ int pf_read_field(pf_handle* h, int32_t comp, void* dst);
 /* Strided sub-box read for FieldMonitor downsample / plane_z
  * (simulation.py:1062-1078); dst holds count[0]*count[1]*count[2] dtype. */
 int pf_read_box(pf_handle* h, int32_t comp, const pf_box* box, void* dst);
+/* A strided sub-box quantised on the device for compressed FieldMonitor
+ * storage; replaces the host quantisation of CompressedFieldSeries.append
+ * (storage.py:128-141). *scale receives max |x| over the box (1 when that is
+ * 0 or not finite); dst receives one value per node: bits 8 -> int8
+ * rint(x / scale * 127), bits 16 -> float16 bits of x / scale, computed in
+ * the field dtype (round half to even). */
+int pf_read_box_q(pf_handle* h, int32_t comp, const pf_box* box, int32_t bits, void* dst,
+                  double* scale);
 /* Overwrites one component from host memory (nx*ny*nz dtype). */
 int pf_write_field(pf_handle* h, int32_t comp, const void* src);
 /* DFT accumulator of monitor `mon`, component slot `slot` (index into its

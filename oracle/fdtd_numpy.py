@@ -240,8 +240,9 @@ def run_oracle(sim, return_state: bool = False):
     dft_acc, dft_w, dft_plane = {}, {}, {}
     for m in sim.monitors:
         if isinstance(m, FieldMonitor):
-            if m.compression is not None:
-                raise NotImplementedError("compressed FieldMonitor is not restated")
+            # A compressed monitor records the snapshots the reference hands to
+            # CompressedFieldSeries.append (simulation.py:1088-1097): the field
+            # values, uncompressed; the quantisation is checked on its own.
             steps = ({int(round(t / dt)) for t in m.times} if m.times is not None
                      else set(range(0, n_steps, m.interval)))
             fm_steps[m.name] = steps
@@ -323,7 +324,9 @@ def run_oracle(sim, return_state: bool = False):
                     for c in m.components:
                         snap = field_view(F[c], m.downsample, fm_z[m.name])
                         if fm_rows[m.name][c] is None:
-                            fm_rows[m.name][c] = np.empty((fm_n[m.name],) + snap.shape, dtype=mon_dt)
+                            fm_rows[m.name][c] = np.empty(
+                                (fm_n[m.name],) + snap.shape,
+                                dtype=snap.dtype if m.compression is not None else mon_dt)
                         fm_rows[m.name][c][row] = snap
                     fm_times[m.name].append(step * dt)
             elif isinstance(m, DFTMonitor):

@@ -76,6 +76,7 @@ SIGNATURES = {
     "pf_advance": (_i32, [_vp, _i64, _i64]),
     "pf_read_field": (_i32, [_vp, _i32, _vp]),
     "pf_read_box": (_i32, [_vp, _i32, _P(PfBox), _vp]),
+    "pf_read_box_q": (_i32, [_vp, _i32, _P(PfBox), _i32, _vp, _P(_f64)]),
     "pf_write_field": (_i32, [_vp, _i32, _vp]),
     "pf_dft_read": (_i32, [_vp, _i32, _i32, _P(_f64)]),
     "pf_dft_flush": (_i32, [_vp]),

@@ -240,6 +240,7 @@ struct pf_handle {
   virtual ~pf_handle() = default;
   virtual void advance(int64_t step0, int64_t n) = 0;
   virtual void read_box(int comp, const pf_box& box, void* dst) = 0;
+  virtual void read_box_q(int comp, const pf_box& box, int bits, void* dst, double* scale) = 0;
   virtual void write_field(int comp, const void* src) = 0;
   virtual void dft_read(int mon, int slot, double* dst) = 0;
   virtual void flux_read(double* dst) = 0;
@@ -278,6 +279,7 @@ class Stepper final : public pf_handle {
   }
   void advance(int64_t step0, int64_t n) override;
   void read_box(int comp, const pf_box& box, void* dst) override;
+  void read_box_q(int comp, const pf_box& box, int bits, void* dst, double* scale) override;
   void write_field(int comp, const void* src) override;
   void dft_read(int mon, int slot, double* dst) override;
   void dft_flush() override;
@@ -395,6 +397,7 @@ class Stepper final : public pf_handle {
   void capture_dft(int64_t step);
   void fold(Dft& m);
   real* staging(size_t elems);
+  const real* gather(int comp, const pf_box& box, int64_t* nbox);
   template <typename F>
   void timed(int cls, F&& launch);
   void collect_timing();
@@ -467,6 +470,9 @@ class Stepper final : public pf_handle {
   std::vector<Flux> flux_;
   real* stage_ = nullptr;
   size_t stage_elems_ = 0;
+  void* qbuf_ = nullptr;          // quantised snapshot (1 or 2 bytes per value)
+  size_t qbuf_bytes_ = 0;
+  double* qpart_ = nullptr;       // partial maxima + the scale of read_box_q
   int32_t* comp_index_dev_ = nullptr;  // {0, 1, 2, 3, 4, 5}
   // event timing
   struct Timed {
@@ -1873,13 +1879,14 @@ real* Stepper<real>::staging(size_t elems) {
   return stage_;
 }
 
+// Gathers one component over a strided box into the staging buffer (on the
+// handle's stream, not waited for); *nbox receives the box's node count.
 template <typename real>
-void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
+const real* Stepper<real>::gather(int comp, const pf_box& box, int64_t* nbox) {
   if (comp < 0 || comp > 5) throw ArgError{"component out of range"};
-  if (!dst) throw ArgError{"null destination"};
   const BoxDev b = check_box(box, shape);
-  const int64_t nbox = b.count[0] * b.count[1] * b.count[2];
-  real* out = staging(nbox);
+  *nbox = b.count[0] * b.count[1] * b.count[2];
+  real* out = staging(*nbox);
   if (ypad_) {
     // padded y rows of a 2-D batch member: gather (x, 1, y) with y as the padded axis
     BoxDev t = b;
@@ -1896,7 +1903,37 @@ void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
                                    (int64_t)args_.ny * zrow_, zrow_, zpad_, out, stream_);
   }
   HIP_CHECK(hipGetLastError());
+  return out;
+}
+
+template <typename real>
+void Stepper<real>::read_box(int comp, const pf_box& box, void* dst) {
+  if (!dst) throw ArgError{"null destination"};
+  int64_t nbox = 0;
+  const real* out = gather(comp, box, &nbox);
   HIP_CHECK(hipMemcpyAsync(dst, out, nbox * sizeof(real), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// A snapshot quantised on the device for compressed storage
+// (storage.py:128-141): bits 8 -> int8, bits 16 -> float16, one per node.
+template <typename real>
+void Stepper<real>::read_box_q(int comp, const pf_box& box, int bits, void* dst, double* scale) {
+  if (!dst || !scale) throw ArgError{"null destination"};
+  if (bits != 8 && bits != 16) throw ArgError{"bits must be 8 or 16"};
+  int64_t nbox = 0;
+  const real* x = gather(comp, box, &nbox);
+  constexpr int kMaxPart = 1024;
+  if (!qpart_) qpart_ = static_cast<double*>(arena_.alloc((kMaxPart + 1) * sizeof(double)));
+  const size_t bytes = (size_t)nbox * (bits / 8);
+  if (bytes > qbuf_bytes_) {
+    qbuf_ = arena_.alloc(bytes);
+    qbuf_bytes_ = bytes;
+  }
+  pfhip::launch_quantize<real>(x, nbox, bits, qpart_, kMaxPart, qbuf_, qpart_ + kMaxPart, stream_);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(dst, qbuf_, bytes, hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipMemcpyAsync(scale, qpart_ + kMaxPart, sizeof(double), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
@@ -2473,6 +2510,12 @@ int pf_read_field(pf_handle* h, int32_t comp, void* dst) {
 int pf_read_box(pf_handle* h, int32_t comp, const pf_box* box, void* dst) {
   if (!h || !box) return fail(PF_E_ARG, "null argument");
   return guarded([&] { h->read_box(comp, *box, dst); });
+}
+
+int pf_read_box_q(pf_handle* h, int32_t comp, const pf_box* box, int32_t bits, void* dst,
+                  double* scale) {
+  if (!h || !box) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { h->read_box_q(comp, *box, bits, dst, scale); });
 }
 
 int pf_write_field(pf_handle* h, int32_t comp, const void* src) {

@@ -444,6 +444,13 @@ void launch_batch_persistent(const BatchArgs<real>& a, bool exact, int waves_per
 template <typename real>
 hipError_t batch_persistent_capacity(bool exact, int waves_per_block, int* per_cu, int* n_cu);
 
+// Snapshot quantisation for compressed FieldMonitor storage: scale = max |x|
+// (1 when 0 or not finite) into *scale, x / scale * 127 rounded half to even
+// into int8 (bits 8) or x / scale rounded to float16 bits (bits 16). part
+// holds max_part doubles of partial maxima.
+template <typename real>
+void launch_quantize(const real* x, int64_t n, int bits, double* part, int max_part, void* out,
+                     double* scale, hipStream_t s);
 template <typename real> void launch_generic_h(const StepArgs<real>& a, bool exact, hipStream_t s);
 template <typename real> void launch_generic_e(const StepArgs<real>& a, bool exact, hipStream_t s);
 template <typename real>

@@ -25,6 +25,7 @@ from . import _native as N
 from .constants import EPS_0, MU_0
 from .monitors import DFTMonitor, FieldMonitor, FluxMonitor
 from .simulation import Result
+from .storage import CompressedFieldSeries, get_codec
 
 # Every name this module takes from its package (constants, monitors,
 # Simulation.Result) exists under the same name in the reference package, and
@@ -120,8 +121,7 @@ def validate(sim) -> np.dtype:
         raise NotImplementedError("ChargedParticle sources are not supported on backend='hip'.")
     for m in sim.monitors:
         if isinstance(m, FieldMonitor) and m.compression is not None:
-            raise NotImplementedError("FieldMonitor(compression=) is a host-side store not "
-                                      "wired to backend='hip'; drop compression.")
+            get_codec(m.compression)       # ImportError / ValueError before any device work
     for s in sim.sources:
         if s.component not in N.COMP_CODE:
             raise ValueError(f"unknown source component {s.component!r}")
@@ -348,6 +348,17 @@ class HipEngine:
                                                out.ctypes.data), "pf_read_box")
         return out
 
+    def read_box_q(self, comp: str, start, count, stride, bits: int):
+        """A box quantised on the device for compressed storage: (int8 or
+        float16 array, scale) as ``CompressedFieldSeries`` stores a frame."""
+        box = N.PfBox()
+        box.start[:], box.count[:], box.stride[:] = start, count, stride
+        out = np.empty(tuple(count), dtype=np.int8 if bits == 8 else np.float16)
+        scale = C.c_double()
+        N.check(self.lib, self.lib.pf_read_box_q(self.h, N.COMP_CODE[comp], C.byref(box), int(bits),
+                                                 out.ctypes.data, C.byref(scale)), "pf_read_box_q")
+        return out, float(scale.value)
+
     def set(self, comp: str, values: np.ndarray) -> None:
         arr = np.ascontiguousarray(values, dtype=self.npdt).reshape(self.shape)
         self._cache.pop(comp, None)
@@ -436,15 +447,30 @@ def run_loop(sim, engine: HipEngine) -> Result:
             zplane = (None if m.plane_z is None else
                       int(np.argmin(np.abs(np.asarray(grid.coords[2]) - m.plane_z))))
             rec[m.name] = dict(mon=m, steps=steps, n=len(steps), rows={c: None for c in m.components},
-                               times=[], box=_snap_box(grid.shape, m.downsample, zplane=zplane))
+                               times=[], box=_snap_box(grid.shape, m.downsample, zplane=zplane),
+                               codec=None)
         elif isinstance(m, DFTMonitor):
             result.dft_freqs[m.name] = np.asarray(m.freqs, dtype=np.float64)
 
     def record(r, step):
         row = len(r["times"])
         start, count, stride = r["box"]
+        m = r["mon"]
+        if m.compression is not None:
+            # quantised on the device; the series runs the byte codec
+            # (storage.py:128-141 on the reference's host)
+            if r["codec"] is None:
+                r["codec"] = get_codec(m.compression)
+            for c in m.components:
+                q, scale = engine.read_box_q(c, start, count, stride, m.compression_bits)
+                if r["rows"][c] is None:
+                    r["rows"][c] = CompressedFieldSeries(q.shape, mon_dtype, r["codec"],
+                                                         bits=m.compression_bits)
+                r["rows"][c].append_quantized(q, scale)
+            r["times"].append(step * dt)
+            return
         full = count == list(grid.shape) and stride == [1, 1, 1]
-        for c in r["mon"].components:
+        for c in m.components:
             snap = engine.get(c) if full else engine.read_box(c, start, count, stride)
             if r["rows"][c] is None:
                 r["rows"][c] = np.empty((r["n"],) + snap.shape, dtype=mon_dtype)
@@ -477,6 +503,9 @@ def run_loop(sim, engine: HipEngine) -> Result:
         if isinstance(m, FieldMonitor):
             r = rec[m.name]
             if r["times"]:
+                if m.compression is not None:
+                    for series in r["rows"].values():
+                        series.finalize()
                 result.fields[m.name] = dict(r["rows"])
                 result.monitor_times[m.name] = np.array(r["times"])
             else:

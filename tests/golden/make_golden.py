@@ -1,7 +1,8 @@
 """Write the golden fixtures from the reference's own NumPy float64 path.
 
 Run in a container that holds the reference checkout (read-only, never
-copied): ``python tests/golden/make_golden.py [/path/to/reference]``.
+copied): ``python tests/golden/make_golden.py [/path/to/reference]
+[--only=scene,scene]``.
 For every scene in ``scenes.py`` this imports ``photonfdtd`` from the
 reference's ``src/`` (no bytecode written), runs it with ``backend="numpy"``
 and stores inputs-derived tables and outputs as ``<scene>.npz`` (float64) and
@@ -17,7 +18,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE))
-from scenes import F32_SCENES, SCENES  # noqa: E402
+from scenes import F32_SCENES, FEATURE_SCENES, SCENES  # noqa: E402
 
 
 def dump(sim, res):
@@ -36,6 +37,8 @@ def dump(sim, res):
         out[f"mt/{name}"] = np.asarray(res.monitor_times[name])
         for c, v in comps.items():
             out[f"field/{name}/{c}"] = np.asarray(v)
+            if type(v).__name__ == "CompressedFieldSeries":
+                out[f"zbytes/{name}/{c}"] = np.int64(v.nbytes)
     for name, comps in res.dft.items():
         out[f"dftf/{name}"] = np.asarray(res.dft_freqs[name])
         for c, v in comps.items():
@@ -45,11 +48,13 @@ def dump(sim, res):
     return out
 
 
-def main(ref_root):
+def main(ref_root, only=None):
     sys.dont_write_bytecode = True
     sys.path.insert(0, str(Path(ref_root) / "src"))
     import photonfdtd as ref
-    for key, builder in SCENES.items():
+    for key, builder in {**SCENES, **FEATURE_SCENES}.items():
+        if only and key not in only:
+            continue
         precs = ["float64"] + (["float32"] if key in F32_SCENES else [])
         for prec in precs:
             sim = builder(ref, "numpy", precision=prec)
@@ -60,4 +65,8 @@ def main(ref_root):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else os.environ.get("PHOTONFDTD_REF", "/root/reference"))
+    # python make_golden.py [reference root] [--only scene,scene]
+    args = [a for a in sys.argv[1:] if not a.startswith("--only")]
+    only = [a.split("=", 1)[1].split(",") for a in sys.argv[1:] if a.startswith("--only=")]
+    main(args[0] if args else os.environ.get("PHOTONFDTD_REF", "/root/reference"),
+         only[0] if only else None)

@@ -122,6 +122,21 @@ def flux2d(pf, backend, precision="float64"):
                          backend=backend, precision=precision)
 
 
+def compressed2d(pf, backend, precision="float64"):
+    """2-D TM dipole beside a box with compressed FieldMonitors: int8 frames
+    (zlib) every 15 steps and float16 frames of every 2nd node at set times."""
+    grid = pf.Grid(size=(2.0e-6, 1.6e-6), cell_size=50e-9, pml_layers=(8, 8, 0))
+    box = pf.Box(center=(0.3e-6, 0.0), size=(0.6e-6, 0.5e-6), medium=pf.Medium.from_index(2.0))
+    src = pf.PointDipole(position=(-0.4e-6, 0.05e-6), component="Ez",
+                         waveform=pf.GaussianPulse(freq0=_f0(pf), fwhm=6e-15))
+    m8 = pf.FieldMonitor(name="q8", components=("Ez", "Hy"), interval=15, compression="zlib",
+                         compression_bits=8)
+    m16 = pf.FieldMonitor(name="q16", components=("Ez", "Hx"), times=[8e-15, 16e-15, 24e-15],
+                          downsample=2, compression="zlib", compression_bits=16)
+    return pf.Simulation(grid, structures=[box], sources=[src], monitors=[m8, m16],
+                         run_time=25e-15, backend=backend, precision=precision)
+
+
 SCENES = {
     "tm2d_box": tm2d_box,
     "te2d_poly": te2d_poly,
@@ -133,6 +148,9 @@ SCENES = {
 }
 # float32 trajectories pinned as well (the reference's NumPy float32 path)
 F32_SCENES = ("tm2d_box", "box3d", "metals_xz")
+# Scenes whose fixtures pin one feature (not full-trajectory oracle scenes):
+# compressed FieldMonitor frames.
+FEATURE_SCENES = {"compressed2d": compressed2d}
 
 
 def dispersive_slab(pf, backend, shape=(48, 72, 140), steps=300, precision="float64",

@@ -27,6 +27,7 @@ REFERENCE_NAMES = {
     "constants": {"C_0", "EPS_0", "MU_0"},                        # constants.py:3-8
     "monitors": {"FieldMonitor", "DFTMonitor", "FluxMonitor"},   # monitors.py:18,95,177
     "simulation": {"Simulation", "Result", "_COMPUTE_KEYS"},     # simulation.py:274,327,361
+    "storage": {"CompressedFieldSeries", "get_codec"},           # storage.py:80,104
 }
 # Modules shipped with the driver itself (copied beside it).
 SHIPPED = {"_native"}

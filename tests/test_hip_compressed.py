@@ -1,0 +1,72 @@
+"""GPU: compressed FieldMonitor frames quantised on the device.
+
+The HIP run of ``compressed2d`` (float64, reference operation order) must
+reproduce the frames and compressed byte counts the reference's NumPy path
+wrote. Crafted snapshots then pin the device rounding against NumPy's own
+expressions (the reference's storage.py:128-141 in the snapshot dtype):
+int8 ties of x / scale * 127 (round half to even) and float16 ties and
+underflow, including doubles whose float16 rounding differs from rounding
+through float32 first."""
+import numpy as np
+import pytest
+
+import golden_io as G
+from scenes import compressed2d
+
+pytestmark = pytest.mark.gpu
+
+
+def test_frames_match_reference_fixture():
+    import photonfdtd_amd as pf
+    print("\n[case] compressed2d float64 HIP vs reference frames", flush=True)
+    res = compressed2d(pf, "hip").run()
+    gold = G.load("compressed2d")
+    for name in ("q8", "q16"):
+        np.testing.assert_array_equal(res.monitor_times[name], gold[f"mt/{name}"])
+        for c, series in res.fields[name].items():
+            assert np.array_equal(np.asarray(series), gold[f"field/{name}/{c}"]), (name, c)
+            assert series.nbytes == int(gold[f"zbytes/{name}/{c}"]), (name, c)
+
+
+def _crafted(npdt, rng):
+    """A 4x8x64 field: random values, exact int8 ties, float16 ties and
+    subnormal-range values, scale 1 (the field holds +-1)."""
+    x = rng.standard_normal((4, 8, 64)) * 0.3
+    x[0, 0, 0] = 1.0
+    x[0, 0, 1] = -1.0
+    x[1, 0, :16] = (np.arange(16) - 8 + 0.5) / 127.0          # int8 ties after * 127
+    h = np.arange(1, 33, dtype=np.uint16).view(np.float16).astype(np.float64)   # subnormals
+    x[1, 1, :32] = h
+    x[1, 2, :32] = h + 2.0 ** -25                                # halfway between subnormals
+    base = np.linspace(0.01, 0.9, 32).astype(np.float16).astype(np.float64)
+    ulp = np.spacing(base.astype(np.float16)).astype(np.float64)
+    x[2, 0, :32] = base + ulp / 2                                # float16 ties
+    x[2, 1, :32] = base + ulp / 2 + ulp * 2.0 ** -30             # just above the tie
+    x[2, 2, :32] = -(base + ulp / 2)
+    return x.astype(npdt)
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_device_rounding_matches_numpy(precision):
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.hip_backend import HipEngine
+    print(f"\n[case] device quantisation rounding {precision}", flush=True)
+    npdt = np.float64 if precision == "float64" else np.float32
+    grid = pf.Grid(size=(4 * 50e-9, 8 * 50e-9, 64 * 50e-9), cell_size=50e-9, pml_layers=(0, 0, 0))
+    sim = pf.Simulation(grid, run_time=1e-16, precision=precision, backend="hip")
+    eng = HipEngine(sim, npdt, True)
+    x = _crafted(npdt, np.random.default_rng(11))
+    eng.set("Ex", x)
+    box = ([0, 0, 0], [4, 8, 64], [1, 1, 1])
+    q8, s8 = eng.read_box_q("Ex", *box, 8)
+    q16, s16 = eng.read_box_q("Ex", *box, 16)
+    sub = ([1, 0, 3], [2, 4, 20], [2, 2, 3])                    # a strided box
+    qs, ss = eng.read_box_q("Ex", *sub, 8)
+    eng.close()
+    assert s8 == s16 == 1.0
+    np.testing.assert_array_equal(q8, np.rint(x / s8 * 127.0).astype(np.int8))
+    np.testing.assert_array_equal(q16.view(np.uint16), (x / s16).astype(np.float16).view(np.uint16))
+    xs = x[1::2, 0:8:2, 3::3][:2, :4, :20]
+    scale = float(np.abs(xs).max())
+    assert ss == scale
+    np.testing.assert_array_equal(qs, np.rint(xs / scale * 127.0).astype(np.int8))

@@ -150,7 +150,10 @@ def test_backend_gate():
     sim = pf.Simulation(g, run_time=1e-15, precision={"default": "float32", "Ez": "float64"})
     with pytest.raises(NotImplementedError):
         sim.run()
-    sim = pf.Simulation(g, run_time=1e-15,
-                        monitors=[pf.FieldMonitor("f", compression="zlib")])
-    with pytest.raises(NotImplementedError):
-        sim.run()
+    # compression codecs resolve before any device work (storage.py:80-101)
+    from photonfdtd_amd import storage
+    if storage._zstd is None:
+        sim = pf.Simulation(g, run_time=1e-15,
+                            monitors=[pf.FieldMonitor("f", compression="zstd")])
+        with pytest.raises(ImportError):
+            sim.run()
