@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 6
+#define PFHIP_ABI_VERSION 7
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -110,6 +110,12 @@ typedef struct pf_desc {
   int32_t n_flux;
   const pf_flux_desc* flux;
   int32_t flags;              /* PF_FLAG_* */
+  /* Subpixel smoothing (simulation.py:956-969): per-component E
+   * coefficients dt/(eps_yy eps0) and dt/(eps_zz eps0) for Ey and Ez, in the
+   * dtype; `ce` is then the Ex coefficient. NULL (both): ce for all three.
+   * Such a handle steps with the generic two-pass kernels. */
+  const void* ce_y;
+  const void* ce_z;
 } pf_desc;
 
 /* pf_desc.flags: use the in-place two-pass kernels only (no fused ping-pong

@@ -111,6 +111,37 @@ def _ade_tables(sim, dt):
     return out
 
 
+def _smoothed_eps(grid, structures, factor):
+    """Subpixel-smoothed ``(eps_xx, eps_yy, eps_zz)`` (``smoothing.py:119-172``),
+    restated over the whole refined domain at once: each native cell's
+    arithmetic / harmonic mean of the permittivity stamped on its ``factor``
+    sub-nodes per active axis, and ``eps_ii = eps_a + (eps_h - eps_a) n_i^2``
+    with n the gradient of eps_a times its reciprocal norm (0 where the norm
+    is below 1e-9 of its maximum)."""
+    from types import SimpleNamespace
+    shape = grid.shape
+    f = [factor if n > 1 else 1 for n in shape]
+    coords = []
+    for a in range(3):
+        n = shape[a] * f[a]
+        d = grid.cell_size[a] / factor if shape[a] > 1 else 0.0
+        coords.append((np.arange(n) - (n - 1) / 2) * d if n > 1 else np.zeros(1))
+    view = SimpleNamespace(shape=tuple(c.size for c in coords), coords=tuple(coords))
+    eps = np.ones(view.shape)
+    for st in structures:
+        eps[st.region_mask(view)] = st.medium.eps_r
+    blocks = eps.reshape(shape[0], f[0], shape[1], f[1], shape[2], f[2])
+    arith = blocks.mean(axis=(1, 3, 5))
+    harm = 1.0 / (1.0 / blocks).mean(axis=(1, 3, 5))
+    g = [np.gradient(arith, grid.cell_size[a], axis=a) if shape[a] > 1 else np.zeros(shape)
+         for a in range(3)]
+    mag = np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
+    ok = mag > max(mag.max() * 1e-9, np.finfo(np.float64).tiny)
+    inv = np.where(ok, 1.0 / np.where(ok, mag, 1.0), 0.0)
+    unit = [gi * inv for gi in g]
+    return tuple(arith + (harm - arith) * u ** 2 for u in unit)
+
+
 def run_oracle(sim, return_state: bool = False):
     """Time-step ``sim`` (a ``photonfdtd_amd.Simulation``) on the CPU.
 
@@ -135,6 +166,13 @@ def run_oracle(sim, return_state: bool = False):
     for s in sim.structures:
         s.stamp(grid, eps)
     ce = (dt / (eps * EPS0)).astype(dts["eps_r"], copy=False)
+    ce_of = {"Ex": ce, "Ey": ce, "Ez": ce}
+    if getattr(sim, "subpixel", False):
+        # per-component coefficients from the smoothed diagonal tensor
+        # (simulation.py:956-969)
+        comps = _smoothed_eps(grid, sim.structures, sim.subpixel_factor)
+        for c, e in zip(("Ex", "Ey", "Ez"), comps):
+            ce_of[c] = (dt / (e.astype(dts["eps_r"]) * EPS0)).astype(dts["eps_r"], copy=False)
     ch = dts["Hx"](dt / MU0)
 
     be, cE, bh, cH = [], [], [], []
@@ -181,7 +219,7 @@ def run_oracle(sim, return_state: bool = False):
                               ccoef[s0:s1].astype(psi_dt).reshape(rs),
                               np.zeros(pshape, dtype=psi_dt)))
             terms.append((ax, src, other, sign, slabs))
-        plans[is_e].append((tgt, reg, ce[reg] if is_e else None, terms))
+        plans[is_e].append((tgt, reg, ce_of[tgt][reg] if is_e else None, terms))
 
     scratch = np.empty(shape, dtype=dts["Ex"])
 

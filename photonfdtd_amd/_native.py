@@ -11,7 +11,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 PF_FLAG_GENERIC = 1
 
 PF_F32, PF_F64 = 0, 1
@@ -50,7 +50,7 @@ class PfDesc(C.Structure):
                 ("src_amp", _P(_f64)), ("src_wave", _P(_i32)), ("n_wave", _i32),
                 ("wave", _P(_f64)), ("n_ade", _i32), ("ade", _P(PfAdeDesc)), ("n_dft", _i32),
                 ("dft", _P(PfDftDesc)), ("n_flux", _i32), ("flux", _P(PfFluxDesc)),
-                ("flags", _i32)]
+                ("flags", _i32), ("ce_y", _vp), ("ce_z", _vp)]
 
 
 class PfNnConv(C.Structure):

@@ -163,6 +163,9 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
   const real hz_x = Hz[ixm], hy_x = Hy[ixm], hx_y = Hx[iym];
   const real ex = a.f[0][id], ey = a.f[1][id], ez = a.f[2][id];
   const real K = a.ce[id];
+  // subpixel smoothing: each E component has its own coefficient
+  const real Ky = a.ce_y ? a.ce_y[id] : K;
+  const real Kz = a.ce_z ? a.ce_z[id] : K;
   real nex = ex, ney = ey, nez = ez;
   bool ux = false, uy = false, uz = false;
   const int mx = nx > 1 ? a.map_e[0][i] : -1;
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
     if (A.on) { A.d = diff<real, EXACT>(hx, hx_z, a.d[2], a.inv_d[2]); pml_term(A, a, true, 2, PSI_EYZ, i, j, k, mz); }
     if (B.on) { B.d = diff<real, EXACT>(hz, hz_x, a.d[0], a.inv_d[0]); pml_term(B, a, true, 0, PSI_EYX, i, j, k, mx); }
     if (A.on || B.on) {
-      ney = apply_terms_v(ey, K, A, B);
+      ney = apply_terms_v(ey, Ky, A, B);
       uy = true;
     }
   }
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(256) void k_update_e(StepArgs<real> a) {
     if (A.on) { A.d = diff<real, EXACT>(hy, hy_x, a.d[0], a.inv_d[0]); pml_term(A, a, true, 0, PSI_EZX, i, j, k, mx); }
     if (B.on) { B.d = diff<real, EXACT>(hx, hx_y, a.d[1], a.inv_d[1]); pml_term(B, a, true, 1, PSI_EZY, i, j, k, my); }
     if (A.on || B.on) {
-      nez = apply_terms_v(ez, K, A, B);
+      nez = apply_terms_v(ez, Kz, A, B);
       uz = true;
     }
   }

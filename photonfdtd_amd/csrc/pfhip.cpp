@@ -529,6 +529,12 @@ Stepper<real>::Stepper(const pf_desc& d) {
   lvl_[0] = arena_.zeros<real>(6 * (size_t)ncell, stream_);
   for (int c = 0; c < 6; ++c) args_.f[c] = lvl_[0] + (size_t)c * ncell;
   args_.ce = arena_.upload(static_cast<const real*>(d.ce), ncell, stream_);
+  if ((d.ce_y == nullptr) != (d.ce_z == nullptr))
+    throw ArgError{"ce_y and ce_z are given together (subpixel smoothing) or not at all"};
+  if (d.ce_y) {
+    args_.ce_y = arena_.upload(static_cast<const real*>(d.ce_y), ncell, stream_);
+    args_.ce_z = arena_.upload(static_cast<const real*>(d.ce_z), ncell, stream_);
+  }
 
   // CPML tables: coefficients rounded to the field dtype, in one table laid
   // out [axis][b_e, c_e, b_h, c_h][node]; compact maps per axis.
@@ -738,7 +744,9 @@ Stepper<real>::Stepper(const pf_desc& d) {
   // Fused ping-pong step: 3-D grids without H-side sources (an H source
   // must land between the H and E halves of the step).
   const char* env_fused = std::getenv("PHOTONFDTD_HIP_FUSED");
-  const bool want_fused = !(env_fused && env_fused[0] == '0') && !(d.flags & PF_FLAG_GENERIC);
+  // (per-component coefficients of subpixel smoothing run on the generic kernels)
+  const bool want_fused = !(env_fused && env_fused[0] == '0') && !(d.flags & PF_FLAG_GENERIC) &&
+                          d.ce_y == nullptr;
   if (want_fused && pfhip::fused_supported<real>(args_) && src_h_.n_groups == 0) {
     fused_ = true;
     const char* env_var = std::getenv("PHOTONFDTD_HIP_VARIANT");
@@ -1647,6 +1655,7 @@ void Stepper<real>::reset(hipStream_t s) {
 template <typename real>
 void Stepper<real>::checkpoint_enable(int interval) {
   if (fused_) throw ArgError{"checkpoints need the generic kernels: create with PF_FLAG_GENERIC"};
+  if (args_.ce_y) throw ArgError{"the adjoint does not support subpixel smoothing"};
   if (interval < 1) throw ArgError{"checkpoint interval must be positive"};
   if (!ade_.empty()) throw ArgError{"the adjoint does not support dispersive media"};
   ckpt_interval_ = interval;
@@ -2574,6 +2583,8 @@ int pf_batch_create(const pf_desc* descs, const int32_t* pols, int32_t n, pf_bat
   return guarded([&] {
     for (int i = 1; i < n; ++i)
       if (descs[i].dtype != descs[0].dtype) throw ArgError{"a batch needs one dtype"};
+    for (int i = 0; i < n; ++i)
+      if (descs[i].ce_y || descs[i].ce_z) throw ArgError{"subpixel smoothing is not supported in a batch"};
     int n3 = 0;   // 3-D grids (nz >= 2)
     for (int i = 0; i < n; ++i) n3 += descs[i].shape[2] > 1;
     if (n3 != 0 && n3 != n) throw ArgError{"a batch holds 2-D grids only or 3-D grids only"};

@@ -17,7 +17,9 @@ enum PsiSlot {
 template <typename real>
 struct StepArgs {
   real* f[6];            // Ex, Ey, Ez, Hx, Hy, Hz
-  const real* ce;        // per-node E coefficient
+  const real* ce;        // per-node E coefficient (of Ex when ce_y / ce_z are set)
+  const real* ce_y;      // subpixel smoothing: Ey / Ez coefficients, or null
+  const real* ce_z;
   real* psi[PSI_COUNT];  // compact CPML state
   // Out-of-place update (null: in place). The H pass reads E and H from f
   // and writes H to fo[3..5]; the E pass reads the new H from fo[3..5] and E

@@ -115,8 +115,6 @@ def validate(sim) -> np.dtype:
     if dts not in ({np.dtype(np.float64)}, {np.dtype(np.float32)}):
         raise NotImplementedError("backend='hip' needs one uniform compute precision "
                                   "('float32' or 'float64'; monitor storage may differ).")
-    if sim.subpixel:
-        raise NotImplementedError("subpixel=True is not supported on backend='hip'.")
     if sim.particle_sources:
         raise NotImplementedError("ChargedParticle sources are not supported on backend='hip'.")
     for m in sim.monitors:
@@ -162,7 +160,15 @@ def build_tables(sim, npdt) -> Dict:
     n_steps = sim.n_steps
     nx, ny, nz = grid.shape
     t: Dict = {}
-    t["ce"] = ce_of(sim, sim.eps_r, npdt)
+    if sim.subpixel:
+        # one coefficient per E component from the smoothed diagonal tensor
+        # (simulation.py:956-969); the generic kernels run such a handle
+        exx, eyy, ezz = sim._smoothed_eps_components()
+        t["ce"] = ce_of(sim, exx, npdt)
+        t["ce_y"] = ce_of(sim, eyy, npdt)
+        t["ce_z"] = ce_of(sim, ezz, npdt)
+    else:
+        t["ce"] = ce_of(sim, sim.eps_r, npdt)
     t["cell"] = tuple(d if d > 0 else 1.0 for d in grid.cell_size)
     t["cpml"] = [np.ascontiguousarray(a, dtype=np.float64)
                  for arrs in (sim._b_e, sim._c_e, sim._b_h, sim._c_h) for a in arrs]
@@ -235,6 +241,9 @@ def build_desc(sim, npdt, exact: bool, flags: int = 0):
     d.cell[:] = t["cell"]
     d.ch = sim.dt / MU_0
     d.ce = t["ce"].ctypes.data
+    if "ce_y" in t:
+        d.ce_y = t["ce_y"].ctypes.data
+        d.ce_z = t["ce_z"].ctypes.data
     cp = t["cpml"]
     for a in range(3):
         d.b_e[a] = N.as_ptr(cp[a], C.c_double)

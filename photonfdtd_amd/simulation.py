@@ -166,6 +166,17 @@ class Simulation:
         self._eps_r = value
         self._eps_r_custom = True
 
+    def _smoothed_eps_components(self):
+        """``(eps_xx, eps_yy, eps_zz)`` of subpixel smoothing; a custom scalar
+        ``eps_r`` has no geometry to supersample and is used for all three
+        (``simulation.py:546-559``)."""
+        from .smoothing import smooth_permittivity
+        if self._eps_r_custom:
+            eps = self.eps_r
+            return (eps, eps, eps)
+        return smooth_permittivity(self.grid, self.structures, background_eps=1.0,
+                                   factor=self.subpixel_factor, dtype=self.dtypes["eps_r"])
+
     def _build_cpml(self) -> None:
         self._b_e, self._c_e, self._b_h, self._c_h = [], [], [], []
         for ax in range(3):

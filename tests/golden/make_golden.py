@@ -24,6 +24,9 @@ from scenes import F32_SCENES, FEATURE_SCENES, SCENES  # noqa: E402
 def dump(sim, res):
     out = {"dt": np.float64(sim.dt), "n_steps": np.int64(sim.n_steps),
            "eps_r": np.asarray(sim.eps_r), "times_len": np.int64(res.times.size)}
+    if getattr(sim, "subpixel", False):
+        for c, e in zip(("xx", "yy", "zz"), sim._smoothed_eps_components()):
+            out[f"eps_{c}"] = np.asarray(e)
     for a in range(3):
         out[f"cpml_b_e{a}"] = np.asarray(sim._b_e[a])
         out[f"cpml_c_e{a}"] = np.asarray(sim._c_e[a])

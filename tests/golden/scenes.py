@@ -122,6 +122,41 @@ def flux2d(pf, backend, precision="float64"):
                          backend=backend, precision=precision)
 
 
+def subpixel2d(pf, backend, precision="float64"):
+    """2-D TE (Ex, Ey, Hz) with subpixel smoothing (factor 3): an oblique
+    triangular PolySlab and a box whose walls fall between nodes."""
+    grid = pf.Grid(size=(1.8e-6, 1.6e-6), cell_size=(40e-9, 40e-9), pml_layers=(8, 8, 0))
+    tri = pf.PolySlab(vertices=((-0.35e-6, -0.3e-6), (0.3e-6, -0.17e-6), (-0.1e-6, 0.33e-6)),
+                      z_bounds=(-1e-6, 1e-6), medium=pf.Medium(7.0))
+    box = pf.Box(center=(0.41e-6, 0.23e-6), size=(0.27e-6, 0.33e-6), medium=pf.Medium(3.2))
+    src = pf.PointDipole(position=(-0.5e-6, 0.4e-6), component="Ex",
+                         waveform=pf.GaussianPulse(freq0=_f0(pf), fwhm=5e-15))
+    mon = pf.FieldMonitor(name="s", components=("Ex", "Ey", "Hz"), interval=30)
+    dft = pf.DFTMonitor(name="d", components=("Ey", "Hz"), freqs=[0.8 * _f0(pf), _f0(pf)])
+    return pf.Simulation(grid, structures=[tri, box], sources=[src], monitors=[mon, dft],
+                         run_time=18e-15, backend=backend, precision=precision, subpixel=True,
+                         subpixel_factor=3)
+
+
+def subpixel3d(pf, backend, precision="float64"):
+    """3-D with subpixel smoothing (factor 2): a box and a z-bounded PolySlab
+    with faces between nodes on all three axes; Ez dipole, x-plane DFT."""
+    grid = pf.Grid(size=(1.0e-6, 0.88e-6, 0.8e-6), cell_size=40e-9, pml_layers=(5, 5, 5))
+    box = pf.Box(center=(0.13e-6, -0.05e-6, 0.07e-6), size=(0.31e-6, 0.23e-6, 0.27e-6),
+                 medium=pf.Medium(5.5))
+    hexa = pf.PolySlab(vertices=((-0.3e-6, 0.1e-6), (-0.1e-6, 0.05e-6), (0.05e-6, 0.25e-6),
+                                 (-0.15e-6, 0.3e-6)), z_bounds=(-0.13e-6, 0.11e-6),
+                       medium=pf.Medium(2.7))
+    src = pf.PointDipole(position=(-0.2e-6, -0.2e-6, 0.0), component="Ez",
+                         waveform=pf.GaussianPulse(freq0=_f0(pf), fwhm=4e-15))
+    mon = pf.FieldMonitor(name="s", components=("Ex", "Ey", "Ez", "Hx"), times=[6e-15, 11e-15])
+    dft = pf.DFTMonitor(name="d", components=("Ey", "Ez", "Hy", "Hz"), freqs=[_f0(pf)],
+                        plane_axis="x", plane_position=0.2e-6)
+    return pf.Simulation(grid, structures=[box, hexa], sources=[src], monitors=[mon, dft],
+                         run_time=12e-15, backend=backend, precision=precision, subpixel=True,
+                         subpixel_factor=2)
+
+
 def compressed2d(pf, backend, precision="float64"):
     """2-D TM dipole beside a box with compressed FieldMonitors: int8 frames
     (zlib) every 15 steps and float16 frames of every 2nd node at set times."""
@@ -145,6 +180,8 @@ SCENES = {
     "metals_xz": metals_xz,
     "mode3d": mode3d,
     "flux2d": flux2d,
+    "subpixel2d": subpixel2d,
+    "subpixel3d": subpixel3d,
 }
 # float32 trajectories pinned as well (the reference's NumPy float32 path)
 F32_SCENES = ("tm2d_box", "box3d", "metals_xz")

@@ -36,7 +36,7 @@ SHIPPED = {"_native"}
 # (simulation.py:361-528,794-808; rustbackend.py:85-166 reads the same set).
 SIM_ATTRS = ("grid", "structures", "sources", "monitors", "dt", "n_steps", "dtypes", "eps_r",
              "_b_e", "_c_e", "_b_h", "_c_h", "subpixel", "particle_sources",
-             "progress_callback", "verbose")
+             "progress_callback", "verbose", "_smoothed_eps_components")
 # Attributes of the reference Grid (grid.py:17-96).
 GRID_ATTRS = ("size", "cell_size", "pml_layers", "shape", "coords", "ndim", "index_at")
 
