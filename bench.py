@@ -517,23 +517,38 @@ def bench_batched3d(args):
     print(json.dumps(out), flush=True)
 
 
+_ADJ_MODE = {}
+
+
+def adjoint3d_guide_mode(pf):
+    """Fundamental mode of the 500 x 220 nm silicon core in silica at 1.55 um,
+    solved on the 2.0 x 1.2 um source window (100 x 60 nodes of 20 nm),
+    cached: (ModeSolver, ModeResult)."""
+    if "m" not in _ADJ_MODE:
+        core = pf.Box(center=(0.0, 0.0), size=(500e-9, 220e-9), medium=pf.Medium(12.1))
+        ms = pf.ModeSolver(size=(2.0e-6, 1.2e-6), cell_size=DX, structures=[core],
+                           wavelength=1.55e-6, background_eps=2.085, num_modes=1,
+                           polarization="te")
+        _ADJ_MODE["m"] = (ms, ms.solve())
+    return _ADJ_MODE["m"]
+
+
 def adjoint3d_scene(pf, precision="float32", steps=3000, rho=None):
     """The 3-D adjoint inverse-design scene: 256 x 256 x 128 cells, dx 20 nm,
     12-cell CPML, silica background, a 500 x 220 nm Si core along x, a
     64 x 64 x 8-cell design region at the centre (eps = 2.085 + rho 10.015),
-    a Gaussian-profile Ey ModeSource at x = -1.6 um and an x-plane DFT monitor
-    (Ey, Ez, Hy, Hz; 8 frequencies in c/1.6..c/1.5 um) at x = +1.6 um."""
+    the guide's solved fundamental TE mode (Ey profile) as a ModeSource at
+    x = -1.6 um and an x-plane DFT monitor (Ey, Ez, Hy, Hz; 8 frequencies in
+    c/1.6..c/1.5 um) at x = +1.6 um."""
     shape = (256, 256, 128)
     grid = pf.Grid(size=tuple(n * DX for n in shape), cell_size=DX, pml_layers=(12, 12, 12))
     L = [n * DX for n in shape]
     structures = [pf.Box(center=(0, 0, 0), size=tuple(3 * x for x in L), medium=pf.Medium(2.085)),
                   pf.Box(center=(0, 0, 0), size=(3 * L[0], 500e-9, 220e-9), medium=pf.Medium(12.1))]
-    yy = np.linspace(-1.0e-6, 1.0e-6, 81)
-    zz = np.linspace(-0.6e-6, 0.6e-6, 49)
-    prof = np.exp(-(yy[:, None] / 0.3e-6) ** 2 - (zz[None, :] / 0.2e-6) ** 2)
+    ms, mr = adjoint3d_guide_mode(pf)
     src = pf.ModeSource(center=(-1.6e-6, 0, 0), size=(0, 2.0e-6, 1.2e-6), component="Ey",
                         waveform=pf.GaussianPulse(freq0=pf.C_0 / 1.55e-6, fwhm=10e-15),
-                        profile=prof, profile_coords=(yy, zz))
+                        profile=np.real(mr.Ey[0]), profile_coords=(ms.y, ms.z))
     mon = pf.DFTMonitor(name="out", components=("Ey", "Ez", "Hy", "Hz"),
                         freqs=np.linspace(pf.C_0 / 1.6e-6, pf.C_0 / 1.5e-6, 8), plane_axis="x",
                         plane_position=1.6e-6)
@@ -550,13 +565,19 @@ def adjoint3d_scene(pf, precision="float32", steps=3000, rho=None):
 
 
 def adjoint3d_mode(sim):
-    """Gaussian fundamental-mode fields on the monitor plane (the overlap
-    target): Ey = g(y, z), Hz = g n_eff / Z0 with n_eff 2.5."""
-    y = np.asarray(sim.grid.coords[1])
-    z = np.asarray(sim.grid.coords[2])
-    g = np.exp(-(y[:, None] / 0.3e-6) ** 2 - (z[None, :] / 0.2e-6) ** 2).astype(np.complex128)
-    zero = np.zeros_like(g)
-    return {"Ey": g, "Ez": zero, "Hy": zero, "Hz": g * 2.5 / 376.730313668}
+    """The overlap target on the monitor plane: the solved mode's tangential
+    fields (Ey, Ez, Hy, Hz) placed in the 2.0 x 1.2 um window at the plane's
+    centre, zero outside."""
+    import photonfdtd_amd as pf
+    ms, mr = adjoint3d_guide_mode(pf)
+    ny, nz = sim.grid.shape[1], sim.grid.shape[2]
+    oy, oz = (ny - ms.ny) // 2, (nz - ms.nz) // 2
+    out = {}
+    for c in ("Ey", "Ez", "Hy", "Hz"):
+        plane = np.zeros((ny, nz), dtype=np.complex128)
+        plane[oy:oy + ms.ny, oz:oz + ms.nz] = getattr(mr, c)[0]
+        out[c] = plane
+    return out
 
 
 def bench_adjoint3d(args):

@@ -130,3 +130,70 @@ class ModeSource:
                 out.append(PointDipole(position=tuple(pos), component=self.component,
                                        waveform=self.waveform, amplitude=w))
         return out
+
+
+def _nearest(coords: np.ndarray, x: float) -> int:
+    return 0 if coords.size == 1 else int(np.argmin(np.abs(coords - x)))
+
+
+@dataclass
+class UniModeSource:
+    """One-way launch of a solved waveguide mode along +x or -x
+    (reference ``sources.py:340-433``).
+
+    Love's equivalence: an electric sheet ``J = x_hat x H_m`` and a magnetic
+    sheet ``M = -x_hat x E_m`` on the injection plane radiate the mode forward
+    and cancel it backward. As soft sources on the plane's nodes (E at the
+    E time, H half a step later, as the stepper staggers them):
+
+        Ey += C Hz_m / (eps0 eps_r)      Ez += -C Hy_m / (eps0 eps_r)
+        Hy += -s C Ez_m / mu0            Hz +=  s C Ey_m / mu0
+
+    with ``s = +1`` for "+x", ``-1`` for "-x". The mode's transverse fields
+    are first rotated by one phase so that Ey + Ez is real at the node where
+    |Ey| + |Ez| peaks, then their real parts are used. Each node of the mode
+    grid maps to the nearest node of the simulation plane through ``center``;
+    per mode node the four components are emitted in the order above, skipping
+    zero amplitudes. ``mode`` is a ``ModeResult`` solved on the plane's
+    (y, z) grid.
+    """
+    center: Tuple[float, float, float]
+    size: Tuple[float, float, float]
+    waveform: GaussianPulse
+    mode: object
+    mode_index: int = 0
+    direction: str = "+x"
+    amplitude: float = 1.0
+
+    def expand(self, grid) -> List[PointDipole]:
+        from .constants import EPS_0, MU_0
+        if self.direction not in ("+x", "-x"):
+            raise ValueError("UniModeSource supports direction '+x' or '-x'")
+        s = 1.0 if self.direction == "+x" else -1.0
+        c = list(self.center) + [0.0] * (3 - len(self.center))
+        m, i = self.mode, self.mode_index
+        ey, ez = np.asarray(m.Ey[i]), np.asarray(m.Ez[i])
+        hy, hz = np.asarray(m.Hy[i]), np.asarray(m.Hz[i])
+        peak = int(np.argmax(np.abs(ey) + np.abs(ez)))
+        rot = np.exp(-1j * np.angle((ey.ravel() + ez.ravel())[peak]))
+        ey, ez, hy, hz = (np.real(f * rot) for f in (ey, ez, hy, hz))
+        eps = np.asarray(m.eps_r)
+        C = float(self.amplitude)
+        sheets = (("Ey", C / (EPS_0 * eps) * hz), ("Ez", -C / (EPS_0 * eps) * hy),
+                  ("Hy", -s * C / MU_0 * ez), ("Hz", s * C / MU_0 * ey))
+        xc, yc, zc = grid.coords
+        ix = _nearest(xc, c[0])
+        x = float(xc[ix] if xc.size > 1 else c[0])
+        out: List[PointDipole] = []
+        for jy, y_m in enumerate(np.asarray(m.y)):
+            iy = _nearest(yc, c[1] + y_m)
+            y = float(yc[iy] if yc.size > 1 else c[1])
+            for jz, z_m in enumerate(np.asarray(m.z)):
+                iz = _nearest(zc, c[2] + z_m)
+                z = float(zc[iz] if zc.size > 1 else c[2])
+                for comp, amp in sheets:
+                    a = float(amp[jy, jz])
+                    if a != 0.0:
+                        out.append(PointDipole(position=(x, y, z), component=comp,
+                                               waveform=self.waveform, amplitude=a))
+        return out

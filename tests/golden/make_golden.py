@@ -18,7 +18,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE))
-from scenes import F32_SCENES, FEATURE_SCENES, SCENES  # noqa: E402
+from scenes import F32_SCENES, FEATURE_SCENES, MODE_CASES, SCENES  # noqa: E402
 
 
 def dump(sim, res):
@@ -65,6 +65,17 @@ def main(ref_root, only=None):
             name = key if prec == "float64" else key + "_f32"
             np.savez_compressed(HERE / f"{name}.npz", **dump(sim, res))
             print("wrote", name, sim.grid.shape, sim.n_steps, flush=True)
+    # mode-solver results: n_eff and every field of the solved modes
+    for key, builder in MODE_CASES.items():
+        if only and key not in only:
+            continue
+        mr = builder(ref).solve()
+        out = {"n_eff": np.asarray(mr.n_eff), "y": mr.y, "z": mr.z, "eps_r": mr.eps_r,
+               "psi": mr.psi}
+        for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+            out[c] = np.asarray(getattr(mr, c))
+        np.savez_compressed(HERE / f"{key}.npz", **out)
+        print("wrote", key, mr.n_eff, flush=True)
 
 
 if __name__ == "__main__":

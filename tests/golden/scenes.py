@@ -157,6 +157,46 @@ def subpixel3d(pf, backend, precision="float64"):
                          subpixel_factor=2)
 
 
+def mode_strip(pf):
+    """Silicon strip (500 x 220 nm) in silica, 1.55 um, 20 nm cells: the two
+    highest-index modes of the full-vectorial solver."""
+    core = pf.Box(center=(0.0, 0.0), size=(0.5e-6, 0.22e-6), medium=pf.Medium(12.1))
+    return pf.ModeSolver(size=(1.2e-6, 0.8e-6), cell_size=20e-9, structures=[core],
+                         wavelength=1.55e-6, background_eps=2.085, num_modes=2)
+
+
+def mode_slab_tm(pf):
+    """Asymmetric slab (silicon nitride on silica, air above), TM filter,
+    (dy, dz) = (30, 15) nm, dispersive core evaluated at 1.31 um."""
+    sub = pf.Box(center=(0.0, -0.35e-6), size=(2.0e-6, 0.6e-6), medium=pf.Medium(2.085))
+    core = pf.Box(center=(0.0, 0.0), size=(2.0e-6, 0.1e-6), medium=pf.silicon_nitride())
+    return pf.ModeSolver(size=(0.9e-6, 0.9e-6), cell_size=(30e-9, 15e-9), structures=[sub, core],
+                         wavelength=1.31e-6, num_modes=1, polarization="tm")
+
+
+MODE_CASES = {"mode_strip": mode_strip, "mode_slab_tm": mode_slab_tm}
+
+
+def unimode3d(pf, backend, precision="float64"):
+    """3-D silicon strip along x; a solved TE-like mode launched toward +x by a
+    UniModeSource (electric + magnetic sheets); DFT port planes before and
+    after the source with the tangential (Ey, Ez, Hy, Hz) fields."""
+    d = 40e-9
+    size = (1.6e-6, 1.2e-6, 0.96e-6)
+    grid = pf.Grid(size=size, cell_size=d, pml_layers=(6, 6, 6))
+    core = pf.Box(center=(0, 0, 0), size=(10e-6, 0.44e-6, 0.24e-6), medium=pf.Medium(12.1))
+    mode = pf.ModeSolver(size=size[1:], cell_size=d, structures=[core], wavelength=1.55e-6,
+                         background_eps=1.0, num_modes=1).solve()
+    f0 = pf.C_0 / 1.55e-6
+    src = pf.UniModeSource(center=(-0.4e-6, 0, 0), size=(0, size[1], size[2]),
+                           waveform=pf.GaussianPulse(freq0=f0, fwhm=5e-15), mode=mode)
+    ports = [pf.DFTMonitor(name=nm, components=("Ey", "Ez", "Hy", "Hz"),
+                           freqs=[0.95 * f0, f0], plane_axis="x", plane_position=x)
+             for nm, x in (("back", -0.56e-6), ("fwd", 0.44e-6))]
+    return pf.Simulation(grid, structures=[core], sources=[src], monitors=ports,
+                         run_time=14e-15, backend=backend, precision=precision)
+
+
 def compressed2d(pf, backend, precision="float64"):
     """2-D TM dipole beside a box with compressed FieldMonitors: int8 frames
     (zlib) every 15 steps and float16 frames of every 2nd node at set times."""
@@ -186,8 +226,8 @@ SCENES = {
 # float32 trajectories pinned as well (the reference's NumPy float32 path)
 F32_SCENES = ("tm2d_box", "box3d", "metals_xz")
 # Scenes whose fixtures pin one feature (not full-trajectory oracle scenes):
-# compressed FieldMonitor frames.
-FEATURE_SCENES = {"compressed2d": compressed2d}
+# compressed FieldMonitor frames, a one-way mode source from a solved mode.
+FEATURE_SCENES = {"compressed2d": compressed2d, "unimode3d": unimode3d}
 
 
 def dispersive_slab(pf, backend, shape=(48, 72, 140), steps=300, precision="float64",
