@@ -179,18 +179,22 @@ int pf_dft_flush(pf_handle* h);
 /* How the handle steps (no reference counterpart: the reference's CUDA
  * stepper has one kernel pair). path: 0 generic two-pass kernels, 1 fused
  * one-launch step on dense rows, 2 the 3-D march, 3 the march with the
- * two-step launch. zpad: nodes per 64-slot padded z block (0: dense rows),
- * zrow: slots per padded row. tiles_* / cells_*: march tiles (and the cells
- * they own) that lie off every CPML slab (bulk, the psi-free kernel) and
- * that touch one (cpml). dft_ring: samples a DFT monitor buffers before a
- * fold (largest over monitors). */
+ * two-step launch. zpad: nodes per 64-slot padded z block (0: dense rows;
+ * 64: rows of the 64-lane march, node k in slot k), zrow: slots per row.
+ * tiles_* / cells_*: march tiles (and the cells they own) that lie
+ * off every CPML slab (bulk, the psi-free kernel) and that touch one (cpml).
+ * cpml_lists: which CPML tile lists are non-empty, bit 0 the tiles in slabs
+ * of several axes (or of any axis with padded rows), bits 1 / 2 / 3 the
+ * tiles in z / y / x slabs only (dense rows: each runs the kernel variant of
+ * that axis). dft_ring: samples a DFT monitor buffers before a fold (largest
+ * over monitors). */
 typedef struct pf_info {
   int32_t path;
   int32_t zpad;
   int32_t zrow;
   int32_t march_nw;
   int32_t pml_nw;
-  int32_t reserved;
+  int32_t cpml_lists;
   int64_t tiles_bulk;
   int64_t tiles_cpml;
   int64_t cells_bulk;

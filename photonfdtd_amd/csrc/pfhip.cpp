@@ -169,6 +169,19 @@ std::vector<Seg> split_even(int n, int b0, int b1, int len) {
   return out;
 }
 
+// Segments of `len` nodes aligned to 0 (the last one shorter), bulk when
+// inside [b0, b1). The dense-row march cuts y and z this way: a segment that
+// reaches into a CPML slab keeps its full length and runs with the kernel
+// variant of that axis, so the CPML tiles stay as full as the bulk ones.
+std::vector<Seg> split_aligned(int n, int b0, int b1, int len) {
+  std::vector<Seg> out;
+  for (int s = 0; s < n; s += len) {
+    const int c = std::min(len, n - s);
+    out.push_back({s, c, s >= b0 && s + c <= b1});
+  }
+  return out;
+}
+
 // Dense rows [nx][ny][nz] -> padded rows [nx][ny][zrow], zpad nodes per
 // block (pad slots zero).
 template <typename real>
@@ -325,6 +338,7 @@ class Stepper final : public pf_handle {
   const pfhip::MarchArgs<real>& march_args(int p) const { return ma_[p]; }
   int level() const { return cur_; }
   int march_dist() const { return march_dist_; }
+  int zpad() const { return zpad_; }
   void after_march(int64_t step);
   // Device tables of the ADE media for one launch of all media per step,
   // indexed [field level read][polarisation level] (the two-step march
@@ -417,7 +431,9 @@ class Stepper final : public pf_handle {
   pfhip::MarchArgs<real> ma_[2]; // march-kernel arguments reading level p
   // Field levels of the march use padded z rows (pfhip::zslot) when zpad_:
   // rows of zrow_ slots, fcell_ elements per component, ce in ce_f_.
-  int zpad_ = 0;                 // nodes per padded block (62, or 60 with the two-step march); 0: dense
+  int zpad_ = 0;                 // nodes per padded block: 64 (dense rows padded at the
+                                 // end, the 64-lane march), 62, or 60 with the two-step
+                                 // march; 0: dense rows with the 62-node march
   // 2-D batch members: padded y rows (ypad_ columns per 64-slot block) of yrow_ slots
   int ypad_ = 0;
   int yrow_ = 0;
@@ -429,9 +445,14 @@ class Stepper final : public pf_handle {
   real* psi_h_lvl1_ = nullptr;           // H-side psi, level 1
   size_t psi_len_h_ = 0;
   real* coef_ = nullptr;
-  int* tiles_[2] = {nullptr, nullptr};   // march tiles: [0] psi-free, [1] with CPML
-  int ntiles_[2] = {0, 0};
-  int64_t tile_cells_[2] = {0, 0};       // owned cells of each list
+  // March tile lists: [0] psi-free, [1] CPML on several axes (or any axis
+  // with padded rows); with dense rows [2] z-only, [3] y-only, [4] x-only CPML
+  // tiles, each launched with the kernel variant of its axis set.
+  static constexpr int NLIST = 5;
+  static constexpr int list_mask_[NLIST] = {0, 7, 4, 2, 1};
+  int* tiles_[NLIST] = {};
+  int ntiles_[NLIST] = {};
+  int64_t tile_cells_[NLIST] = {};       // owned cells of each list
   int march_nw_ = pfhip::MARCH_NW;
   int pml_nw_ = pfhip::MARCH_NW;  // wavefronts per workgroup of the CPML-tile launch
   int march_dist_ = 2;            // planes of prefetch in the march kernel
@@ -757,8 +778,12 @@ Stepper<real>::Stepper(const pf_desc& d) {
     // 62-node blocks for the single-step march's tiles; 60 when the
     // two-step march (two-node apron) shares the levels
     const char* env_pair = std::getenv("PHOTONFDTD_HIP_PAIR");
+    // 64: dense rows padded at the end only, the 64-lane march (default);
+    // 62: 62 nodes per 64-slot block; 0: dense rows with the 62-node march
     zpad_ = variant == 2 && !(env_zpad && env_zpad[0] == '0')
-                ? (env_pair && env_pair[0] == '1' ? pfhip::MARCH2_OWNED_Z : 62)
+                ? (env_pair && env_pair[0] == '1'
+                       ? pfhip::MARCH2_OWNED_Z
+                       : (env_zpad && std::atoi(env_zpad) == 62 ? 62 : pfhip::ZPAD_BLOCK))
                 : 0;
     if (zpad_) {
       zrow_ = pfhip::zpad_row(nz, zpad_);
@@ -794,8 +819,11 @@ Stepper<real>::Stepper(const pf_desc& d) {
     real* psi_h1[6];
     for (int s = 0; s < 6; ++s)
       psi_h1[s] = psi_h_lvl1_ ? psi_h_lvl1_ + (args_.psi[6 + s] - args_.psi[6]) : nullptr;
+    // planes per march tile: with dense rows about 48, so that each list
+    // holds several workgroups per CU slot and the last round of tiles
+    // leaves little of the chip idle (measured at 512^3 float32)
     const char* env_xc = std::getenv("PHOTONFDTD_HIP_XCHUNK");
-    const int xchunk = env_xc ? std::max(1, std::atoi(env_xc)) : 128;
+    const int xchunk = env_xc ? std::max(1, std::atoi(env_xc)) : zpad_ == pfhip::ZPAD_BLOCK ? 48 : 128;
     for (int p = 0; p < 2; ++p) {
       pfhip::FusedArgs<real>& fa = fa_[p];
       std::memset(&fa, 0, sizeof(fa));
@@ -905,7 +933,11 @@ void Stepper<real>::build_march_tiles(int xchunk) {
   for (int a = 0; a < 3; ++a) {
     const int b0 = std::max(slab_e_[a][0], slab_h_[a][0] + 1);
     const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]);
-    if (a == 2 && zpad_) {
+    if (a > 0 && zpad_ == pfhip::ZPAD_BLOCK && split_) {
+      seg[a] = split_aligned(n[a], b0, b1, a == 2 ? pfhip::ZPAD_BLOCK : len[a]);
+    } else if (a == 2 && zpad_ == pfhip::ZPAD_BLOCK) {
+      seg[2] = split_aligned(n[2], 0, n[2], pfhip::ZPAD_BLOCK);
+    } else if (a == 2 && zpad_) {
       // padded rows: z tiles on the fixed grid of zpad_ nodes (each owns
       // one 64-slot block); a tile is psi-free when it lies inside [b0, b1)
       for (int z = 0; z < n[2]; z += zpad_) {
@@ -916,12 +948,23 @@ void Stepper<real>::build_march_tiles(int xchunk) {
       seg[a] = split_ ? split_axis(n[a], b0, b1, len[a], a > 0) : split_axis(n[a], 0, 0, len[a], false);
     }
   }
-  std::vector<pfhip::MarchTile> bulk, pml;
+  std::vector<pfhip::MarchTile> lists[NLIST];
+  std::vector<pfhip::MarchTile>& bulk = lists[0];
+  std::vector<pfhip::MarchTile>& pml = lists[1];
   for (const Seg& sx : seg[0])
     for (const Seg& sy : seg[1])
       for (const Seg& sz : seg[2]) {
         pfhip::MarchTile t{sx.start, sx.start + sx.count, sy.start, sy.count, sz.start, sz.count, 0, 0};
-        (split_ && sx.bulk && sy.bulk && sz.bulk ? bulk : pml).push_back(t);
+        const int mask = (sx.bulk ? 0 : 1) | (sy.bulk ? 0 : 2) | (sz.bulk ? 0 : 4);
+        if (!split_) {
+          pml.push_back(t);
+        } else if (zpad_ == pfhip::ZPAD_BLOCK) {
+          int q = 0;
+          while (q < NLIST && list_mask_[q] != mask) ++q;
+          lists[q < NLIST ? q : 1].push_back(t);
+        } else {
+          lists[mask ? 1 : 0].push_back(t);
+        }
       }
   if (split_) {
     // CPML tiles in shorter x chunks: that launch holds one workgroup per
@@ -943,12 +986,11 @@ void Stepper<real>::build_march_tiles(int xchunk) {
         }
     pml.swap(cut);
   }
-  const std::vector<pfhip::MarchTile>* lists[2] = {&bulk, &pml};
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < NLIST; ++q) {
     tile_cells_[q] = 0;
-    for (const pfhip::MarchTile& t : *lists[q]) tile_cells_[q] += (int64_t)(t.i1 - t.i0) * t.jn * t.kn;
+    for (const pfhip::MarchTile& t : lists[q]) tile_cells_[q] += (int64_t)(t.i1 - t.i0) * t.jn * t.kn;
     std::vector<pfhip::MarchTile> ordered =
-        band_order_ ? xcd_band_order(*lists[q]) : xcd_order(*lists[q]);
+        band_order_ ? xcd_band_order(lists[q]) : xcd_order(lists[q]);
     ntiles_[q] = (int)ordered.size();
     tiles_[q] = ordered.empty() ? nullptr
                                 : reinterpret_cast<int*>(arena_.upload(ordered.data(), ordered.size(), stream_));
@@ -1267,20 +1309,30 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
     const int in = cur_;
     timed(PF_K_YEE_FUSED, [&] {
       if (fa_[in].variant == 2) {
-        // CPML tiles on the side stream when both lists are in use, so
-        // the two launches overlap (they write disjoint tiles of level out
-        // and only read level in)
-        const bool fork = side_ && ntiles_[0] > 0 && ntiles_[1] > 0;
+        // The all-axis, x and y CPML lists on the side stream when the
+        // main stream has tiles too, so the launches overlap (they write
+        // disjoint tiles of level out and only read level in); the slowest
+        // list goes first.
+        static constexpr int order[NLIST] = {1, 4, 3, 2, 0};
+        static constexpr bool on_side[NLIST] = {false, true, false, true, true};
+        bool any_side = false, any_main = false;
+        for (int q = 0; q < NLIST; ++q) (on_side[q] ? any_side : any_main) |= ntiles_[q] > 0;
+        const bool fork = side_ && any_side && any_main;
         if (fork) {
           HIP_CHECK(hipEventRecord(fork_ev_[0], stream_));
           HIP_CHECK(hipStreamWaitEvent(side_, fork_ev_[0], 0));
         }
-        for (int q = 1; q >= 0; --q) {
+        for (int q : order) {
+          if (ntiles_[q] == 0) continue;
           pfhip::MarchArgs<real> ma = ma_[in];
           ma.tiles = tiles_[q];
           ma.ntiles = ntiles_[q];
-          pfhip::launch_march<real>(ma, exact_, q == 1, q == 1 ? pml_nw_ : march_nw_, march_dist_,
-                                    fork && q == 1 ? side_ : stream_);
+          hipStream_t s = fork && on_side[q] ? side_ : stream_;
+          if (zpad_ == pfhip::ZPAD_BLOCK)
+            pfhip::launch_march64<real>(ma, exact_, list_mask_[q], q == 1 ? pml_nw_ : march_nw_, s);
+          else
+            pfhip::launch_march<real>(ma, exact_, q == 1, q == 1 ? pml_nw_ : march_nw_,
+                                      march_dist_, s);
         }
         if (fork) {
           HIP_CHECK(hipEventRecord(fork_ev_[1], side_));
@@ -1371,7 +1423,9 @@ std::vector<pfhip::MarchTile> Stepper<real>::march_tile_list(int nw, int xchunk,
   for (int a = 0; a < 3; ++a) {
     const int b0 = std::max(slab_e_[a][0], slab_h_[a][0] + 1);
     const int b1 = std::min(slab_e_[a][1], slab_h_[a][1]);
-    if (a == 2 && zpad_) {
+    if (a > 0 && zpad_ == pfhip::ZPAD_BLOCK) {
+      seg[a] = split_aligned(n[a], b0, b1, a == 2 ? pfhip::ZPAD_BLOCK : len[a]);
+    } else if (a == 2 && zpad_) {
       for (int z = 0; z < n[2]; z += zpad_) {
         const int c = std::min(zpad_, n[2] - z);
         seg[2].push_back({z, c, z >= b0 && z + c <= b1});
@@ -1837,9 +1891,12 @@ void Stepper<real>::info(pf_info* out) const {
   out->march_nw = march_nw_;
   out->pml_nw = pml_nw_;
   out->tiles_bulk = ntiles_[0];
-  out->tiles_cpml = ntiles_[1];
   out->cells_bulk = tile_cells_[0];
-  out->cells_cpml = tile_cells_[1];
+  for (int q = 1; q < NLIST; ++q) {
+    out->tiles_cpml += ntiles_[q];
+    out->cells_cpml += tile_cells_[q];
+    if (ntiles_[q] > 0) out->cpml_lists |= 1 << (q - 1);
+  }
   out->pair_tiles_bulk = ntiles2_[0];
   out->pair_tiles_cpml = ntiles2_[1];
   for (const Dft& m : dft_) out->dft_ring = std::max<int64_t>(out->dft_ring, m.K);
@@ -2286,8 +2343,11 @@ class Batch3 final : public BatchBase {
       n_steps_max_ = std::max<int64_t>(n_steps_max_, d.n_steps);
     }
     dist_ = sims_[0]->march_dist();
-    for (auto& s : sims_)
+    zpad_ = sims_[0]->zpad();
+    for (auto& s : sims_) {
       if (s->march_dist() != dist_) throw ArgError{"a batch needs one march prefetch distance"};
+      if (s->zpad() != zpad_) throw ArgError{"a batch needs one field row layout"};
+    }
     for (int p = 0; p < 2; ++p) {
       std::vector<pfhip::MarchArgs<real>> t;
       for (auto& s : sims_) t.push_back(s->march_args(p));
@@ -2420,7 +2480,10 @@ class Batch3 final : public BatchBase {
         L.tiles = tiles_[q];
         L.ntiles = (int)prefix_[q][running];
         L.sims = table_[p];
-        if (L.ntiles > 0) pfhip::launch_march<real>(L, exact_, q == 1, nw_, dist_, stream_);
+        if (L.ntiles > 0) {
+          if (zpad_ == pfhip::ZPAD_BLOCK) pfhip::launch_march64<real>(L, exact_, q == 1 ? 7 : 0, nw_, stream_);
+          else pfhip::launch_march<real>(L, exact_, q == 1, nw_, dist_, stream_);
+        }
       }
       // ADE media and E-side sources of every running simulation, then
       // each simulation's level flip and DFT capture (the single run's order)
@@ -2460,6 +2523,7 @@ class Batch3 final : public BatchBase {
   int max_groups_ = 0;
   int* tiles_[2] = {nullptr, nullptr};   // psi-free tiles, CPML tiles
   int nw_ = 0, xchunk_ = 0, dist_ = 2;
+  int zpad_ = 0;    // row layout of the member simulations (ZPAD_BLOCK: the 64-lane march)
   int64_t step_ = 0, n_steps_max_ = 0;
   bool exact_;
   hipEvent_t ev_[2] = {nullptr, nullptr};

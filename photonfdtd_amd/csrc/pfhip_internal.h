@@ -481,6 +481,12 @@ void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int 
 constexpr int MARCH_NW = 10;   // wavefronts per march workgroup (rows j0-1 .. j0+NW-2)
 template <typename real>
 void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, int dist, hipStream_t s);
+// The march on dense rows (zpad == ZPAD_BLOCK: node k in slot k) with tiles of
+// up to 64 z nodes starting on 64-node boundaries (k_yee_m64). mask: the CPML
+// axes of the tiles (bit 0 x, 1 y, 2 z): 0 and 7 for any launch, 1, 2 and 4
+// for single-simulation launches.
+template <typename real>
+void launch_march64(MarchArgs<real>& a, bool exact, int mask, int nw, hipStream_t s);
 template <typename real> bool fused_supported(const StepArgs<real>& a);
 template <typename real> void launch_fused(FusedArgs<real>& a, bool exact, hipStream_t s);
 template <typename real>

@@ -136,5 +136,45 @@ __device__ __forceinline__ void bst<double>(double* base, uint32_t boff, double 
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), buf_rsrc(base), boff, 0, 0);
 }
+// Buffer store with the nontemporal cache policy (streaming output).
+constexpr int BUF_NT = 2;
+template <typename real>
+__device__ __forceinline__ void bst_nt(real* base, uint32_t boff, real v);
+template <>
+__device__ __forceinline__ void bst_nt<float>(float* base, uint32_t boff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), buf_rsrc(base), boff, 0,
+                                        BUF_NT);
+}
+template <>
+__device__ __forceinline__ void bst_nt<double>(double* base, uint32_t boff, double v) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), buf_rsrc(base), boff, 0,
+                                        BUF_NT);
+}
+
+// Value of lane l+1 / l-1 of the wavefront (DPP wavefront shifts; the lane
+// with no source keeps its own value).
+__device__ __forceinline__ float lane_next(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_next(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+  const int nlo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+  const int nhi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)nhi << 32) | (unsigned)nlo);
+}
+__device__ __forceinline__ double lane_prev(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+  const int nlo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+  const int nhi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)nhi << 32) | (unsigned)nlo);
+}
 
 }  // namespace pfhip

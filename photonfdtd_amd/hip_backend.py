@@ -378,7 +378,7 @@ class HipEngine:
         """Stepping path, z-row padding and march tile counts (``pf_stepper_info``)."""
         out = N.PfInfo()
         N.check(self.lib, self.lib.pf_stepper_info(self.h, C.byref(out)), "pf_stepper_info")
-        return {name: int(getattr(out, name)) for name, _ in out._fields_ if name != "reserved"}
+        return {name: int(getattr(out, name)) for name, _ in out._fields_}
 
     def dft_flush(self) -> None:
         """Fold every buffered DFT sample into the accumulators (synchronous)."""

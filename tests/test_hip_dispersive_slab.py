@@ -2,13 +2,16 @@
 Drude cube, silica cladding, Ey dipole, two 32-frequency x-plane DFT
 monitors, 12-cell CPML on six faces) against the float64 oracle.
 
-The grid (48 x 72 x 140) is sized so the march runs its psi-free kernel on
-padded rows: the z block of nodes [62, 124) lies inside the CPML-free range
-[12, 128), and so do x/y tiles, so the bulk tile list is non-empty and the
-silicon box crosses those tiles. Bars: float64 in the reference operation
-order within 1e-12 relative L2 of the oracle (fields and both DFT planes),
-float32 in the fast order within 1e-4; dense z rows (every tile on the CPML
-kernel) give the same bits as the padded split launch.
+The grid (48 x 72 x 140) is sized so the 64-lane march runs every kernel
+variant: the z tile of nodes [64, 128) lies inside the CPML-free range
+[13, 128), and so do x segments and y blocks, so the psi-free tile list is
+non-empty and the silicon box crosses those tiles; the tiles that reach into
+the z slabs only, the y slabs only, the x slabs only and several slabs each
+run their own variant. Bars: float64 in the reference operation order
+within 1e-12 relative L2 of the oracle (fields and both DFT planes), float32
+in the fast order within 1e-4; the other layouts (dense rows with 62-node
+windows, every tile on the all-axis kernel; padded rows of 62 nodes per 64
+slots) give the same bits as the 64-lane split launch.
 """
 import numpy as np
 import pytest
@@ -59,12 +62,13 @@ def _worst(out, dft, ref, state):
     return w
 
 
-def test_fp64_exact_psi_free_tiles_match_oracle(oracle_run, monkeypatch):
+def test_fp64_exact_split_march_matches_oracle(oracle_run, monkeypatch):
     ref, state = oracle_run
-    print("[case] dispersive slab float64 exact, padded split march", flush=True)
+    print("[case] dispersive slab float64 exact, 64-lane split march", flush=True)
     info, out, dft = _gpu("float64", True, {}, monkeypatch)
-    assert info["path"] == 2 and info["zpad"] == 62, info
+    assert info["path"] == 2 and info["zpad"] == 64, info   # the 64-lane march on dense rows
     assert info["tiles_bulk"] > 0 and info["cells_bulk"] > 0, info
+    assert info["cpml_lists"] == 0b1111, info   # several-axis, z-, y- and x-only tiles
     worst = _worst(out, dft, ref, state)
     print(f"[fp64] dispersive slab worst rel L2 = {worst:.3e}; psi-free tiles "
           f"{info['tiles_bulk']} ({info['cells_bulk']} cells), CPML tiles {info['tiles_cpml']}",
@@ -75,20 +79,22 @@ def test_fp64_exact_psi_free_tiles_match_oracle(oracle_run, monkeypatch):
 
 def test_fp32_fast_within_1e4_of_oracle(oracle_run, monkeypatch):
     ref, state = oracle_run
-    print("[case] dispersive slab float32 fast, padded split march", flush=True)
+    print("[case] dispersive slab float32 fast, 64-lane split march", flush=True)
     info, out, dft = _gpu("float32", False, {}, monkeypatch)
-    assert info["tiles_bulk"] > 0, info
+    assert info["tiles_bulk"] > 0 and info["cpml_lists"] == 0b1111, info
     worst = _worst(out, dft, ref, state)
     print(f"[fp32] dispersive slab worst rel L2 vs float64 oracle = {worst:.3e}", flush=True)
     assert worst <= 1e-4
 
 
+@pytest.mark.parametrize("zpad", ["0", "62"])
 @pytest.mark.parametrize("precision,exact", [("float64", True), ("float32", False)])
-def test_dense_rows_same_bits_as_padded_split(precision, exact, monkeypatch):
-    print(f"[case] dispersive slab padded vs dense rows {precision}", flush=True)
+def test_other_layouts_same_bits_as_split_march(precision, exact, zpad, monkeypatch):
+    print(f"[case] dispersive slab 64-lane split vs zpad {zpad} {precision}", flush=True)
     ia, fa, da = _gpu(precision, exact, {}, monkeypatch)
-    ib, fb, db = _gpu(precision, exact, {"PHOTONFDTD_HIP_ZPAD": "0"}, monkeypatch)
-    assert ia["tiles_bulk"] > 0 and ib["zpad"] == 0 and ib["tiles_bulk"] == 0, (ia, ib)
+    ib, fb, db = _gpu(precision, exact, {"PHOTONFDTD_HIP_ZPAD": zpad}, monkeypatch)
+    assert ia["tiles_bulk"] > 0 and ib["zpad"] == int(zpad), (ia, ib)
+    assert (ib["tiles_bulk"] > 0) == (zpad != "0"), ib
     for c in FIELDS:
         assert np.array_equal(fa[c], fb[c]), c
     for nm in ("in", "out"):

@@ -1,12 +1,13 @@
 """GPU: the 3-D march gives the same bits under every layout and launch shape.
 
-The march stores fields in padded z rows by default (node k in slot
-k + 2 * (k // 62), so a 62-node z tile's row store covers whole lines) and
-splits each step into a psi-free launch and a CPML launch on a second
-stream. Dense rows, a single launch, no overlap and other tile shapes must
-give identical fields, DFT planes, flux totals and field snapshots; the
-snapshots and DFT planes also read the padded rows through the same slot
-mapping, and write_field / read_field round-trip through it.
+By default the march runs 64 lanes per z tile on rows of a multiple of 64
+slots (node k in slot k) and splits each step into a psi-free launch and
+one launch per CPML axis set, the several-axis, x- and y-only lists on a
+second stream. The other layouts (dense rows with 62-node windows; padded
+rows, node k in slot k + 2 * (k // 62)), a single launch, no overlap and
+other tile shapes must give identical fields, DFT planes, flux totals and
+field snapshots; snapshots and DFT planes read the rows through the same
+slot mapping, and write_field / read_field round-trip through it.
 """
 import numpy as np
 import pytest
@@ -17,14 +18,16 @@ pytestmark = pytest.mark.gpu
 
 CONFIGS = {
     "dense single launch": {"PHOTONFDTD_HIP_ZPAD": "0", "PHOTONFDTD_HIP_SPLIT": "0"},
-    "padded single launch": {"PHOTONFDTD_HIP_SPLIT": "0"},
-    "padded split, no overlap": {"PHOTONFDTD_HIP_OVERLAP": "0"},
-    "padded split, 5-plane CPML chunks, 10 waves": {"PHOTONFDTD_HIP_PML_XCHUNK": "5",
-                                                    "PHOTONFDTD_HIP_NW": "10",
-                                                    "PHOTONFDTD_HIP_PML_NW": "10"},
-    "padded split, 16 waves, 7-plane chunks": {"PHOTONFDTD_HIP_NW": "16",
-                                               "PHOTONFDTD_HIP_XCHUNK": "7"},
+    "64-lane single launch": {"PHOTONFDTD_HIP_SPLIT": "0"},
+    "64-lane split, no overlap": {"PHOTONFDTD_HIP_OVERLAP": "0"},
+    "64-lane split, 5-plane CPML chunks, 10 waves": {"PHOTONFDTD_HIP_PML_XCHUNK": "5",
+                                                     "PHOTONFDTD_HIP_NW": "10",
+                                                     "PHOTONFDTD_HIP_PML_NW": "10"},
+    "64-lane split, 16 waves, 7-plane chunks": {"PHOTONFDTD_HIP_NW": "16",
+                                                "PHOTONFDTD_HIP_XCHUNK": "7"},
     "dense split": {"PHOTONFDTD_HIP_ZPAD": "0", "PHOTONFDTD_HIP_SPLIT": "1"},
+    "padded split": {"PHOTONFDTD_HIP_ZPAD": "62"},
+    "padded single launch": {"PHOTONFDTD_HIP_ZPAD": "62", "PHOTONFDTD_HIP_SPLIT": "0"},
 }
 KEYS = ("PHOTONFDTD_HIP_ZPAD", "PHOTONFDTD_HIP_SPLIT", "PHOTONFDTD_HIP_OVERLAP",
         "PHOTONFDTD_HIP_PML_XCHUNK", "PHOTONFDTD_HIP_NW", "PHOTONFDTD_HIP_PML_NW",
@@ -62,14 +65,18 @@ def test_layouts_and_launch_shapes_agree(precision, monkeypatch):
             assert np.array_equal(np.asarray(got.flux[m]), np.asarray(ref.flux[m])), (label, m)
 
 
-def test_write_and_read_field_through_padded_rows(monkeypatch):
-    """pf_write_field / pf_read_box on a stepper with padded rows: a written
-    field reads back unchanged."""
+@pytest.mark.parametrize("zpad", [None, "62"])
+def test_write_and_read_field_through_padded_rows(zpad, monkeypatch):
+    """pf_write_field / pf_read_box on a stepper whose rows carry slots past
+    nz (the default 64-slot rows; padded 62-node blocks): a written field
+    reads back unchanged."""
     import photonfdtd_amd as pf
     from photonfdtd_amd.hip_backend import HipEngine
     for k in KEYS:
         monkeypatch.delenv(k, raising=False)
-    print("\n[case] padded write/read round trip", flush=True)
+    if zpad:
+        monkeypatch.setenv("PHOTONFDTD_HIP_ZPAD", zpad)
+    print(f"\n[case] write/read round trip, zpad {zpad or 'default'}", flush=True)
     sim = _scene(pf, "float32")
     eng = HipEngine(sim, np.float32, True)
     try:
