@@ -450,6 +450,7 @@ def bench_batched(args):
                      "kernel_ms_per_step": round(dev_ms / args.steps, 4)},
         "march": {"psi_free_cell_fraction": round(info["cells_bulk"] / cells, 4),
                   "tiles_psi_free": info["tiles_bulk"], "tiles_cpml": info["tiles_cpml"],
+                  "cpml_lists": info["cpml_lists"], "zrow": info["zrow"],
                   "dft_ring": info["dft_ring"]},
         "dft_finite": bool(finite),
         "setup_s": round(setup_s, 1),
@@ -557,6 +558,7 @@ def bench_batched3d(args):
         "speedup_vs_single_runs": round(single / wall, 3),
         "march": {"psi_free_cell_fraction": round(info["cells_bulk"] / cells, 4),
                   "tiles_psi_free": info["tiles_bulk"], "tiles_cpml": info["tiles_cpml"],
+                  "cpml_lists": info["cpml_lists"], "zrow": info["zrow"],
                   "dft_ring": info["dft_ring"]},
         "dft_finite": bool(finite),
     }
@@ -870,10 +872,10 @@ def main():
                                   copy_fetch_cells=big[0] * yrow)
         else:
             n = args.n
-            # the march layout pads z rows to 64 slots per 62 nodes: the
-            # calibrating gather reads whole padded rows
-            zrow = 64 * ((n + 61) // 62)
-            traffic_from_counters(*args.traffic, kernel="k_yee_march", copy_cells=n ** 3,
+            # the march keeps z rows of a multiple of 64 slots: the
+            # calibrating gather reads whole rows
+            zrow = 64 * ((n + 63) // 64)
+            traffic_from_counters(*args.traffic, kernel="k_yee_m64", copy_cells=n ** 3,
                                   alg_bytes=n ** 3 * 13 * item, itemsize=item,
                                   name=f"dispersive_slab_{n}_{args.precision}",
                                   workload=f"3-D dispersive slab {n}^3 {args.precision}",
@@ -987,6 +989,7 @@ def main():
         "hbm_fraction_of_value": round(value * 1e9 * bytes_per_update / 1e9 / HBM_PEAK_GBS, 4),
         "march": {"psi_free_cell_fraction": round(info["cells_bulk"] / cells, 4),
                   "tiles_psi_free": info["tiles_bulk"], "tiles_cpml": info["tiles_cpml"],
+                  "cpml_lists": info["cpml_lists"], "zrow": info["zrow"],
                   "dft_ring": info["dft_ring"]},
         "dft_finite": bool(finite),
         "ey_max_final": field_max,
