@@ -359,29 +359,52 @@ __global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __
 // Fold `count` buffered samples into the complex128 accumulators, one sample
 // at a time in step order: acc += (phase * f) * dt, exactly the sequence of
 // the reference's per-step `acc += phase * snap * dt` (simulation.py:1133).
+// A thread owns one box node of one component for FOLD_FB consecutive
+// frequencies (blockIdx.y = component x frequency block, so the phase
+// factors are wave-uniform scalar loads), and reads each buffered sample once
+// for all of them.
+constexpr int FOLD_FB = 8;
+
 template <typename real>
-__global__ void k_dft_fold(const real* __restrict__ ring, int64_t nbox, int n_comp,
-                           const int32_t* __restrict__ is_h, int count, int64_t sample0,
-                           const double* __restrict__ phase_e, const double* __restrict__ phase_h,
-                           int n_freq, double dt, double* __restrict__ acc) {
-  // one thread per (component, frequency, box node); consecutive threads take
-  // consecutive nodes, so ring reads and accumulator updates are coalesced
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nbox * n_comp * n_freq) return;
-  const int64_t r = t % nbox;
-  const int64_t cf = t / nbox;
-  const int f = (int)(cf % n_freq), c = (int)(cf / n_freq);
+__global__ __launch_bounds__(256) void k_dft_fold(const real* __restrict__ ring, int64_t nbox,
+                                                  int n_comp, const int32_t* __restrict__ is_h,
+                                                  int count, int64_t sample0,
+                                                  const double* __restrict__ phase_e,
+                                                  const double* __restrict__ phase_h, int n_freq,
+                                                  double dt, double* __restrict__ acc) {
+  const int nfb = (n_freq + FOLD_FB - 1) / FOLD_FB;
+  const int c = (int)blockIdx.y / nfb, f0 = ((int)blockIdx.y % nfb) * FOLD_FB;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nbox) return;
+  const int nf = min(FOLD_FB, n_freq - f0);
   const double* ph = is_h[c] ? phase_h : phase_e;
-  double* A = acc + 2 * (((int64_t)c * n_freq + f) * nbox + r);
-  double re = A[0], im = A[1];
-  for (int s = 0; s < count; ++s) {
-    const double v = (double)ring[((int64_t)s * n_comp + c) * nbox + r];
-    const double* p = ph + 2 * ((sample0 + s) * n_freq + f);
-    re = re + (p[0] * v) * dt;
-    im = im + (p[1] * v) * dt;
+  double* A = acc + 2 * (((int64_t)c * n_freq + f0) * nbox + r);
+  double re[FOLD_FB], im[FOLD_FB];
+#pragma unroll
+  for (int q = 0; q < FOLD_FB; ++q) {
+    re[q] = q < nf ? A[2 * q * nbox] : 0.0;
+    im[q] = q < nf ? A[2 * q * nbox + 1] : 0.0;
   }
-  A[0] = re;
-  A[1] = im;
+  const real* src = ring + (int64_t)c * nbox + r;
+  const int64_t stride = (int64_t)n_comp * nbox;
+  for (int s = 0; s < count; ++s) {
+    const double v = (double)src[s * stride];
+    const double* p = ph + 2 * ((sample0 + s) * n_freq + f0);
+#pragma unroll
+    for (int q = 0; q < FOLD_FB; ++q) {
+      if (q < nf) {
+        re[q] = re[q] + (p[2 * q] * v) * dt;
+        im[q] = im[q] + (p[2 * q + 1] * v) * dt;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < FOLD_FB; ++q) {
+    if (q < nf) {
+      A[2 * q * nbox] = re[q];
+      A[2 * q * nbox + 1] = im[q];
+    }
+  }
 }
 
 // Plane Poynting flux, stage 1: per-block partial sums of S . n.
@@ -479,8 +502,10 @@ template <typename real>
 void launch_dft_fold(const real* ring, int64_t nbox, int n_comp, const int32_t* is_h, int count,
                      int64_t sample0, const double* phase_e, const double* phase_h, int n_freq,
                      double dt, double* acc, hipStream_t s) {
-  k_dft_fold<real><<<blocks_for(nbox * n_comp * n_freq, 256), 256, 0, s>>>(
-      ring, nbox, n_comp, is_h, count, sample0, phase_e, phase_h, n_freq, dt, acc);
+  if (nbox <= 0 || n_comp <= 0 || n_freq <= 0 || count <= 0) return;
+  const dim3 grid(blocks_for(nbox, 256), (unsigned)(n_comp * ((n_freq + FOLD_FB - 1) / FOLD_FB)));
+  k_dft_fold<real><<<grid, 256, 0, s>>>(ring, nbox, n_comp, is_h, count, sample0, phase_e,
+                                        phase_h, n_freq, dt, acc);
 }
 
 template <typename real>

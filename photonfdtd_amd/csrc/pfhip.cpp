@@ -985,6 +985,18 @@ void Stepper<real>::build_march_tiles(int xchunk) {
           cut.push_back(u);
         }
     pml.swap(cut);
+    // the single-axis CPML lists run at the CPML workgroup height too
+    for (int q = 2; q < NLIST; ++q) {
+      std::vector<pfhip::MarchTile> rows;
+      for (const pfhip::MarchTile& t : lists[q])
+        for (int y = t.j0; y < t.j0 + t.jn; y += py) {
+          pfhip::MarchTile u = t;
+          u.j0 = y;
+          u.jn = std::min(py, t.j0 + t.jn - y);
+          rows.push_back(u);
+        }
+      lists[q].swap(rows);
+    }
   }
   for (int q = 0; q < NLIST; ++q) {
     tile_cells_[q] = 0;
@@ -1329,7 +1341,7 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
           ma.ntiles = ntiles_[q];
           hipStream_t s = fork && on_side[q] ? side_ : stream_;
           if (zpad_ == pfhip::ZPAD_BLOCK)
-            pfhip::launch_march64<real>(ma, exact_, list_mask_[q], q == 1 ? pml_nw_ : march_nw_, s);
+            pfhip::launch_march64<real>(ma, exact_, list_mask_[q], q == 0 ? march_nw_ : pml_nw_, s);
           else
             pfhip::launch_march<real>(ma, exact_, q == 1, q == 1 ? pml_nw_ : march_nw_,
                                       march_dist_, s);
