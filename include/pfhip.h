@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 7
+#define PFHIP_ABI_VERSION 8
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -170,6 +170,33 @@ int pf_write_field(pf_handle* h, int32_t comp, const void* src);
  * comps[]): complex128 interleaved [n_freq][box cells]. Replaces the host
  * accumulation of rustbackend.py:384-397. */
 int pf_dft_read(pf_handle* h, int32_t mon, int32_t slot, double* dst);
+/* Device-resident arrays for zero-copy handoff to torch or another device
+ * library (the survey's pf_field_ptr; the reference hands fields and DFT
+ * planes over through host arrays, rustbackend.py:297-304,384-397). ptr is a
+ * device address, dtype PF_F32 / PF_F64, shape and strides (in elements,
+ * row-major) fill the first ndim entries; stream is the handle's hipStream_t
+ * (as an integer), on which the data was written and which has been
+ * synchronized before the call returns. */
+typedef struct pf_view {
+  void* ptr;
+  int32_t dtype;
+  int32_t ndim;
+  int64_t shape[6];
+  int64_t strides[6];
+  uint64_t stream;
+} pf_view;
+/* Field component comp (0..5: Ex Ey Ez Hx Hy Hz) of the current time level,
+ * shape (nx, ny, nz), strides (ny*zrow, zrow, 1). Valid until the next
+ * pf_advance, pf_write_field, pf_reset or pf_destroy on the handle; writes
+ * through it change the simulation state. PF_E_STATE when the stepper keeps
+ * its rows in padded blocks (62 or 60 nodes per 64 slots), which are not a
+ * strided view. */
+int pf_field_ptr(pf_handle* h, int32_t comp, pf_view* out);
+/* DFT accumulators of monitor mon with every buffered sample folded in:
+ * float64, shape (n_comp, n_freq, bx, by, bz, 2) (real, imaginary; components
+ * in the order of the monitor's comps[]), contiguous. Valid until the next
+ * pf_advance or pf_destroy. */
+int pf_dft_ptr(pf_handle* h, int32_t mon, pf_view* out);
 /* Folds the DFT samples the stepper still buffers into the complex128
  * accumulators of every monitor and waits for it: afterwards the
  * accumulators hold every step advanced so far (the reference accumulates
@@ -213,7 +240,8 @@ int pf_flux_read(pf_handle* h, double* dst);
  * After the forward run reaches the end of the waveform table, pf_adjoint
  * sweeps backwards (recomputing each C-step segment from its checkpoint) with
  * the DFT cotangents as time-reversed sources and writes dJ/dce for every
- * node (float64, nx*ny*nz) to grad_ce. Dispersive media are rejected. */
+ * node (float64, nx*ny*nz) to grad_ce. The cotangent arrays and grad_ce may
+ * be host or device memory. Dispersive media are rejected. */
 int pf_checkpoint_enable(pf_handle* h, int32_t interval);
 int pf_adjoint(pf_handle* h, int32_t n_cot, const pf_cotangent* cot, double* grad_ce);
 /* Batched 2-D stepping (no reference counterpart: the reference runs one

@@ -11,7 +11,7 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 PF_FLAG_GENERIC = 1
 
 PF_F32, PF_F64 = 0, 1
@@ -51,6 +51,11 @@ class PfDesc(C.Structure):
                 ("wave", _P(_f64)), ("n_ade", _i32), ("ade", _P(PfAdeDesc)), ("n_dft", _i32),
                 ("dft", _P(PfDftDesc)), ("n_flux", _i32), ("flux", _P(PfFluxDesc)),
                 ("flags", _i32), ("ce_y", _vp), ("ce_z", _vp)]
+
+
+class PfView(C.Structure):
+    _fields_ = [("ptr", _vp), ("dtype", _i32), ("ndim", _i32), ("shape", _i64 * 6),
+                ("strides", _i64 * 6), ("stream", C.c_uint64)]
 
 
 class PfNnConv(C.Structure):
@@ -106,6 +111,8 @@ SIGNATURES = {
                           C.c_float, _i32, _vp]),
     "pf_nn_to_bf16": (_i32, [_i64, _vp, _vp, _vp]),
     "pf_nn_last_error": (C.c_char_p, []),
+    "pf_field_ptr": (_i32, [_vp, _i32, _P(PfView)]),
+    "pf_dft_ptr": (_i32, [_vp, _i32, _P(PfView)]),
     "pf_batch_dft_read": (_i32, [_vp, _i32, _i32, _i32, _P(_f64)]),
     "pf_batch_read_field": (_i32, [_vp, _i32, _i32, _vp]),
 }
@@ -147,3 +154,70 @@ def check(lib: C.CDLL, status: int, what: str) -> None:
 
 def as_ptr(arr, ctype):
     return arr.ctypes.data_as(_P(ctype))
+
+
+# ---- zero-copy handoff of device arrays to torch (DLPack) -------------------
+# pf_field_ptr / pf_dft_ptr describe a device array of the library; to_torch
+# wraps it in a DLPack capsule, which torch consumes without a copy. The
+# capsule's deleter keeps `owner` (the engine that owns the memory) alive
+# until torch frees the tensor.
+
+
+class _DLDevice(C.Structure):
+    _fields_ = [("device_type", C.c_int32), ("device_id", C.c_int32)]
+
+
+class _DLDataType(C.Structure):
+    _fields_ = [("code", C.c_uint8), ("bits", C.c_uint8), ("lanes", C.c_uint16)]
+
+
+class _DLTensor(C.Structure):
+    _fields_ = [("data", _vp), ("device", _DLDevice), ("ndim", C.c_int32),
+                ("dtype", _DLDataType), ("shape", _P(_i64)), ("strides", _P(_i64)),
+                ("byte_offset", C.c_uint64)]
+
+
+class _DLManagedTensor(C.Structure):
+    pass
+
+
+_DELETER = C.CFUNCTYPE(None, _P(_DLManagedTensor))
+_DLManagedTensor._fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", _vp),
+                             ("deleter", _DELETER)]
+_KDL_ROCM = 10        # DLPack device type of AMD GPUs
+_KDL_FLOAT = 2
+_LIVE = {}            # manager_ctx -> (managed tensor, shape, strides, owner)
+
+
+@_DELETER
+def _dl_delete(ptr):
+    _LIVE.pop(ptr.contents.manager_ctx, None)
+
+
+def to_torch(view: PfView, owner, device_index: int = 0):
+    """A torch tensor aliasing the device array described by ``view`` (no
+    copy). Valid as long as the library keeps the array where it is (see
+    pf_field_ptr / pf_dft_ptr); ``owner`` stays referenced until torch frees
+    the tensor."""
+    import torch
+    from torch.utils import dlpack
+    nd = int(view.ndim)
+    shape = (_i64 * nd)(*view.shape[:nd])
+    strides = (_i64 * nd)(*view.strides[:nd])
+    m = _DLManagedTensor()
+    m.dl_tensor.data = view.ptr
+    m.dl_tensor.device = _DLDevice(_KDL_ROCM, int(device_index))
+    m.dl_tensor.ndim = nd
+    m.dl_tensor.dtype = _DLDataType(_KDL_FLOAT, 32 if view.dtype == PF_F32 else 64, 1)
+    m.dl_tensor.shape = shape
+    m.dl_tensor.strides = strides
+    m.dl_tensor.byte_offset = 0
+    key = id(m)
+    m.manager_ctx = key
+    m.deleter = _dl_delete
+    _LIVE[key] = (m, shape, strides, owner)
+    new_capsule = C.pythonapi.PyCapsule_New
+    new_capsule.restype = C.py_object
+    new_capsule.argtypes = [_vp, C.c_char_p, _vp]
+    cap = new_capsule(C.addressof(m), b"dltensor", None)
+    return dlpack.from_dlpack(cap)

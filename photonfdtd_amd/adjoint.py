@@ -12,7 +12,10 @@ The forward pass is the HIP stepper with field checkpoints every
 hand-written discrete adjoint (``pf_adjoint``): time-reversed transposed Yee +
 CPML steps with the DFT cotangents as sources, each checkpoint segment
 recomputed once. ``FDTDFunction`` wraps both as a ``torch.autograd.Function``
-so the loss can be any torch expression of the DFT tensors.
+so the loss can be any torch expression of the DFT tensors. With ``eps`` on
+the GPU the DFT outputs, their cotangents and the gradient stay on the device
+(``pf_dft_ptr``, device cotangents in ``pf_adjoint``); with ``eps`` on the CPU
+they are host arrays, as in the reference.
 """
 from __future__ import annotations
 
@@ -73,18 +76,33 @@ def _make_function():
             ctx.keys = _dft_keys(sim)
             ctx.save_for_backward(eps)
             ctx.dt = run.dt
+            if eps.is_cuda:
+                # device handoff: a device copy of the accumulators (the
+                # engine's own memory stays with the engine until backward)
+                dev = {n: eng.dft_tensors(n) for n in {k for k, _ in ctx.keys}}
+                return tuple(dev[n][c].clone() for n, c in ctx.keys)
             return tuple(torch.from_numpy(np.ascontiguousarray(res.dft[n][c]))
                          for n, c in ctx.keys)
 
         @staticmethod
         def backward(ctx, *grads):
             (eps,) = ctx.saved_tensors
+            eng = ctx.engine
+            if eps.is_cuda:
+                cot_d: Dict[str, Dict] = {}
+                for (n, c), g in zip(ctx.keys, grads):
+                    if g is not None:
+                        cot_d.setdefault(n, {})[c] = g
+                grad_ce = eng.adjoint_device(cot_d)
+                eng.close()
+                e = eps.detach().to(torch.float64)
+                grad_eps = grad_ce * (-ctx.dt / (e * e * EPS_0))
+                return grad_eps.to(eps.dtype), None, None
             cot: Dict[str, Dict[str, np.ndarray]] = {}
             for (n, c), g in zip(ctx.keys, grads):
                 if g is None:
                     continue
                 cot.setdefault(n, {})[c] = g.detach().cpu().numpy()
-            eng = ctx.engine
             grad_ce = eng.adjoint(cot)
             eng.close()
             e = eps.detach().cpu().numpy()
@@ -108,7 +126,7 @@ def FDTDFunction():
 
 def dft_outputs(sim, eps, checkpoint_interval: int = 100):
     """Differentiable DFT outputs ``{name: {comp: complex tensor}}`` for a torch
-    ``eps`` tensor of the grid's shape."""
+    ``eps`` tensor of the grid's shape, on ``eps``'s device."""
     outs = FDTDFunction().apply(eps, sim, int(checkpoint_interval))
     d: Dict[str, Dict] = {}
     for (n, c), t in zip(_dft_keys(sim), outs):
@@ -116,12 +134,16 @@ def dft_outputs(sim, eps, checkpoint_interval: int = 100):
     return d
 
 
-def value_and_grad_eps(sim, loss: Callable, checkpoint_interval: int = 100):
+def value_and_grad_eps(sim, loss: Callable, checkpoint_interval: int = 100,
+                       device: str = "cpu"):
     """``(value, d value / d eps_r)`` for a scalar ``loss(out)``, ``out`` =
-    ``{'dft': {name: {comp: complex tensor}}}`` (``jaxbackend.py:666-722``)."""
+    ``{'dft': {name: {comp: complex tensor}}}`` (``jaxbackend.py:666-722``).
+    ``device="cuda"`` hands the DFT tensors to ``loss`` on the GPU (no host
+    copies of the outputs, cotangents or gradient until the returned array)."""
     torch = _require_torch()
     _check_sim(sim)
-    eps = torch.tensor(np.asarray(sim.eps_r, dtype=np.float64), requires_grad=True)
+    eps = torch.tensor(np.asarray(sim.eps_r, dtype=np.float64), device=device,
+                       requires_grad=True)
     val = loss({"dft": dft_outputs(sim, eps, checkpoint_interval)})
     val.backward()
     return float(val.detach()), eps.grad.detach().cpu().numpy()

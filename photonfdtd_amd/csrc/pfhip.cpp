@@ -263,6 +263,8 @@ struct pf_handle {
   virtual void timing_read(double* ms, int64_t* launches) = 0;
   virtual void dft_flush() = 0;
   virtual void info(pf_info* out) const = 0;
+  virtual void field_view(int comp, pf_view* out) = 0;
+  virtual void dft_view(int mon, pf_view* out) = 0;
   int64_t shape[3];
 };
 
@@ -297,6 +299,8 @@ class Stepper final : public pf_handle {
   void dft_read(int mon, int slot, double* dst) override;
   void dft_flush() override;
   void info(pf_info* out) const override;
+  void field_view(int comp, pf_view* out) override;
+  void dft_view(int mon, pf_view* out) override;
   void flux_read(double* dst) override;
   void timing_enable(bool on) override;
   void checkpoint_enable(int interval) override;
@@ -484,6 +488,7 @@ class Stepper final : public pf_handle {
   size_t psi_len_e_ = 0;
   int64_t steps_done_ = 0;
   void build_march_tiles(int xchunk);
+  void check_march_tiles(const std::vector<pfhip::MarchTile>& t, int nw) const;
   int slab_e_[3][2], slab_h_[3][2];
   SourceGroupsDev src_h_, src_e_;
   std::vector<Ade> ade_;
@@ -999,6 +1004,7 @@ void Stepper<real>::build_march_tiles(int xchunk) {
     }
   }
   for (int q = 0; q < NLIST; ++q) {
+    check_march_tiles(lists[q], q == 0 ? march_nw_ : pml_nw_);
     tile_cells_[q] = 0;
     for (const pfhip::MarchTile& t : lists[q]) tile_cells_[q] += (int64_t)(t.i1 - t.i0) * t.jn * t.kn;
     std::vector<pfhip::MarchTile> ordered =
@@ -1461,7 +1467,23 @@ std::vector<pfhip::MarchTile> Stepper<real>::march_tile_list(int nw, int xchunk,
           out.push_back({x, std::min(sx.start + sx.count, x + px), sy.start, sy.count, sz.start,
                          sz.count, 0, 0});
       }
+  check_march_tiles(out, nw);
   return out;
+}
+
+// The single-step march kernels assume these tile bounds (their lane and
+// wave roles, the LDS plane table and the z block of each layout); a tile
+// outside them is a host bug and never reaches the device.
+template <typename real>
+void Stepper<real>::check_march_tiles(const std::vector<pfhip::MarchTile>& tiles, int nw) const {
+  const int nx = args_.nx, ny = args_.ny, nz = args_.nz;
+  const int zmax = zpad_ ? zpad_ : 62;
+  for (const pfhip::MarchTile& t : tiles) {
+    const bool ok = t.i0 >= 0 && t.i1 > t.i0 && t.i1 <= nx && t.i1 - t.i0 <= pfhip::MARCH_MAX_XCHUNK &&
+                    t.j0 >= 0 && t.jn >= 1 && t.jn <= nw - 2 && t.j0 + t.jn <= ny && t.k0 >= 0 &&
+                    t.kn >= 1 && t.kn <= zmax && t.k0 + t.kn <= nz && (!zpad_ || t.k0 % zpad_ == 0);
+    if (!ok) throw HipError{PF_E_STATE, "march tile outside the kernel's bounds"};
+  }
 }
 
 template <typename real>
@@ -1732,10 +1754,10 @@ void Stepper<real>::checkpoint_enable(int interval) {
 }
 
 template <typename real>
-void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_host) {
+void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out) {
   if (ckpt_interval_ <= 0) throw ArgError{"pf_checkpoint_enable was not called before the forward run"};
   if (steps_done_ != n_steps_) throw ArgError{"run the forward pass to the last step first"};
-  if (!grad_host) throw ArgError{"null gradient destination"};
+  if (!grad_out) throw ArgError{"null gradient destination"};
   const int64_t ncell = shape[0] * shape[1] * shape[2];
   const size_t nf = 6 * (size_t)ncell;
   const size_t elems = nf + psi_len_e_ + psi_len_h_;
@@ -1751,7 +1773,7 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_hos
       throw ArgError{"bad cotangent"};
     const Dft& m = dft_[g.mon];
     const size_t n = (size_t)m.n_freq * m.nbox * 2;
-    copy_sync(cot_dev[g.mon] + (size_t)g.slot * n, g.g, n * sizeof(double), hipMemcpyHostToDevice,
+    copy_sync(cot_dev[g.mon] + (size_t)g.slot * n, g.g, n * sizeof(double), hipMemcpyDefault,
               stream_);
   }
   real* lam = scratch.zeros<real>(elems, stream_);
@@ -1863,7 +1885,7 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_hos
     }
   }
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(grad_host, grad, (size_t)ncell * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipMemcpyAsync(grad_out, grad, (size_t)ncell * sizeof(double), hipMemcpyDefault, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
   scratch.release();
 }
@@ -2044,6 +2066,44 @@ void Stepper<real>::dft_read(int mon, int slot, double* dst) {
   HIP_CHECK(hipMemcpyAsync(dst, m.acc + (size_t)slot * n, n * sizeof(double),
                            hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+template <typename real>
+void Stepper<real>::field_view(int comp, pf_view* out) {
+  if (comp < 0 || comp > 5) throw ArgError{"field component out of range"};
+  if (zpad_ && zpad_ != pfhip::ZPAD_BLOCK)
+    throw HipError{PF_E_STATE, "fields are kept in padded 64-slot blocks; use pf_read_field"};
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  std::memset(out, 0, sizeof(*out));
+  out->ptr = f_[fused_ ? cur_ : 0][comp];
+  out->dtype = sizeof(real) == 4 ? PF_F32 : PF_F64;
+  out->ndim = 3;
+  for (int a = 0; a < 3; ++a) out->shape[a] = shape[a];
+  out->strides[0] = shape[1] * zrow_;
+  out->strides[1] = zrow_;
+  out->strides[2] = 1;
+  out->stream = (uint64_t)(uintptr_t)stream_;
+}
+
+template <typename real>
+void Stepper<real>::dft_view(int mon, pf_view* out) {
+  if (mon < 0 || mon >= (int)dft_.size()) throw ArgError{"DFT monitor index out of range"};
+  Dft& m = dft_[mon];
+  fold(m);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  std::memset(out, 0, sizeof(*out));
+  out->ptr = m.acc;
+  out->dtype = PF_F64;
+  out->ndim = 6;
+  const int64_t dims[6] = {m.n_comp, m.n_freq, m.box.count[0], m.box.count[1], m.box.count[2], 2};
+  int64_t stride = 1;
+  for (int a = 5; a >= 0; --a) {
+    out->shape[a] = dims[a];
+    out->strides[a] = stride;
+    stride *= dims[a];
+  }
+  out->stream = (uint64_t)(uintptr_t)stream_;
 }
 
 template <typename real>
@@ -2616,6 +2676,16 @@ int pf_dft_read(pf_handle* h, int32_t mon, int32_t slot, double* dst) {
 int pf_stepper_info(pf_handle* h, pf_info* out) {
   if (!h || !out) return fail(PF_E_ARG, "null argument");
   return guarded([&] { h->info(out); });
+}
+
+int pf_field_ptr(pf_handle* h, int32_t comp, pf_view* out) {
+  if (!h || !out) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { h->field_view(comp, out); });
+}
+
+int pf_dft_ptr(pf_handle* h, int32_t mon, pf_view* out) {
+  if (!h || !out) return fail(PF_E_ARG, "null argument");
+  return guarded([&] { h->dft_view(mon, out); });
 }
 
 int pf_dft_flush(pf_handle* h) {

@@ -395,6 +395,29 @@ class HipEngine:
             out[c] = buf
         return out
 
+    def field_tensor(self, comp: str):
+        """The current level of field ``comp`` as a torch tensor on the device
+        that aliases the stepper's memory (``pf_field_ptr``; no copy). It is
+        valid until the next advance, ``set`` or ``close``; writes through it
+        change the simulation state."""
+        import torch
+        v = N.PfView()
+        N.check(self.lib, self.lib.pf_field_ptr(self.h, N.COMP_CODE[comp], C.byref(v)),
+                "pf_field_ptr")
+        return N.to_torch(v, self, torch.cuda.current_device())
+
+    def dft_tensors(self, name: str):
+        """``{comp: complex128 tensor (n_freq, *box)}`` on the device, aliasing
+        the accumulators of DFT monitor ``name`` after every buffered sample
+        is folded in (``pf_dft_ptr``; no copy). Valid until the next advance
+        or ``close``."""
+        import torch
+        mon, comps, count, n_freq = self.dft_slots[name]
+        v = N.PfView()
+        N.check(self.lib, self.lib.pf_dft_ptr(self.h, mon, C.byref(v)), "pf_dft_ptr")
+        acc = torch.view_as_complex(N.to_torch(v, self, torch.cuda.current_device()))
+        return {c: acc[slot].reshape((n_freq,) + tuple(count)) for slot, c in enumerate(comps)}
+
     def enable_checkpoints(self, interval: int) -> None:
         N.check(self.lib, self.lib.pf_checkpoint_enable(self.h, int(interval)),
                 "pf_checkpoint_enable")
@@ -417,6 +440,29 @@ class HipEngine:
         grad = np.zeros(self.shape, dtype=np.float64)
         N.check(self.lib, self.lib.pf_adjoint(self.h, len(items), cot, N.as_ptr(grad, C.c_double)),
                 "pf_adjoint")
+        return grad
+
+    def adjoint_device(self, cotangents):
+        """``adjoint`` with the cotangents given as torch tensors on the
+        device; returns dJ/dce as a float64 device tensor (no host copies)."""
+        import torch
+        keep, items = [], []
+        for name, comps in cotangents.items():
+            mon, names, count, n_freq = self.dft_slots[name]
+            for c, g in comps.items():
+                t = torch.view_as_real(g.to(torch.complex128).reshape((n_freq,) + tuple(count))
+                                       .contiguous())
+                keep.append(t)
+                items.append((mon, names.index(c), t))
+        cot = (N.PfCotangent * max(len(items), 1))()
+        for n_, (mon, slot, t) in enumerate(items):
+            cot[n_].mon, cot[n_].slot = mon, slot
+            cot[n_].g = C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double))
+        grad = torch.zeros(self.shape, dtype=torch.float64, device=keep[0].device if keep else "cuda")
+        torch.cuda.synchronize()
+        N.check(self.lib, self.lib.pf_adjoint(self.h, len(items), cot,
+                                              C.cast(C.c_void_p(grad.data_ptr()),
+                                                     C.POINTER(C.c_double))), "pf_adjoint")
         return grad
 
     def flux(self) -> Dict[str, float]:
