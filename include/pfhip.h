@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 8
+#define PFHIP_ABI_VERSION 9
 
 enum pf_dtype { PF_F32 = 0, PF_F64 = 1 };
 enum pf_comp { PF_EX = 0, PF_EY = 1, PF_EZ = 2, PF_HX = 3, PF_HY = 4, PF_HZ = 5 };
@@ -116,11 +116,23 @@ typedef struct pf_desc {
    * Such a handle steps with the generic two-pass kernels. */
   const void* ce_y;
   const void* ce_z;
+  /* Beyond-HBM stepping (PF_FLAG_STREAM): steps per tile visit (0: 32) and
+   * owned x planes per tile (0: sized from the free device memory). */
+  int32_t t_block;
+  int32_t tile_rows;
 } pf_desc;
 
 /* pf_desc.flags: use the in-place two-pass kernels only (no fused ping-pong
  * step); required for pf_checkpoint_enable / pf_adjoint. */
 #define PF_FLAG_GENERIC 1
+/* Keep the domain in pinned host memory and stream x-slab tiles through the
+ * device, t_block steps per visit (the reference's StreamingCudaStepper,
+ * rust/src/cuda.rs:576-1116, picked by rustbackend.py:264-290 when the run
+ * does not fit the device). Same bits as the resident stepper. Supported:
+ * CPML, soft sources, DFT monitors, field reads/writes; not: dispersive
+ * media, flux monitors, subpixel coefficients, checkpoints / adjoint,
+ * pf_field_ptr (the fields are not on the device). */
+#define PF_FLAG_STREAM 2
 
 /* Cotangent of one DFT accumulator: g = dJ/d(Re F) + i dJ/d(Im F) for monitor
  * `mon`, component slot `slot`, complex128 interleaved [n_freq][box cells]. */
@@ -206,7 +218,8 @@ int pf_dft_flush(pf_handle* h);
 /* How the handle steps (no reference counterpart: the reference's CUDA
  * stepper has one kernel pair). path: 0 generic two-pass kernels, 1 fused
  * one-launch step on dense rows, 2 the 3-D march, 3 the march with the
- * two-step launch. zpad: nodes per 64-slot padded z block (0: dense rows;
+ * two-step launch, 4 beyond-HBM streaming of x-slab tiles (tiles_bulk: the
+ * number of tiles). zpad: nodes per 64-slot padded z block (0: dense rows;
  * 64: rows of the 64-lane march, node k in slot k), zrow: slots per row.
  * tiles_* / cells_*: march tiles (and the cells they own) that lie
  * off every CPML slab (bulk, the psi-free kernel) and that touch one (cpml).

@@ -11,8 +11,9 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
-ABI_VERSION = 8
+ABI_VERSION = 9
 PF_FLAG_GENERIC = 1
+PF_FLAG_STREAM = 2
 
 PF_F32, PF_F64 = 0, 1
 KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux", "yee_pair", "dft_fold")
@@ -50,7 +51,8 @@ class PfDesc(C.Structure):
                 ("src_amp", _P(_f64)), ("src_wave", _P(_i32)), ("n_wave", _i32),
                 ("wave", _P(_f64)), ("n_ade", _i32), ("ade", _P(PfAdeDesc)), ("n_dft", _i32),
                 ("dft", _P(PfDftDesc)), ("n_flux", _i32), ("flux", _P(PfFluxDesc)),
-                ("flags", _i32), ("ce_y", _vp), ("ce_z", _vp)]
+                ("flags", _i32), ("ce_y", _vp), ("ce_z", _vp), ("t_block", _i32),
+                ("tile_rows", _i32)]
 
 
 class PfView(C.Structure):

@@ -23,7 +23,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", f"-I{PKG.parent / 'include'}"]
 
-SOURCES = ["kernels_generic.hip", "yee3d_fused.hip", "yee3d_march64.hip", "yee3d_march2.hip", "adjoint.hip", "batch2d.hip", "nn_mfma.hip", "monitor_q.hip", "pfhip.cpp"]
+SOURCES = ["kernels_generic.hip", "yee3d_fused.hip", "yee3d_march64.hip", "yee3d_march2.hip", "adjoint.hip", "batch2d.hip", "nn_mfma.hip", "monitor_q.hip", "pfhip.cpp", "stream_stepper.cpp"]
 
 
 def hipcc() -> str:

@@ -325,11 +325,12 @@ __global__ void k_ade_apply(real* const* __restrict__ fields, const int64_t* __r
 }
 
 // Copy a strided box of one or more components into a dense buffer:
-// out[(c * nbox) + flat box index] = field_c[node].
+// out[(c * cstride) + flat box index] = field_c[node] (cstride = the box's
+// node count, or that of a larger box whose x slab this one is).
 template <typename real>
 __global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __restrict__ comps,
                              int n_comp, BoxDev box, int64_t plane, int row, int zpad,
-                             real* __restrict__ out) {
+                             int64_t cstride, real* __restrict__ out) {
   // blockIdx.y = component (its field pointer is one scalar load per wave);
   // x covers the box's nodes, z fastest
   const int c = blockIdx.y;
@@ -353,7 +354,7 @@ __global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __
   const int64_t node = (box.start[0] + bi * box.stride[0]) * plane +
                        (box.start[1] + bj * box.stride[1]) * row +
                        zslot(box.start[2] + bk * box.stride[2], zpad);
-  out[(int64_t)c * nbox + r] = F[node];
+  out[(int64_t)c * cstride + r] = F[node];
 }
 
 // Fold `count` buffered samples into the complex128 accumulators, one sample
@@ -491,11 +492,13 @@ void launch_ade_apply(real* const* fields_dev, const int64_t* cells, int64_t n, 
 
 template <typename real>
 void launch_gather_box(real* const* fields_dev, const int32_t* comps, int n_comp, const BoxDev& box,
-                       int64_t plane, int row, int zpad, real* out, hipStream_t s) {
+                       int64_t plane, int row, int zpad, real* out, hipStream_t s,
+                       int64_t cstride) {
   const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
   if (nbox <= 0 || n_comp <= 0) return;
   const dim3 grid(blocks_for(nbox, 256), (unsigned)n_comp);
-  k_gather_box<real><<<grid, 256, 0, s>>>(fields_dev, comps, n_comp, box, plane, row, zpad, out);
+  k_gather_box<real><<<grid, 256, 0, s>>>(fields_dev, comps, n_comp, box, plane, row, zpad,
+                                          cstride > 0 ? cstride : nbox, out);
 }
 
 template <typename real>
@@ -531,7 +534,7 @@ void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int 
   template void launch_ade_apply<real>(real* const*, const int64_t*, int64_t, const real*, real*,  \
                                        real*, const real*, int, real, hipStream_t);                \
   template void launch_gather_box<real>(real* const*, const int32_t*, int, const BoxDev&, int64_t, \
-                                        int, int, real*, hipStream_t);                             \
+                                        int, int, real*, hipStream_t, int64_t);                    \
   template void launch_dft_fold<real>(const real*, int64_t, int, const int32_t*, int, int64_t,     \
                                       const double*, const double*, int, double, double*,          \
                                       hipStream_t);                                                \

@@ -23,6 +23,7 @@
 
 #include "../../include/pfhip.h"
 #include "pfhip_internal.h"
+#include "pfhip_host.h"
 
 namespace {
 
@@ -33,74 +34,10 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-struct HipError {
-  int code;
-  std::string msg;
-};
-
-#define HIP_CHECK(expr)                                                                    \
-  do {                                                                                     \
-    hipError_t _e = (expr);                                                                \
-    if (_e != hipSuccess)                                                                  \
-      throw HipError{PF_E_HIP, std::string("HIP error: ") + hipGetErrorString(_e) + " (" + \
-                                   #expr + ")"};                                           \
-  } while (0)
-
-struct ArgError {
-  std::string msg;
-};
-
-// Host <-> device copy ordered on stream s (after the work already queued
-// there, such as the clearing of a fresh allocation) that returns once it is
-// complete, so the host buffer may be a temporary. The handles' streams are
-// non-blocking, so a plain hipMemcpy on the null stream is not ordered
-// against them.
-void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
-  if (bytes == 0) return;
-  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-}
-
-// Device allocations of one handle, freed together.
-class DeviceArena {
- public:
-  ~DeviceArena() { release(); }
-  void* alloc(size_t bytes) {
-    if (bytes == 0) return nullptr;
-    void* p = nullptr;
-    HIP_CHECK(hipMalloc(&p, bytes));
-    ptrs_.push_back(p);
-    return p;
-  }
-  template <typename T>
-  T* upload(const T* host, size_t n, hipStream_t s) {
-    if (n == 0) return nullptr;
-    T* d = static_cast<T*>(alloc(n * sizeof(T)));
-    copy_sync(d, host, n * sizeof(T), hipMemcpyHostToDevice, s);
-    return d;
-  }
-  template <typename T>
-  T* zeros(size_t n, hipStream_t s) {
-    if (n == 0) return nullptr;
-    T* d = static_cast<T*>(alloc(n * sizeof(T)));
-    HIP_CHECK(hipMemsetAsync(d, 0, n * sizeof(T), s));
-    return d;
-  }
-  void release() {
-    for (void* p : ptrs_) (void)hipFree(p);
-    ptrs_.clear();
-  }
-  // Free one allocation of this arena (no work may still use it).
-  void free(void* p) {
-    auto it = std::find(ptrs_.begin(), ptrs_.end(), p);
-    if (it == ptrs_.end()) return;
-    HIP_CHECK(hipFree(p));
-    ptrs_.erase(it);
-  }
-
- private:
-  std::vector<void*> ptrs_;
-};
+using pfhost::ArgError;
+using pfhost::DeviceArena;
+using pfhost::HipError;
+using pfhost::copy_sync;
 
 // True when the nodes with map >= 0 are exactly [0, lo) U [hi, n) with the
 // compact index running in node order; stores (lo, hi).
@@ -234,39 +171,9 @@ using pfhip::BoxDev;
 using pfhip::SourceGroupsDev;
 using pfhip::StepArgs;
 
-BoxDev check_box(const pf_box& b, const int64_t shape[3]) {
-  BoxDev out;
-  for (int a = 0; a < 3; ++a) {
-    if (b.count[a] < 1 || b.stride[a] < 1 || b.start[a] < 0 ||
-        b.start[a] + (b.count[a] - 1) * b.stride[a] >= shape[a])
-      throw ArgError{"box exceeds the grid on axis " + std::to_string(a)};
-    out.start[a] = b.start[a];
-    out.count[a] = b.count[a];
-    out.stride[a] = b.stride[a];
-  }
-  return out;
-}
+using pfhost::check_box;
 
 }  // namespace
-
-struct pf_handle {
-  virtual ~pf_handle() = default;
-  virtual void advance(int64_t step0, int64_t n) = 0;
-  virtual void read_box(int comp, const pf_box& box, void* dst) = 0;
-  virtual void read_box_q(int comp, const pf_box& box, int bits, void* dst, double* scale) = 0;
-  virtual void write_field(int comp, const void* src) = 0;
-  virtual void dft_read(int mon, int slot, double* dst) = 0;
-  virtual void flux_read(double* dst) = 0;
-  virtual void timing_enable(bool on) = 0;
-  virtual void checkpoint_enable(int interval) = 0;
-  virtual void adjoint(int n_cot, const pf_cotangent* cot, double* grad) = 0;
-  virtual void timing_read(double* ms, int64_t* launches) = 0;
-  virtual void dft_flush() = 0;
-  virtual void info(pf_info* out) const = 0;
-  virtual void field_view(int comp, pf_view* out) = 0;
-  virtual void dft_view(int mon, pf_view* out) = 0;
-  int64_t shape[3];
-};
 
 namespace {
 
@@ -953,8 +860,7 @@ void Stepper<real>::build_march_tiles(int xchunk) {
       seg[a] = split_ ? split_axis(n[a], b0, b1, len[a], a > 0) : split_axis(n[a], 0, 0, len[a], false);
     }
   }
-  std::vector<pfhip::MarchTile> lists[NLIST];
-  std::vector<pfhip::MarchTile>& bulk = lists[0];
+  std::vector<pfhip::MarchTile> lists[NLIST];   // [0] psi-free, [1] several CPML axes, ...
   std::vector<pfhip::MarchTile>& pml = lists[1];
   for (const Seg& sx : seg[0])
     for (const Seg& sy : seg[1])
@@ -2626,7 +2532,8 @@ int pf_create(const pf_desc* desc, pf_handle** out) {
   if (!desc || !out) return fail(PF_E_ARG, "null argument");
   *out = nullptr;
   return guarded([&] {
-    if (desc->dtype == PF_F32) *out = new Stepper<float>(*desc);
+    if (desc->flags & PF_FLAG_STREAM) *out = pfhost::make_streamer(*desc);
+    else if (desc->dtype == PF_F32) *out = new Stepper<float>(*desc);
     else if (desc->dtype == PF_F64) *out = new Stepper<double>(*desc);
     else throw ArgError{"dtype must be PF_F32 or PF_F64"};
   });

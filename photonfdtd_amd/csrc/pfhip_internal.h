@@ -468,7 +468,8 @@ void launch_ade_apply(real* const* fields_dev, const int64_t* cells, int64_t n, 
 // plane / row: field strides in elements (ny * row, row); zpad: padded rows.
 template <typename real>
 void launch_gather_box(real* const* fields_dev, const int32_t* comps, int n_comp, const BoxDev& box,
-                       int64_t plane, int row, int zpad, real* out, hipStream_t s);
+                       int64_t plane, int row, int zpad, real* out, hipStream_t s,
+                       int64_t cstride = 0);
 template <typename real>
 void launch_dft_fold(const real* ring, int64_t nbox, int n_comp, const int32_t* is_h, int count,
                      int64_t sample0, const double* phase_e, const double* phase_h, int n_freq,

@@ -311,14 +311,44 @@ def build_desc(sim, npdt, exact: bool, flags: int = 0):
     return d, keep, t, dft_slots, flux_names
 
 
-class HipEngine:
-    """One device-resident simulation behind the C ABI."""
+def streaming_mode(sim, npdt) -> bool:
+    """Whether ``run_hip`` keeps the domain in host memory and streams x-slab
+    tiles through the device (``rustbackend.py:264-269``): forced by
+    ``PHOTONFDTD_HIP_STREAM=1``, refused by ``=0``, otherwise chosen when the
+    resident stepper (two field levels, ce and its march copy, psi) would
+    take more than 85 % of the free device memory."""
+    env = os.environ.get("PHOTONFDTD_HIP_STREAM")
+    if env in ("0", "1"):
+        return env == "1"
+    nx, ny, nz = sim.grid.shape
+    need_mb = 15.0 * nx * ny * nz * np.dtype(npdt).itemsize / 1e6
+    _, free_mb, _ = device_info()
+    return need_mb > 0.85 * free_mb
 
-    def __init__(self, sim, npdt, exact: bool, flags: int = 0):
+
+class HipEngine:
+    """One simulation behind the C ABI: device-resident, or with
+    ``streaming`` host-resident and streamed through the device in x-slab
+    tiles (``PHOTONFDTD_HIP_TB`` steps per visit, ``PHOTONFDTD_HIP_TILE``
+    planes per tile)."""
+
+    def __init__(self, sim, npdt, exact: bool, flags: int = 0, streaming: bool = False):
         self.lib = N.load_library()
         self.npdt = np.dtype(npdt).type
         self.shape = sim.grid.shape
+        self.streaming = bool(streaming)
+        if self.streaming:
+            flags |= N.PF_FLAG_STREAM
         d, keep, self.t, self.dft_slots, self.flux_names = build_desc(sim, self.npdt, exact, flags)
+        if self.streaming:
+            what = ("dispersive media" if d.n_ade else "flux monitors" if d.n_flux else
+                    "subpixel smoothing" if d.ce_y else None)
+            if what:
+                raise NotImplementedError(
+                    f"{what} need the device-resident HIP stepper, and this run streams its "
+                    "domain through the device (set PHOTONFDTD_HIP_STREAM=0 to keep it resident)")
+            d.t_block = int(os.environ.get("PHOTONFDTD_HIP_TB", "32"))
+            d.tile_rows = int(os.environ.get("PHOTONFDTD_HIP_TILE", "0"))
 
         h = C.c_void_p()
         N.check(self.lib, self.lib.pf_create(C.byref(d), C.byref(h)), "pf_create")
@@ -577,11 +607,13 @@ def run_loop(sim, engine: HipEngine) -> Result:
 def run_hip(sim) -> Result:
     """Run ``sim`` on the gfx950 stepper and return a ``Result``."""
     npdt = validate(sim)
-    engine = HipEngine(sim, npdt, exact_mode(npdt))
+    streaming = streaming_mode(sim, npdt)
+    engine = HipEngine(sim, npdt, exact_mode(npdt), streaming=streaming)
     if sim.verbose:
         name, free_mb, total_mb = device_info()
-        print(f"[photonfdtd_amd] HIP backend, device: {name} ({free_mb}/{total_mb} MB free)",
-              file=sys.stderr, flush=True)
+        mode = "streaming (host-resident)" if streaming else "device-resident"
+        print(f"[photonfdtd_amd] HIP backend ({mode}), device: {name} "
+              f"({free_mb}/{total_mb} MB free)", file=sys.stderr, flush=True)
     try:
         return run_loop(sim, engine)
     finally:
