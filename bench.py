@@ -69,6 +69,19 @@ def dispersive_slab(pf, n=512, steps=2000, precision="float32", backend="hip"):
                          backend=backend)
 
 
+def dielectric_slab(pf, n=512, steps=128, precision="float32", backend="hip"):
+    """The slab scene without dispersion (the silicon box as eps 12.1, no gold
+    cube): the workload of the beyond-HBM streaming stepper, which does not
+    step ADE media."""
+    sim = dispersive_slab(pf, n, steps, precision, backend)
+    sim.structures = [sim.structures[0],
+                      pf.Box(center=sim.structures[1].center, size=sim.structures[1].size,
+                             medium=pf.Medium(12.1))]
+    return type(sim)(sim.grid, structures=sim.structures, sources=sim.sources,
+                     monitors=sim.monitors, run_time=sim.run_time, courant=COURANT,
+                     precision=precision, backend=backend)
+
+
 def batched_2d_scenes(pf, n_sims=64, lo=256, hi=1024, steps=4000, precision="float32", seed=0,
                       backend="hip", component="Ex"):
     """Independent 2-D scenes of uneven size for batched stepping.
@@ -512,6 +525,58 @@ def batched_3d_scenes(pf, n_sims=16, lo=64, hi=128, steps=220, precision="float3
     return out
 
 
+def bench_stream3d(args):
+    """Beyond-HBM streaming: the dielectric slab at n^3 kept in pinned host
+    memory, x-slab tiles of --tile-rows planes with a 32-plane halo advanced
+    32 steps per visit; against the device-resident stepper on the same scene
+    (same bits required)."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.hip_backend import HipEngine, device_info, exact_mode
+    os.environ.setdefault("PHOTONFDTD_HIP_TB", "32")
+    os.environ["PHOTONFDTD_HIP_TILE"] = str(args.tile_rows)
+    total = args.warmup + args.steps
+    npdt = np.float32 if args.precision == "float32" else np.float64
+    out = {}
+    for mode in ("streaming", "resident"):
+        sim = dielectric_slab(pf, args.n, total, args.precision)
+        eng = HipEngine(sim, npdt, exact_mode(npdt), streaming=mode == "streaming")
+        eng.advance(0, args.warmup)
+        t0 = time.perf_counter()
+        eng.advance(args.warmup, args.steps)
+        eng.dft_flush()
+        wall = time.perf_counter() - t0
+        info = eng.info()
+        dft = eng.dft("out")
+        eng.close()
+        out[mode] = dict(wall=wall, info=info, dft=dft)
+        print(f"[bench] {mode}: {wall:.2f} s for {args.steps} steps", file=sys.stderr, flush=True)
+    cells = args.n ** 3
+    item = np.dtype(npdt).itemsize
+    T = int(os.environ["PHOTONFDTD_HIP_TB"])
+    tiles = out["streaming"]["info"]["tiles_bulk"]
+    # per chunk of T steps: every tile uploads its planes plus halo (six
+    # fields and ce) and downloads its owned planes (six fields); the CPML
+    # psi rows, a few percent more, are left out
+    planes_up = args.n + 2 * T * (tiles - 1)
+    link = (planes_up * 7 + args.n * 6) * args.n * args.n * item / T
+    same = all(np.array_equal(out["streaming"]["dft"][c], out["resident"]["dft"][c])
+               for c in out["resident"]["dft"])
+    value = cells * args.steps / out["streaming"]["wall"] / 1e9
+    line = {"metric": "Gcell-updates/s (beyond-HBM streaming)", "value": round(value, 3),
+            "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(out["streaming"]["wall"] / args.steps * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32" if npdt == np.float32 else "f64", "data": "synthetic",
+            "config": {"workload": f"3-D dielectric slab {args.n}^3 streamed from pinned host "
+                                   f"memory, {args.tile_rows}-plane tiles, {T}-step visits",
+                       "tiles": tiles, "t_block": T, "device": device_info()[0]},
+            "host_link": {"bytes_per_step": round(link),
+                          "GB_per_s": round(link * args.steps / out["streaming"]["wall"] / 1e9, 2)},
+            "resident_value": round(cells * args.steps / out["resident"]["wall"] / 1e9, 3),
+            "dft_bitwise_equal_to_resident": bool(same)}
+    print(json.dumps(line), flush=True)
+
+
 def bench_batched3d(args):
     """Batched 3-D workload: 16 uneven scenes of 64-128 nodes per axis in one
     batch, against the same scenes stepped one after another."""
@@ -831,10 +896,13 @@ def main():
     ap.add_argument("--rounds", type=int, default=6, help="surrogate design rounds")
     ap.add_argument("--fd-voxels", type=int, default=20, help="adjoint3d: central-difference voxels")
     ap.add_argument("--workload", default="slab3d",
-                    choices=("slab3d", "batched2d", "batched3d", "surrogate", "adjoint3d"),
+                    choices=("slab3d", "batched2d", "batched3d", "surrogate", "adjoint3d",
+                             "stream3d"),
                     help="slab3d: the 3-D dispersive slab (headline); batched2d: 64 uneven "
                          "2-D TE scenes in one launch; batched3d: 16 uneven 3-D scenes in one "
                          "batch")
+    ap.add_argument("--tile-rows", type=int, default=64,
+                    help="stream3d: owned x planes per streamed tile")
     ap.add_argument("--batch-config", default=None, metavar="MODE,WAVES,ROWS",
                     help="fixed batched launch setting instead of the autotuner")
     ap.add_argument("--batch-modes", default="graph,stream,persistent",
@@ -898,6 +966,9 @@ def main():
         return
     if args.workload == "adjoint3d":
         bench_adjoint3d(args)
+        return
+    if args.workload == "stream3d":
+        bench_stream3d(args)
         return
 
     import photonfdtd_amd as pf
