@@ -621,10 +621,6 @@ def bench_batched3d(args):
         "batch_device_ms_per_step": round(dev_ms / args.steps, 4),
         "single_runs_ms_per_step": round(single / args.steps * 1e3, 4),
         "speedup_vs_single_runs": round(single / wall, 3),
-        "march": {"psi_free_cell_fraction": round(info["cells_bulk"] / cells, 4),
-                  "tiles_psi_free": info["tiles_bulk"], "tiles_cpml": info["tiles_cpml"],
-                  "cpml_lists": info["cpml_lists"], "zrow": info["zrow"],
-                  "dft_ring": info["dft_ring"]},
         "dft_finite": bool(finite),
     }
     print(json.dumps(out), flush=True)

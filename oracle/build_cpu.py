@@ -15,7 +15,9 @@ OUT = HERE / "libpfcpu.so"
 
 def build(verbose: bool = False) -> Path:
     cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
-    tmp = OUT.with_suffix(".so.tmp")
+    # a private temporary per process, then an atomic rename: parallel test
+    # workers building at once never see a half-written library
+    tmp = OUT.with_name(f".{OUT.name}.{os.getpid()}.tmp")
     cmd = [cxx, "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
            "-march=x86-64-v3", f"-I{HERE.parent / 'include'}", str(HERE / "cpu_fdtd.cpp"),
            "-o", str(tmp)]

@@ -130,7 +130,8 @@ class SimBatch:
     def configure(self, mode: str, waves: int = 4, rows: int = 64) -> None:
         """Launch mode, wavefronts per workgroup (1, 2, 4, 8) and owned rows
         per wavefront chunk; for 3-D: "stream", march wavefronts per
-        workgroup (10, 12, 16) and x planes per tile."""
+        workgroup (10, 12, 16) and x planes per tile; 3-D runs in "stream" or
+        "graph" mode (pairs of steps replayed from a captured hipGraph)."""
         if mode not in MODES:
             raise ValueError(f"mode must be one of {sorted(MODES)}")
         N.check(self.lib, self.lib.pf_batch_configure(self.h, MODES[mode], int(waves), int(rows)),
@@ -158,7 +159,7 @@ class SimBatch:
         chunk) candidate over ``trial_steps`` steps from a reset state with
         HIP events, keep the fastest, and reset the batch to step 0."""
         if self.is3d:
-            d_modes, d_waves, d_rows = ("stream",), (10, 12, 16), (32, 64, 128)
+            d_modes, d_waves, d_rows = ("stream", "graph"), (10, 12, 16), (32, 64, 128)
         else:
             d_modes, d_waves, d_rows = ("graph", "stream", "persistent"), (2, 4, 8), (16, 32, 64, 128)
         modes = d_modes if modes is None else modes

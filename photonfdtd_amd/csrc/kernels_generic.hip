@@ -234,24 +234,39 @@ __global__ void k_inject(real* const* __restrict__ fields, const int32_t* __rest
 }
 
 // Batched sources: job blockIdx.y, group blockIdx.x * blockDim.x + lane.
+// The step is `step`, or *step_dev when that is set (launches captured in a
+// graph read the step from a device counter that k_step_tick advances).
 template <typename real>
-__global__ void k_src_jobs(const SrcJob<real>* __restrict__ jobs, int64_t step) {
+__global__ void k_src_jobs(const SrcJob<real>* __restrict__ jobs, int64_t step,
+                           const int64_t* __restrict__ step_dev) {
   const SrcJob<real>& J = jobs[blockIdx.y];
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= J.g.n_groups) return;
-  inject_group<real>(J.fields, J.g, J.n_steps, step, g);
+  inject_group<real>(J.fields, J.g, J.n_steps, step_dev ? *step_dev : step, g);
 }
 
 template <typename real>
-void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64_t step, hipStream_t s) {
+void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64_t step, hipStream_t s,
+                     const int64_t* step_dev) {
   if (n_jobs <= 0 || max_groups <= 0) return;
   dim3 grid((unsigned)((max_groups + 63) / 64), (unsigned)n_jobs);
-  k_src_jobs<real><<<grid, 64, 0, s>>>(jobs, step);
+  k_src_jobs<real><<<grid, 64, 0, s>>>(jobs, step, step_dev);
 }
+
+__global__ void k_step_tick(int64_t* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step = *step + 1;
+}
+__global__ void k_step_set(int64_t* step, int64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step = v;
+}
+void launch_step_tick(int64_t* step, hipStream_t s) { k_step_tick<<<1, 64, 0, s>>>(step); }
+void launch_step_set(int64_t* step, int64_t v, hipStream_t s) { k_step_set<<<1, 64, 0, s>>>(step, v); }
 // Batched DFT capture: job blockIdx.y, element blockIdx.x * blockDim.x +
 // lane of [comp][box node], the element order of k_gather_box.
 template <typename real>
-__global__ void k_dft_jobs(const DftJob<real>* __restrict__ jobs, int64_t step) {
+__global__ void k_dft_jobs(const DftJob<real>* __restrict__ jobs, int64_t step_arg,
+                           const int64_t* __restrict__ step_dev) {
+  const int64_t step = step_dev ? *step_dev : step_arg;
   const DftJob<real>& J = jobs[blockIdx.y];
   if (step % J.interval != 0) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -272,16 +287,20 @@ __global__ void k_dft_jobs(const DftJob<real>* __restrict__ jobs, int64_t step) 
 
 template <typename real>
 void launch_dft_jobs(const DftJob<real>* jobs, int n_jobs, int64_t max_elems, int64_t step,
-                     hipStream_t s) {
+                     hipStream_t s, const int64_t* step_dev) {
   if (n_jobs <= 0 || max_elems <= 0) return;
   dim3 grid((unsigned)((max_elems + 255) / 256), (unsigned)n_jobs);
-  k_dft_jobs<real><<<grid, 256, 0, s>>>(jobs, step);
+  k_dft_jobs<real><<<grid, 256, 0, s>>>(jobs, step, step_dev);
 }
-template void launch_dft_jobs<float>(const DftJob<float>*, int, int64_t, int64_t, hipStream_t);
-template void launch_dft_jobs<double>(const DftJob<double>*, int, int64_t, int64_t, hipStream_t);
+template void launch_dft_jobs<float>(const DftJob<float>*, int, int64_t, int64_t, hipStream_t,
+                                     const int64_t*);
+template void launch_dft_jobs<double>(const DftJob<double>*, int, int64_t, int64_t, hipStream_t,
+                                      const int64_t*);
 
-template void launch_src_jobs<float>(const SrcJob<float>*, int, int, int64_t, hipStream_t);
-template void launch_src_jobs<double>(const SrcJob<double>*, int, int, int64_t, hipStream_t);
+template void launch_src_jobs<float>(const SrcJob<float>*, int, int, int64_t, hipStream_t,
+                                     const int64_t*);
+template void launch_src_jobs<double>(const SrcJob<double>*, int, int, int64_t, hipStream_t,
+                                      const int64_t*);
 
 // E^n on a dispersive region, captured before the curl update
 // (simulation.py:638-643).

@@ -180,7 +180,9 @@ namespace {
 template <typename real>
 class Stepper final : public pf_handle {
  public:
-  explicit Stepper(const pf_desc& d);
+  // zpad_default: the march's row layout when PHOTONFDTD_HIP_ZPAD does not
+  // name one (0: chosen by grid size)
+  explicit Stepper(const pf_desc& d, int zpad_default = 0);
   ~Stepper() override {
     if (stream_) {
       (void)hipStreamSynchronize(stream_);
@@ -273,6 +275,12 @@ class Stepper final : public pf_handle {
   // the batched launch wrote.
   void dft_jobs(int p, int64_t step0, std::vector<pfhip::DftJob<real>>& out) const;
   void finish_step_dft(int64_t step);
+  // True when the sample of `step` fills a DFT ring (finish_step_dft folds it)
+  bool dft_ring_fills(int64_t step) const {
+    for (const Dft& m : dft_)
+      if (step % m.interval == 0 && m.count + 1 == m.K) return true;
+    return false;
+  }
 
  private:
   struct Ade {
@@ -423,7 +431,7 @@ class Stepper final : public pf_handle {
 };
 
 template <typename real>
-Stepper<real>::Stepper(const pf_desc& d) {
+Stepper<real>::Stepper(const pf_desc& d, int zpad_default) {
   for (int a = 0; a < 3; ++a) {
     shape[a] = d.shape[a];
     if (d.shape[a] < 1) throw ArgError{"shape must be positive"};
@@ -652,6 +660,9 @@ Stepper<real>::Stepper(const pf_desc& d) {
     // the complex128 accumulators, so fewer folds save that traffic
     const int64_t budget = 1ll << 30;
     st.K = (int)std::max<int64_t>(1, std::min<int64_t>(128, budget / std::max<int64_t>(sample_bytes, 1)));
+    // a smaller ring on request (tests of the fold schedule)
+    if (const char* env_ring = std::getenv("PHOTONFDTD_HIP_DFT_RING"))
+      st.K = std::max(1, std::min(st.K, std::atoi(env_ring)));
     st.ring = static_cast<real*>(arena_.alloc((size_t)st.K * sample_bytes));
     st.count = 0;
     st.sample0 = 0;
@@ -690,12 +701,19 @@ Stepper<real>::Stepper(const pf_desc& d) {
     // 62-node blocks for the single-step march's tiles; 60 when the
     // two-step march (two-node apron) shares the levels
     const char* env_pair = std::getenv("PHOTONFDTD_HIP_PAIR");
-    // 64: dense rows padded at the end only, the 64-lane march (default);
-    // 62: 62 nodes per 64-slot block; 0: dense rows with the 62-node march
-    zpad_ = variant == 2 && !(env_zpad && env_zpad[0] == '0')
+    // 64: rows padded at the end only, the 64-lane march with one launch
+    // per CPML axis set (the default from 2^25 nodes: its psi-free tiles
+    // pay for the extra launches; 1.74 against 2.01 ms per step at 512^3);
+    // 62: 62 nodes per 64-slot block, two launches (the default below, where
+    // most tiles touch a slab: 0.082 against 0.113 ms at 128^3); 0: dense
+    // rows with the 62-node march
+    const int zdef = zpad_default ? zpad_default
+                                  : ncell >= (int64_t(1) << 25) ? pfhip::ZPAD_BLOCK : 62;
+    const int zenv = env_zpad ? std::atoi(env_zpad) : -1;
+    zpad_ = variant == 2 && zenv != 0
                 ? (env_pair && env_pair[0] == '1'
                        ? pfhip::MARCH2_OWNED_Z
-                       : (env_zpad && std::atoi(env_zpad) == 62 ? 62 : pfhip::ZPAD_BLOCK))
+                       : zenv == 62 || zenv == pfhip::ZPAD_BLOCK ? zenv : zdef)
                 : 0;
     if (zpad_) {
       zrow_ = pfhip::zpad_row(nz, zpad_);
@@ -2314,7 +2332,10 @@ class Batch3 final : public BatchBase {
     for (int i = 0; i < n; ++i) {
       pf_desc d = descs[i];
       d.flags &= ~PF_FLAG_GENERIC;
-      sims_.emplace_back(new Stepper<real>(d));
+      // padded 62-node blocks for every member: one layout for the whole batch,
+      // and the batch runs only the psi-free and all-axis CPML tile lists,
+      // where that layout's kernel is the faster one
+      sims_.emplace_back(new Stepper<real>(d, 62));
       sims_.back()->check_march_batch();
       sims_.back()->adopt_stream(stream_);
       n_steps_.push_back(d.n_steps);
@@ -2372,10 +2393,12 @@ class Batch3 final : public BatchBase {
     }
     HIP_CHECK(hipEventCreate(&ev_[0]));
     HIP_CHECK(hipEventCreate(&ev_[1]));
+    step_dev_ = arena_.zeros<int64_t>(1, stream_);
     configure(0, pfhip::MARCH_NW, pfhip::MARCH_MAX_XCHUNK);
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
   ~Batch3() override {
+    drop_graphs();
     for (hipEvent_t e : ev_)
       if (e) (void)hipEventDestroy(e);
     if (stream_) {
@@ -2397,16 +2420,19 @@ class Batch3 final : public BatchBase {
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
 
-  // mode: 0 only (launches per step, no graph or persistent mode);
-  // waves_per_block: wavefronts per march workgroup (10, 12 or 16); rows: x
-  // planes per psi-free tile (1 .. MARCH_MAX_XCHUNK).
+  // mode: 0 launches per step, 2 pairs of steps replayed from a captured
+  // hipGraph (no persistent mode); waves_per_block: wavefronts per march
+  // workgroup (10, 12 or 16); rows: x planes per psi-free tile
+  // (1 .. MARCH_MAX_XCHUNK).
   void configure(int mode, int waves_per_block, int rows) override {
-    if (mode != 0) throw ArgError{"3-D batches run in stream mode (0)"};
+    if (mode != 0 && mode != 2) throw ArgError{"3-D batches run in stream (0) or graph (2) mode"};
     if ((waves_per_block != 10 && waves_per_block != 12 && waves_per_block != 16) || rows < 1 ||
         rows > pfhip::MARCH_MAX_XCHUNK)
       throw ArgError{"bad 3-D batch configuration: waves 10, 12 or 16; rows 1 .. 128"};
+    mode_ = mode;
     if (waves_per_block == nw_ && rows == xchunk_) return;
     HIP_CHECK(hipStreamSynchronize(stream_));
+    drop_graphs();   // they hold the old tile lists
     for (int q = 0; q < 2; ++q) {   // q = 0: psi-free tiles, 1: CPML tiles
       std::vector<pfhip::MarchTile> all;
       prefix_[q].assign(1, 0);
@@ -2448,30 +2474,26 @@ class Batch3 final : public BatchBase {
       }
     }
     HIP_CHECK(hipEventRecord(ev_[0], stream_));
-    for (int64_t s = step_; s < step_ + n; ++s) {
-      size_t running = 0;   // simulations with steps left: a prefix of order_
-      while (running < order_.size() && n_steps_[order_[running]] > s) ++running;
-      const int p = (int)(s & 1);
-      for (int q = 0; q < 2; ++q) {   // psi-free tiles, then CPML tiles
-        pfhip::MarchArgs<real> L;
-        std::memset(&L, 0, sizeof(L));
-        L.tiles = tiles_[q];
-        L.ntiles = (int)prefix_[q][running];
-        L.sims = table_[p];
-        if (L.ntiles > 0) {
-          if (zpad_ == pfhip::ZPAD_BLOCK) pfhip::launch_march64<real>(L, exact_, q == 1 ? 7 : 0, nw_, stream_);
-          else pfhip::launch_march<real>(L, exact_, q == 1, nw_, dist_, stream_);
-        }
-      }
-      // ADE media and E-side sources of every running simulation, then
-      // each simulation's level flip and DFT capture (the single run's order)
-      pfhip::launch_ade_jobs<real>(ade_tab_[p], ade_prefix_[p][running], max_ade_, stream_);
-      pfhip::launch_src_jobs<real>(src_tab_[p], src_prefix_[p][running], max_groups_, s, stream_);
-      pfhip::launch_dft_jobs<real>(dft_tab_[p], dft_prefix_[running], max_dft_, s, stream_);
-      for (size_t r = 0; r < running; ++r) {
-        Stepper<real>& sim = *sims_[order_[r]];
-        if (sim.level() != p) throw HipError{PF_E_STATE, "batch simulation out of step"};
-        sim.finish_step_dft(s);
+    int64_t counter = -1;   // the device step counter as the queued work leaves it
+    for (int64_t s = step_; s < step_ + n;) {
+      const size_t running = running_at(s);
+      // a pair of steps from the graph: even first step, the same running
+      // set for both, and no DFT ring filling at the first (its fold must
+      // come before the second step's capture)
+      bool pair = mode_ == 2 && (s & 1) == 0 && s + 2 <= step_ + n && running > 0 &&
+                  running_at(s + 1) == running;
+      for (size_t r = 0; pair && r < running; ++r) pair = !sims_[order_[r]]->dft_ring_fills(s);
+      if (pair) {
+        if (counter != s) pfhip::launch_step_set(step_dev_, s, stream_);
+        HIP_CHECK(hipGraphLaunch(pair_graph(running), stream_));
+        counter = s + 2;
+        finish_step(s, running);
+        finish_step(s + 1, running);
+        s += 2;
+      } else {
+        enqueue_step(s, running, nullptr);
+        finish_step(s, running);
+        s += 1;
       }
     }
     HIP_CHECK(hipGetLastError());
@@ -2484,6 +2506,71 @@ class Batch3 final : public BatchBase {
   }
 
  private:
+  // simulations with steps left at step s: a prefix of order_
+  size_t running_at(int64_t s) const {
+    size_t running = 0;
+    while (running < order_.size() && n_steps_[order_[running]] > s) ++running;
+    return running;
+  }
+
+  // The launches of one step of the first `running` simulations of order_:
+  // the march lists, then the batched ADE, E-source and DFT-capture launches
+  // (the single run's order). With step_dev the step comes from the device
+  // counter, which the step then advances (graph capture).
+  void enqueue_step(int64_t s, size_t running, int64_t* step_dev) {
+    const int p = (int)(s & 1);
+    for (int q = 0; q < 2; ++q) {   // psi-free tiles, then CPML tiles
+      pfhip::MarchArgs<real> L;
+      std::memset(&L, 0, sizeof(L));
+      L.tiles = tiles_[q];
+      L.ntiles = (int)prefix_[q][running];
+      L.sims = table_[p];
+      if (L.ntiles > 0) {
+        if (zpad_ == pfhip::ZPAD_BLOCK) pfhip::launch_march64<real>(L, exact_, q == 1 ? 7 : 0, nw_, stream_);
+        else pfhip::launch_march<real>(L, exact_, q == 1, nw_, dist_, stream_);
+      }
+    }
+    pfhip::launch_ade_jobs<real>(ade_tab_[p], ade_prefix_[p][running], max_ade_, stream_);
+    pfhip::launch_src_jobs<real>(src_tab_[p], src_prefix_[p][running], max_groups_, s, stream_, step_dev);
+    pfhip::launch_dft_jobs<real>(dft_tab_[p], dft_prefix_[running], max_dft_, s, stream_, step_dev);
+    if (step_dev) pfhip::launch_step_tick(step_dev, stream_);
+  }
+
+  // Host bookkeeping of a step: level flip and DFT ring fill (a full ring
+  // is folded on the stream, after the step's capture).
+  void finish_step(int64_t s, size_t running) {
+    const int p = (int)(s & 1);
+    for (size_t r = 0; r < running; ++r) {
+      Stepper<real>& sim = *sims_[order_[r]];
+      if (sim.level() != p) throw HipError{PF_E_STATE, "batch simulation out of step"};
+      sim.finish_step_dft(s);
+    }
+  }
+
+  // The graph of an even step and the next one for `running` simulations,
+  // captured on first use.
+  hipGraphExec_t pair_graph(size_t running) {
+    for (auto& g : graphs_)
+      if (g.first == running) return g.second;
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    enqueue_step(0, running, step_dev_);
+    enqueue_step(1, running, step_dev_);
+    HIP_CHECK(hipStreamEndCapture(stream_, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_CHECK(e);
+    graphs_.emplace_back(running, exec);
+    return exec;
+  }
+
+  void drop_graphs() {
+    if (!graphs_.empty() && stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+    graphs_.clear();
+  }
+
   std::vector<std::unique_ptr<Stepper<real>>> sims_;
   std::vector<int64_t> n_steps_;
   std::vector<int> order_;          // simulations by decreasing step count
@@ -2506,6 +2593,9 @@ class Batch3 final : public BatchBase {
   bool exact_;
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double last_ms_ = 0.0;
+  int mode_ = 0;
+  int64_t* step_dev_ = nullptr;   // step counter of graph replays
+  std::vector<std::pair<size_t, hipGraphExec_t>> graphs_;   // per running count
 };
 
 template <typename F>

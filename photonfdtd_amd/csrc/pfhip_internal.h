@@ -324,11 +324,16 @@ struct DftJob {
 };
 template <typename real>
 void launch_ade_jobs(const AdeJob<real>* jobs, int n_jobs, int64_t max_n, hipStream_t s);
+// The step is `step`, or *step_dev when step_dev is set (graph replays).
 template <typename real>
 void launch_dft_jobs(const DftJob<real>* jobs, int n_jobs, int64_t max_elems, int64_t step,
-                     hipStream_t s);
+                     hipStream_t s, const int64_t* step_dev = nullptr);
 template <typename real>
-void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64_t step, hipStream_t s);
+void launch_src_jobs(const SrcJob<real>* jobs, int n_jobs, int max_groups, int64_t step, hipStream_t s,
+                     const int64_t* step_dev = nullptr);
+// Device step counter of graph replays: += 1, = v.
+void launch_step_tick(int64_t* step, hipStream_t s);
+void launch_step_set(int64_t* step, int64_t v, hipStream_t s);
 
 // ---- batched 2-D stepping -------------------------------------------------
 struct DftDev {

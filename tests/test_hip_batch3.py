@@ -86,20 +86,21 @@ def _assert_same(batch, i, ref, label):
     assert np.abs(fields["Ez"]).max() > 0 and np.isfinite(fields["Ez"]).all()
 
 
+@pytest.mark.parametrize("mode", ["stream", "graph"])
 @pytest.mark.parametrize("precision", ["float64", "float32"])
 @pytest.mark.parametrize("waves,rows", [(10, 128), (12, 7), (16, 5)])
-def test_batch3_matches_single_runs(precision, waves, rows, monkeypatch):
+def test_batch3_matches_single_runs(precision, waves, rows, mode, monkeypatch):
     from photonfdtd_amd.batch import SimBatch
     monkeypatch.setenv("PHOTONFDTD_HIP_EXACT", "1")
     sims = _scenes(precision)
     # the first simulation twice: the same tables in two slots of the batch
     sims = sims + [sims[0]]
-    print(f"\n[case] 3-D batch {precision} waves={waves} rows={rows} "
+    print(f"\n[case] 3-D batch {precision} {mode} waves={waves} rows={rows} "
           f"shapes={[s.grid.shape for s in sims]} steps={[s.n_steps for s in sims]}", flush=True)
     assert len({s.n_steps for s in sims}) > 1
     refs = [_single(s) for s in sims[:3]] + [None]
     refs[3] = refs[0]
-    b = SimBatch(sims, waves=waves, rows=rows)
+    b = SimBatch(sims, mode=mode, waves=waves, rows=rows)
     try:
         b.advance(31)
         b.advance()                                 # in two pieces
@@ -109,15 +110,18 @@ def test_batch3_matches_single_runs(precision, waves, rows, monkeypatch):
         b.close()
 
 
-def test_batch3_fast_order_reset_and_interval(monkeypatch):
-    """Fast operation order (float32), DFT every 3rd step; a reset and a
-    second run give the same bits again."""
+@pytest.mark.parametrize("mode", ["stream", "graph"])
+def test_batch3_fast_order_reset_and_interval(mode, monkeypatch):
+    """Fast operation order (float32), DFT every 3rd step into rings of 5
+    samples (folds at both step parities, so graph pairs stop short of them);
+    a reset and a second run give the same bits again."""
     from photonfdtd_amd.batch import SimBatch
     monkeypatch.setenv("PHOTONFDTD_HIP_EXACT", "0")
+    monkeypatch.setenv("PHOTONFDTD_HIP_DFT_RING", "5")
     sims = _scenes("float32", interval=3)
-    print("\n[case] 3-D batch fast order, interval 3, reset", flush=True)
+    print(f"\n[case] 3-D batch fast order, interval 3, ring 5, reset, {mode}", flush=True)
     refs = [_single(s) for s in sims]
-    b = SimBatch(sims)
+    b = SimBatch(sims, mode=mode)
     try:
         b.advance(40)
         b.reset()
@@ -139,8 +143,8 @@ def test_batch3_tune_and_set_eps(monkeypatch):
     b = SimBatch(sims)
     try:
         best = b.tune(trial_steps=20, rows=(32, 128), log=lambda m: print(m, flush=True))
-        assert best["mode"] == "stream" and best["ms_per_step"] > 0
-        assert len(best["trials"]) == 6
+        assert best["mode"] in ("stream", "graph") and best["ms_per_step"] > 0
+        assert len(best["trials"]) == 12
         b.set_eps(1, other.eps_r)
         b.advance()
         _assert_same(b, 0, _single(sims[0]), "tuned")

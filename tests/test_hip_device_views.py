@@ -35,7 +35,7 @@ def test_field_and_dft_views_equal_host_reads(layout, precision, monkeypatch):
     import photonfdtd_amd as pf
     from photonfdtd_amd import _native as N
     print(f"\n[case] device views {layout} {precision}", flush=True)
-    env = {"dense": {"PHOTONFDTD_HIP_ZPAD": "0"}}.get(layout, {})
+    env = {"dense": {"PHOTONFDTD_HIP_ZPAD": "0"}, "march64": {"PHOTONFDTD_HIP_ZPAD": "64"}}.get(layout, {})
     flags = N.PF_FLAG_GENERIC if layout == "generic" else 0
     sim, eng = _engine(pf, precision, env, monkeypatch, flags)
     try:

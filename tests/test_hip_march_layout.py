@@ -1,13 +1,13 @@
 """GPU: the 3-D march gives the same bits under every layout and launch shape.
 
-By default the march runs 64 lanes per z tile on rows of a multiple of 64
-slots (node k in slot k) and splits each step into a psi-free launch and
-one launch per CPML axis set, the several-axis, x- and y-only lists on a
-second stream. The other layouts (dense rows with 62-node windows; padded
-rows, node k in slot k + 2 * (k // 62)), a single launch, no overlap and
-other tile shapes must give identical fields, DFT planes, flux totals and
-field snapshots; snapshots and DFT planes read the rows through the same
-slot mapping, and write_field / read_field round-trip through it.
+On grids below 2^25 nodes (this scene) the march keeps padded rows (node k
+in slot k + 2 * (k // 62)) with a psi-free and a CPML launch; from 2^25
+nodes it runs 64 lanes per z tile on rows of a multiple of 64 slots (node k
+in slot k) with one launch per CPML axis set. Every layout (those two and
+dense rows with 62-node windows), a single launch, no overlap and other
+tile shapes must give identical fields, DFT planes, flux totals and field
+snapshots; snapshots and DFT planes read the rows through the same slot
+mapping, and write_field / read_field round-trip through it.
 """
 import numpy as np
 import pytest
@@ -18,15 +18,18 @@ pytestmark = pytest.mark.gpu
 
 CONFIGS = {
     "dense single launch": {"PHOTONFDTD_HIP_ZPAD": "0", "PHOTONFDTD_HIP_SPLIT": "0"},
-    "64-lane single launch": {"PHOTONFDTD_HIP_SPLIT": "0"},
-    "64-lane split, no overlap": {"PHOTONFDTD_HIP_OVERLAP": "0"},
-    "64-lane split, 5-plane CPML chunks, 10 waves": {"PHOTONFDTD_HIP_PML_XCHUNK": "5",
+    "64-lane split": {"PHOTONFDTD_HIP_ZPAD": "64"},
+    "64-lane single launch": {"PHOTONFDTD_HIP_ZPAD": "64", "PHOTONFDTD_HIP_SPLIT": "0"},
+    "64-lane split, no overlap": {"PHOTONFDTD_HIP_ZPAD": "64", "PHOTONFDTD_HIP_OVERLAP": "0"},
+    "64-lane split, 5-plane CPML chunks, 10 waves": {"PHOTONFDTD_HIP_ZPAD": "64",
+                                                     "PHOTONFDTD_HIP_PML_XCHUNK": "5",
                                                      "PHOTONFDTD_HIP_NW": "10",
                                                      "PHOTONFDTD_HIP_PML_NW": "10"},
-    "64-lane split, 16 waves, 7-plane chunks": {"PHOTONFDTD_HIP_NW": "16",
+    "64-lane split, 16 waves, 7-plane chunks": {"PHOTONFDTD_HIP_ZPAD": "64",
+                                                "PHOTONFDTD_HIP_NW": "16",
                                                 "PHOTONFDTD_HIP_XCHUNK": "7"},
     "dense split": {"PHOTONFDTD_HIP_ZPAD": "0", "PHOTONFDTD_HIP_SPLIT": "1"},
-    "padded split": {"PHOTONFDTD_HIP_ZPAD": "62"},
+    "padded split, no overlap": {"PHOTONFDTD_HIP_ZPAD": "62", "PHOTONFDTD_HIP_OVERLAP": "0"},
     "padded single launch": {"PHOTONFDTD_HIP_ZPAD": "62", "PHOTONFDTD_HIP_SPLIT": "0"},
 }
 KEYS = ("PHOTONFDTD_HIP_ZPAD", "PHOTONFDTD_HIP_SPLIT", "PHOTONFDTD_HIP_OVERLAP",
@@ -65,11 +68,11 @@ def test_layouts_and_launch_shapes_agree(precision, monkeypatch):
             assert np.array_equal(np.asarray(got.flux[m]), np.asarray(ref.flux[m])), (label, m)
 
 
-@pytest.mark.parametrize("zpad", [None, "62"])
+@pytest.mark.parametrize("zpad", [None, "64"])
 def test_write_and_read_field_through_padded_rows(zpad, monkeypatch):
     """pf_write_field / pf_read_box on a stepper whose rows carry slots past
-    nz (the default 64-slot rows; padded 62-node blocks): a written field
-    reads back unchanged."""
+    nz (padded 62-node blocks, the default here; 64-slot rows): a written
+    field reads back unchanged."""
     import photonfdtd_amd as pf
     from photonfdtd_amd.hip_backend import HipEngine
     for k in KEYS:
