@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <tuple>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -372,6 +373,13 @@ class Stepper final : public pf_handle {
   int* tiles_[NLIST] = {};
   int ntiles_[NLIST] = {};
   int64_t tile_cells_[NLIST] = {};       // owned cells of each list
+  // Lists launched together with the psi-free list (bit q: list q; lists 2
+  // and 3, z- and y-only CPML): one launch over the union in spatial order,
+  // so CPML tiles and their psi-free neighbours share the apron lines.
+  int mix_mask_ = 0;
+  bool even_x_ = false;           // x chunks of nearly equal length (64-lane march)
+  int* tiles_mix_ = nullptr;
+  int ntiles_mix_ = 0;
   int march_nw_ = pfhip::MARCH_NW;
   int pml_nw_ = pfhip::MARCH_NW;  // wavefronts per workgroup of the CPML-tile launch
   int march_dist_ = 2;            // planes of prefetch in the march kernel
@@ -715,6 +723,15 @@ Stepper<real>::Stepper(const pf_desc& d, int zpad_default) {
                        ? pfhip::MARCH2_OWNED_Z
                        : zenv == 62 || zenv == pfhip::ZPAD_BLOCK ? zenv : zdef)
                 : 0;
+    if (zpad_ == pfhip::ZPAD_BLOCK) {
+      // the 64-lane march addresses three field components, and each psi
+      // side, through one buffer resource with 32-bit offsets below 2^31
+      const double lim = 2147483648.0;
+      const double comp = (double)nx * ny * pfhip::zpad_row(nz, zpad_) * sizeof(real);
+      if (3.0 * comp >= lim || (double)psi_len_e_ * sizeof(real) >= lim ||
+          (double)psi_len_h_ * sizeof(real) >= lim)
+        zpad_ = 62;
+    }
     if (zpad_) {
       zrow_ = pfhip::zpad_row(nz, zpad_);
       fcell_ = (int64_t)nx * ny * zrow_;
@@ -749,11 +766,11 @@ Stepper<real>::Stepper(const pf_desc& d, int zpad_default) {
     real* psi_h1[6];
     for (int s = 0; s < 6; ++s)
       psi_h1[s] = psi_h_lvl1_ ? psi_h_lvl1_ + (args_.psi[6 + s] - args_.psi[6]) : nullptr;
-    // planes per march tile: with dense rows about 48, so that each list
-    // holds several workgroups per CU slot and the last round of tiles
-    // leaves little of the chip idle (measured at 512^3 float32)
+    // planes per march tile: with dense rows at most 36 (x ranges cut into
+    // chunks of nearly equal length), measured fastest at 512^3 float32
+    // with the mixed launch (32-48 planes within 2 %)
     const char* env_xc = std::getenv("PHOTONFDTD_HIP_XCHUNK");
-    const int xchunk = env_xc ? std::max(1, std::atoi(env_xc)) : zpad_ == pfhip::ZPAD_BLOCK ? 48 : 128;
+    const int xchunk = env_xc ? std::max(1, std::atoi(env_xc)) : zpad_ == pfhip::ZPAD_BLOCK ? 36 : 128;
     for (int p = 0; p < 2; ++p) {
       pfhip::FusedArgs<real>& fa = fa_[p];
       std::memset(&fa, 0, sizeof(fa));
@@ -859,6 +876,8 @@ template <typename real>
 void Stepper<real>::build_march_tiles(int xchunk) {
   const int n[3] = {args_.nx, args_.ny, args_.nz};
   const int len[3] = {xchunk, march_nw_ - 2, 62};
+  const char* env_even = std::getenv("PHOTONFDTD_HIP_EVEN_X");
+  even_x_ = !(env_even && env_even[0] == '0');
   std::vector<Seg> seg[3];
   for (int a = 0; a < 3; ++a) {
     const int b0 = std::max(slab_e_[a][0], slab_h_[a][0] + 1);
@@ -874,6 +893,11 @@ void Stepper<real>::build_march_tiles(int xchunk) {
         const int c = std::min(zpad_, n[2] - z);
         seg[2].push_back({z, c, z >= b0 && z + c <= b1});
       }
+    } else if (a == 0 && zpad_ == pfhip::ZPAD_BLOCK && split_ && even_x_) {
+      // x chunks of nearly equal length (at most xchunk planes) in each of
+      // the CPML and bulk ranges: no short last chunk re-reading its two
+      // boundary planes for a few planes of work
+      seg[0] = split_even(n[0], b0, b1, len[0]);
     } else {
       seg[a] = split_ ? split_axis(n[a], b0, b1, len[a], a > 0) : split_axis(n[a], 0, 0, len[a], false);
     }
@@ -926,6 +950,32 @@ void Stepper<real>::build_march_tiles(int xchunk) {
         }
       lists[q].swap(rows);
     }
+  }
+  // z-only and y-only CPML tiles in the psi-free launch (float32 default;
+  // PHOTONFDTD_HIP_MIX = bit mask of lists 2, 3 and 4: z-, y- and x-only),
+  // 12-wave workgroups. The float64 mixed kernel needs 141 VGPRs (one
+  // workgroup per CU), so float64 keeps one launch per list.
+  const char* env_mix = std::getenv("PHOTONFDTD_HIP_MIX");
+  mix_mask_ = (env_mix ? std::atoi(env_mix) : sizeof(real) == 4 ? 0b1100 : 0) & 0b11100;
+  if ((mix_mask_ & 0b10000) && !(mix_mask_ & 0b1000)) mix_mask_ &= ~0b10000;   // x only with y
+  if (zpad_ != pfhip::ZPAD_BLOCK || !split_ || march_nw_ != 12 || pml_nw_ != 12 || lists[0].empty())
+    mix_mask_ = 0;
+  if (mix_mask_) {
+    std::vector<pfhip::MarchTile> m;
+    for (int q = 0; q < NLIST; ++q) {
+      if (q != 0 && !(mix_mask_ >> q & 1)) continue;
+      check_march_tiles(lists[q], 12);
+      for (pfhip::MarchTile t : lists[q]) {
+        t.pad1 = list_mask_[q];
+        m.push_back(t);
+      }
+    }
+    std::sort(m.begin(), m.end(), [](const pfhip::MarchTile& u, const pfhip::MarchTile& v) {
+      return std::tie(u.i0, u.j0, u.k0) < std::tie(v.i0, v.j0, v.k0);
+    });
+    std::vector<pfhip::MarchTile> ordered = xcd_order(m);
+    ntiles_mix_ = (int)ordered.size();
+    tiles_mix_ = reinterpret_cast<int*>(arena_.upload(ordered.data(), ordered.size(), stream_));
   }
   for (int q = 0; q < NLIST; ++q) {
     check_march_tiles(lists[q], q == 0 ? march_nw_ : pml_nw_);
@@ -1265,12 +1315,16 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
           HIP_CHECK(hipStreamWaitEvent(side_, fork_ev_[0], 0));
         }
         for (int q : order) {
-          if (ntiles_[q] == 0) continue;
+          if (ntiles_[q] == 0 || (mix_mask_ >> q & 1)) continue;
           pfhip::MarchArgs<real> ma = ma_[in];
           ma.tiles = tiles_[q];
           ma.ntiles = ntiles_[q];
           hipStream_t s = fork && on_side[q] ? side_ : stream_;
-          if (zpad_ == pfhip::ZPAD_BLOCK)
+          if (q == 0 && mix_mask_) {
+            ma.tiles = tiles_mix_;
+            ma.ntiles = ntiles_mix_;
+            pfhip::launch_march64_mix<real>(ma, exact_, 12, (mix_mask_ >> 3 & 1) << 1 | (mix_mask_ >> 4 & 1), s);
+          } else if (zpad_ == pfhip::ZPAD_BLOCK)
             pfhip::launch_march64<real>(ma, exact_, list_mask_[q], q == 0 ? march_nw_ : pml_nw_, s);
           else
             pfhip::launch_march<real>(ma, exact_, q == 1, q == 1 ? pml_nw_ : march_nw_,

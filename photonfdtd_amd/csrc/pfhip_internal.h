@@ -493,6 +493,11 @@ void launch_march(MarchArgs<real>& a, bool exact, bool pml, int nw, int dist, hi
 // for single-simulation launches.
 template <typename real>
 void launch_march64(MarchArgs<real>& a, bool exact, int mask, int nw, hipStream_t s);
+// Tiles of the psi-free and z-only CPML sets in one launch, with the y-only
+// (sets bit 1) and x-only (sets bit 0; with y only) sets, the set of each
+// tile in its pad1 field (0, 4, 2, 1).
+template <typename real>
+void launch_march64_mix(MarchArgs<real>& a, bool exact, int nw, int sets, hipStream_t s);
 template <typename real> bool fused_supported(const StepArgs<real>& a);
 template <typename real> void launch_fused(FusedArgs<real>& a, bool exact, hipStream_t s);
 template <typename real>

@@ -64,24 +64,33 @@ enum Apron {
 
 }  // namespace
 
-// PMASK: the CPML axes the tile's nodes may lie in (bit 0 x, 1 y, 2 z); the
-// psi terms of the other axes are left out at compile time. DIST: planes of
-// load prefetch (2: three rotating plane buffers; 1: two).
-// float32 variants other than the all-axis one are held to 80 VGPRs (six
-// waves per SIMD: two 12-wave workgroups per CU).
-template <typename real, bool EXACT, int NW, int PMASK, int DIST, bool BATCH = false>
-__global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) void k_yee_m64(
-    MarchArgs<real> arg) {
+constexpr int M64_XS = MARCH_MAX_XCHUNK + 1;
+
+// LDS of one march workgroup: the row exchange of E and H (double-buffered by
+// plane parity) and the x-axis CPML coefficients of the tile's planes.
+template <typename real, int NW>
+struct M64Lds {
+  real ex[2][NW * 64], ez[2][NW * 64], hx[2][NW * 64], hz[2][NW * 64];
+  real x[4][M64_XS];
+};
+
+// One tile of the march: x planes [i0, i1) of rows j0..j0+jn-1, z nodes
+// k0..k0+kn-1. Every wave of the workgroup must call it (it holds LDS
+// barriers); it returns with the workgroup's LDS still in use by other
+// waves, so a caller that runs another tile next separates the two with a
+// barrier.
+template <typename real, bool EXACT, int NW, int PMASK, int DIST, bool BATCH>
+__device__ __forceinline__ void m64_tile(const MarchArgs<real>& arg, const MarchTile& tile,
+                                         M64Lds<real, NW>& S) {
   constexpr bool PX = (PMASK & 1) != 0, PY = (PMASK & 2) != 0, PZ = (PMASK & 4) != 0;
   constexpr bool PML = PMASK != 0;
   constexpr uint32_t B = sizeof(real);
-  constexpr int XS = MARCH_MAX_XCHUNK + 1;
-  __shared__ real sEx[2][NW * 64], sEz[2][NW * 64], sHx[2][NW * 64], sHz[2][NW * 64];
-  __shared__ real sX[4][XS];
-  if ((int)blockIdx.x >= arg.ntiles) return;
-  const MarchTile tile = reinterpret_cast<const MarchTile*>(arg.tiles)[blockIdx.x];
-  if (tile.i1 <= tile.i0) return;
-  const MarchArgs<real> a = BATCH ? arg.sims[tile.pad0] : arg;
+  auto& sEx = S.ex;
+  auto& sEz = S.ez;
+  auto& sHx = S.hx;
+  auto& sHz = S.hz;
+  auto& sX = S.x;
+  const MarchArgs<real>& a = BATCH ? arg.sims[tile.pad0] : arg;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nyz = (size_t)ny * nz;
   const size_t fplane = (size_t)ny * a.zrow, fcell = (size_t)nx * fplane;
@@ -102,30 +111,30 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
   const bool lo_ok = k0 > 0;            // an apron node below the tile exists
   const bool up_ok = k0 + kn < nz;      // and above it
 
-  const real* Ein0 = a.Lin;
-  const real* Ein1 = a.Lin + fcell;
-  const real* Ein2 = a.Lin + 2 * fcell;
-  const real* Hin0 = a.Lin + 3 * fcell;
-  const real* Hin1 = a.Lin + 4 * fcell;
-  const real* Hin2 = a.Lin + 5 * fcell;
-  real* Eo0 = a.Lout;
-  real* Eo1 = a.Lout + fcell;
-  real* Eo2 = a.Lout + 2 * fcell;
-  real* Ho0 = a.Lout + 3 * fcell;
-  real* Ho1 = a.Lout + 4 * fcell;
-  real* Ho2 = a.Lout + 5 * fcell;
+  // Buffer resources made once per tile, one for the E and one for the H
+  // components of each level (component and plane offsets go in the scalar
+  // operand), one for ce and one per psi side. The host runs this kernel
+  // only while three field components, and each psi side, span less than
+  // 2^31 bytes, so every scalar offset stays below 2^31.
+  const uint32_t fcB = (uint32_t)(fcell * B), fpB = (uint32_t)(fplane * B);
+  const __amdgpu_buffer_rsrc_t rEi = buf_rsrc(a.Lin), rHi = buf_rsrc(a.Lin + 3 * fcell);
+  const __amdgpu_buffer_rsrc_t rEo = buf_rsrc(a.Lout), rHo = buf_rsrc(a.Lout + 3 * fcell);
+  const __amdgpu_buffer_rsrc_t rCe = buf_rsrc(a.ce), rPE = buf_rsrc(a.psiE),
+                               rPHi = buf_rsrc(a.psiHin), rPHo = buf_rsrc(a.psiHout);
+  const uint32_t c1 = fcB, c2 = 2 * fcB;
   const size_t sy_e = (size_t)nx * a.pe[1] * nz, sz_e = (size_t)nx * ny * a.pe[2],
                sx_e = (size_t)a.pe[0] * nyz;
   const size_t sy_h = (size_t)nx * a.ph[1] * nz, sz_h = (size_t)nx * ny * a.ph[2],
                sx_h = (size_t)a.ph[0] * nyz;
-  real* pExY = a.psiE;
-  real* pExZ = pExY + sy_e;
-  real* pEyZ = pExZ + sz_e;
-  real* pEyX = pEyZ + sz_e;
-  real* pEzX = pEyX + sx_e;
-  real* pEzY = pEzX + sx_e;
+  // byte offsets of the psi slots in their side's allocation:
+  // E side ExY ExZ EyZ EyX EzX EzY, H side HxY HxZ HyZ HyX HzX HzY
+  const uint32_t eoB[6] = {0, (uint32_t)(sy_e * B), (uint32_t)((sy_e + sz_e) * B),
+                           (uint32_t)((sy_e + 2 * sz_e) * B), (uint32_t)((sy_e + 2 * sz_e + sx_e) * B),
+                           (uint32_t)((sy_e + 2 * sz_e + 2 * sx_e) * B)};
   const size_t hoff[6] = {0, sy_h, sy_h + sz_h, sy_h + 2 * sz_h, sy_h + 2 * sz_h + sx_h,
                           sy_h + 2 * sz_h + 2 * sx_h};
+  const uint32_t hoB[6] = {0, (uint32_t)(hoff[1] * B), (uint32_t)(hoff[2] * B), (uint32_t)(hoff[3] * B),
+                           (uint32_t)(hoff[4] * B), (uint32_t)(hoff[5] * B)};
   const real* coef_y = a.coef + 4 * (size_t)nx;
   const real* coef_z = a.coef + 4 * (size_t)(nx + ny);
 
@@ -161,6 +170,10 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
   const uint32_t cze_o = (uint32_t)(jc * a.pe[2] + ze.m(kc)) * B;
   const size_t ys_h = (size_t)a.ph[1] * nz, zs_h = (size_t)ny * a.ph[2];
   const size_t ys_e = (size_t)a.pe[1] * nz, zs_e = (size_t)ny * a.pe[2];
+  // per-plane byte strides of the y- and z-slab psi slots, of an x-slab psi plane
+  const uint32_t ysB_h = (uint32_t)(ys_h * B), zsB_h = (uint32_t)(zs_h * B);
+  const uint32_t ysB_e = (uint32_t)(ys_e * B), zsB_e = (uint32_t)(zs_e * B);
+  const uint32_t xsB = (uint32_t)(nyz * B);
 
   // Lane offsets of every access, BUF_NONE where the lane does not take part
   // (the access is issued on every path; see BUF_NONE).
@@ -230,47 +243,49 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
     real apr;     // apron gather
   };
   auto fetch = [&](Plane& P, int i) __attribute__((always_inline)) {
-    const size_t o = (size_t)i * fplane;
-    const size_t on = (size_t)min(i + 1, nx - 1) * fplane;
+    const uint32_t so = (uint32_t)i * fpB;
+    const uint32_t son = (uint32_t)min(i + 1, nx - 1) * fpB;
     const uint32_t oe = i + 1 < nx ? f_l : BUF_NONE;
-    P.e[0] = bld(Ein0 + on, oe);
-    P.e[1] = bld(Ein1 + on, oe);
-    P.e[2] = bld(Ein2 + on, oe);
-    P.h[0] = bld(Hin0 + o, f_h);
-    P.h[1] = bld(Hin1 + o, f_h);
-    P.h[2] = bld(Hin2 + o, f_h);
-    P.ce = bld(a.ce + o, f_ce);
+    P.e[0] = blds<real>(rEi, oe, son);
+    P.e[1] = blds<real>(rEi, oe, son + c1);
+    P.e[2] = blds<real>(rEi, oe, son + c2);
+    P.h[0] = blds<real>(rHi, f_h, so);
+    P.h[1] = blds<real>(rHi, f_h, so + c1);
+    P.h[2] = blds<real>(rHi, f_h, so + c2);
+    P.ce = blds<real>(rCe, f_ce, so);
     P.apr = apron(i + 1, i, true);
     for (int c = 0; c < 6; ++c) P.ph[c] = P.pe[c] = real(0);
     if constexpr (PY) {
-      P.ph[0] = bld(a.psiHin + hoff[0] + i * ys_h, o_pyh);
-      P.ph[5] = bld(a.psiHin + hoff[5] + i * ys_h, o_pyh);
-      P.pe[0] = bld(pExY + i * ys_e, o_pye);
-      P.pe[5] = bld(pEzY + i * ys_e, o_pye);
+      const uint32_t yh = (uint32_t)i * ysB_h, ye = (uint32_t)i * ysB_e;
+      P.ph[0] = blds<real>(rPHi, o_pyh, hoB[0] + yh);
+      P.ph[5] = blds<real>(rPHi, o_pyh, hoB[5] + yh);
+      P.pe[0] = blds<real>(rPE, o_pye, eoB[0] + ye);
+      P.pe[5] = blds<real>(rPE, o_pye, eoB[5] + ye);
     }
     if constexpr (PZ) {
-      P.ph[1] = bld(a.psiHin + hoff[1] + i * zs_h, o_pzh);
-      P.ph[2] = bld(a.psiHin + hoff[2] + i * zs_h, o_pzh);
-      P.pe[1] = bld(pExZ + i * zs_e, o_pze);
-      P.pe[2] = bld(pEyZ + i * zs_e, o_pze);
+      const uint32_t zh = (uint32_t)i * zsB_h, ze = (uint32_t)i * zsB_e;
+      P.ph[1] = blds<real>(rPHi, o_pzh, hoB[1] + zh);
+      P.ph[2] = blds<real>(rPHi, o_pzh, hoB[2] + zh);
+      P.pe[1] = blds<real>(rPE, o_pze, eoB[1] + ze);
+      P.pe[2] = blds<real>(rPE, o_pze, eoB[2] + ze);
     }
     if constexpr (PX) {
       const bool xhi = xh.in(i), xei = xe.in(i);
-      const size_t xoh = (size_t)(xhi ? xh.m(i) : 0) * nyz, xoe = (size_t)(xei ? xe.m(i) : 0) * nyz;
-      P.ph[3] = bld(a.psiHin + hoff[3] + xoh, xhi ? o_h : BUF_NONE);
-      P.ph[4] = bld(a.psiHin + hoff[4] + xoh, xhi ? o_h : BUF_NONE);
-      P.pe[3] = bld(pEyX + xoe, xei ? o_e : BUF_NONE);
-      P.pe[4] = bld(pEzX + xoe, xei ? o_e : BUF_NONE);
+      const uint32_t xoh = (uint32_t)(xhi ? xh.m(i) : 0) * xsB, xoe = (uint32_t)(xei ? xe.m(i) : 0) * xsB;
+      P.ph[3] = blds<real>(rPHi, xhi ? o_h : BUF_NONE, hoB[3] + xoh);
+      P.ph[4] = blds<real>(rPHi, xhi ? o_h : BUF_NONE, hoB[4] + xoh);
+      P.pe[3] = blds<real>(rPE, xei ? o_e : BUF_NONE, eoB[3] + xoe);
+      P.pe[4] = blds<real>(rPE, xei ? o_e : BUF_NONE, eoB[4] + xoe);
     }
   };
 
   // E^n(ib) and its apron (E slots at plane ib)
   real ecur[3];
   {
-    const size_t o = (size_t)ib * fplane;
-    ecur[0] = bld(Ein0 + o, f_l);
-    ecur[1] = bld(Ein1 + o, f_l);
-    ecur[2] = bld(Ein2 + o, f_l);
+    const uint32_t so = (uint32_t)ib * fpB;
+    ecur[0] = blds<real>(rEi, f_l, so);
+    ecur[1] = blds<real>(rEi, f_l, so + c1);
+    ecur[2] = blds<real>(rEi, f_l, so + c2);
   }
   real acur = apron(ib, ib, false);
   real hy_prev = 0, hz_prev = 0;
@@ -280,7 +295,7 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
     if (i + DIST < i1) fetch(Next, i + DIST);
     const bool live = i >= i0;
     const int bi = i & 1;
-    const size_t o = (size_t)i * fplane;
+    const uint32_t so = (uint32_t)i * fpB;
     const real ex_jp = w < NW - 1 ? sEx[bi][min(tid + 64, NW * 64 - 1)] : real(0);
     const real ez_jp = w < NW - 1 ? sEz[bi][min(tid + 64, NW * 64 - 1)] : real(0);
     // E(k+1): the next lane, the apron above the tile for the last node
@@ -323,17 +338,19 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
       const bool ozx = own && live && zh_in && xh_in;
       const bool oxy = own && live && xh_in && yh_in;
       if constexpr (PY) {
-        bst(a.psiHout + hoff[0] + i * ys_h, oyz ? o_pyh : BUF_NONE, nps[0]);
-        bst(a.psiHout + hoff[5] + i * ys_h, oxy ? o_pyh : BUF_NONE, nps[5]);
+        const uint32_t yh = (uint32_t)i * ysB_h;
+        bsts(rPHo, oyz ? o_pyh : BUF_NONE, hoB[0] + yh, nps[0]);
+        bsts(rPHo, oxy ? o_pyh : BUF_NONE, hoB[5] + yh, nps[5]);
       }
       if constexpr (PZ) {
-        bst(a.psiHout + hoff[1] + i * zs_h, oyz ? o_pzh : BUF_NONE, nps[1]);
-        bst(a.psiHout + hoff[2] + i * zs_h, ozx ? o_pzh : BUF_NONE, nps[2]);
+        const uint32_t zh = (uint32_t)i * zsB_h;
+        bsts(rPHo, oyz ? o_pzh : BUF_NONE, hoB[1] + zh, nps[1]);
+        bsts(rPHo, ozx ? o_pzh : BUF_NONE, hoB[2] + zh, nps[2]);
       }
       if constexpr (PX) {
-        const size_t xo = (size_t)(pxh ? xh.m(i) : 0) * nyz;
-        bst(a.psiHout + hoff[3] + xo, ozx && pxh ? o_h : BUF_NONE, nps[3]);
-        bst(a.psiHout + hoff[4] + xo, oxy && pxh ? o_h : BUF_NONE, nps[4]);
+        const uint32_t xo = (uint32_t)(pxh ? xh.m(i) : 0) * xsB;
+        bsts(rPHo, ozx && pxh ? o_h : BUF_NONE, hoB[3] + xo, nps[3]);
+        bsts(rPHo, oxy && pxh ? o_h : BUF_NONE, hoB[4] + xo, nps[4]);
       }
     }
     // H^{n+1/2}(i) at k0-1, recomputed as its owner does (wave-uniform)
@@ -362,9 +379,9 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
       }
     }
     const uint32_t fs = live ? f_st : BUF_NONE;
-    bst_nt(Ho0 + o, fs, hx);
-    bst_nt(Ho1 + o, fs, hy);
-    bst_nt(Ho2 + o, fs, hz);
+    bsts<BUF_NT>(rHo, fs, so, hx);
+    bsts<BUF_NT>(rHo, fs, so + c1, hy);
+    bsts<BUF_NT>(rHo, fs, so + c2, hz);
     sHx[bi][tid] = hx;
     sHz[bi][tid] = hz;
     sEx[bi ^ 1][tid] = P.e[0];
@@ -405,25 +422,27 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
                          P.pe[5], bye, cye, &npe[5]);
       }
     }
-    bst_nt(Eo0 + o, fs, ex);
-    bst_nt(Eo1 + o, fs, ey);
-    bst_nt(Eo2 + o, fs, ez);
+    bsts<BUF_NT>(rEo, fs, so, ex);
+    bsts<BUF_NT>(rEo, fs, so + c1, ey);
+    bsts<BUF_NT>(rEo, fs, so + c2, ez);
     if constexpr (PML) {
       const bool oyz = own && live && ye_in && ze_in;
       const bool ozx = own && live && ze_in && xe_in;
       const bool oxy = own && live && xe_in && ye_in;
       if constexpr (PY) {
-        bst(pExY + i * ys_e, oyz ? o_pye : BUF_NONE, npe[0]);
-        bst(pEzY + i * ys_e, oxy ? o_pye : BUF_NONE, npe[5]);
+        const uint32_t ye = (uint32_t)i * ysB_e;
+        bsts(rPE, oyz ? o_pye : BUF_NONE, eoB[0] + ye, npe[0]);
+        bsts(rPE, oxy ? o_pye : BUF_NONE, eoB[5] + ye, npe[5]);
       }
       if constexpr (PZ) {
-        bst(pExZ + i * zs_e, oyz ? o_pze : BUF_NONE, npe[1]);
-        bst(pEyZ + i * zs_e, ozx ? o_pze : BUF_NONE, npe[2]);
+        const uint32_t ze = (uint32_t)i * zsB_e;
+        bsts(rPE, oyz ? o_pze : BUF_NONE, eoB[1] + ze, npe[1]);
+        bsts(rPE, ozx ? o_pze : BUF_NONE, eoB[2] + ze, npe[2]);
       }
       if constexpr (PX) {
-        const size_t xo = (size_t)(pxe ? xe.m(i) : 0) * nyz;
-        bst(pEyX + xo, ozx && pxe ? o_e : BUF_NONE, npe[3]);
-        bst(pEzX + xo, oxy && pxe ? o_e : BUF_NONE, npe[4]);
+        const uint32_t xo = (uint32_t)(pxe ? xe.m(i) : 0) * xsB;
+        bsts(rPE, ozx && pxe ? o_e : BUF_NONE, eoB[3] + xo, npe[3]);
+        bsts(rPE, oxy && pxe ? o_e : BUF_NONE, eoB[4] + xo, npe[4]);
       }
     }
     hy_prev = hy;
@@ -482,6 +501,39 @@ __global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) v
   }
 }
 
+// PMASK: the CPML axes the tile's nodes may lie in (bit 0 x, 1 y, 2 z); the
+// psi terms of the other axes are left out at compile time. DIST: planes of
+// load prefetch (2: three rotating plane buffers; 1: two).
+// float32 variants other than the all-axis one are held to 80 VGPRs (six
+// waves per SIMD: two 12-wave workgroups per CU).
+template <typename real, bool EXACT, int NW, int PMASK, int DIST, bool BATCH = false>
+__global__ __launch_bounds__(NW * 64, sizeof(real) == 4 && PMASK != 7 ? 6 : 1) void k_yee_m64(
+    MarchArgs<real> arg) {
+  __shared__ M64Lds<real, NW> S;
+  if ((int)blockIdx.x >= arg.ntiles) return;
+  const MarchTile tile = reinterpret_cast<const MarchTile*>(arg.tiles)[blockIdx.x];
+  if (tile.i1 <= tile.i0) return;
+  m64_tile<real, EXACT, NW, PMASK, DIST, BATCH>(arg, tile, S);
+}
+
+// Tiles of several CPML axis sets in one launch, one workgroup per tile, the
+// set of each tile in its pad1 field (0: psi-free, 4: z slabs only). The
+// list is in spatial order (z tiles of a row block next to each other), so
+// a CPML tile and its psi-free z neighbours run at the same time on one XCD
+// and share the z apron lines in its L2.
+template <typename real, bool EXACT, int NW, int SETS>
+__global__ __launch_bounds__(NW * 64, sizeof(real) == 4 ? 6 : 1) void k_yee_m64_mix(
+    MarchArgs<real> arg) {
+  __shared__ M64Lds<real, NW> S;
+  if ((int)blockIdx.x >= arg.ntiles) return;
+  const MarchTile tile = reinterpret_cast<const MarchTile*>(arg.tiles)[blockIdx.x];
+  if (tile.i1 <= tile.i0) return;
+  if (tile.pad1 == 0) m64_tile<real, EXACT, NW, 0, 2, false>(arg, tile, S);
+  else if (tile.pad1 == 4) m64_tile<real, EXACT, NW, 4, 2, false>(arg, tile, S);
+  else if ((SETS & 2) && tile.pad1 == 2) m64_tile<real, EXACT, NW, 2, 2, false>(arg, tile, S);
+  else if (SETS & 1) m64_tile<real, EXACT, NW, 1, 2, false>(arg, tile, S);
+}
+
 // Variants: psi-free and single-axis CPML tiles prefetch two planes (three
 // buffers; they fit two 12-wave workgroups per CU), the all-axis CPML
 // variant one (two buffers; one workgroup per CU within 168 VGPRs).
@@ -525,5 +577,23 @@ void launch_march64(MarchArgs<real>& a, bool exact, int mask, int nw, hipStream_
 
 template void launch_march64<float>(MarchArgs<float>&, bool, int, int, hipStream_t);
 template void launch_march64<double>(MarchArgs<double>&, bool, int, int, hipStream_t);
+
+template <typename real>
+void launch_march64_mix(MarchArgs<real>& a, bool exact, int nw, int sets, hipStream_t s) {
+  if (a.ntiles == 0) return;
+  if (nw != 12) throw std::runtime_error("march64: the mixed launch runs 12-wave workgroups");
+  const dim3 g(a.ntiles), b(12 * 64);
+  // sets besides psi-free and z-only: bit 1 y-only, bit 0 x-only
+  switch (sets) {
+    case 0: exact ? k_yee_m64_mix<real, true, 12, 0><<<g, b, 0, s>>>(a) : k_yee_m64_mix<real, false, 12, 0><<<g, b, 0, s>>>(a); return;
+    case 2: exact ? k_yee_m64_mix<real, true, 12, 2><<<g, b, 0, s>>>(a) : k_yee_m64_mix<real, false, 12, 2><<<g, b, 0, s>>>(a); return;
+    case 3: exact ? k_yee_m64_mix<real, true, 12, 3><<<g, b, 0, s>>>(a) : k_yee_m64_mix<real, false, 12, 3><<<g, b, 0, s>>>(a); return;
+    default: break;
+  }
+  throw std::runtime_error("march64: no mixed kernel for these CPML axis sets");
+}
+
+template void launch_march64_mix<float>(MarchArgs<float>&, bool, int, int, hipStream_t);
+template void launch_march64_mix<double>(MarchArgs<double>&, bool, int, int, hipStream_t);
 
 }  // namespace pfhip

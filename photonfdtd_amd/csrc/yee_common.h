@@ -152,6 +152,31 @@ __device__ __forceinline__ void bst_nt<double>(double* base, uint32_t boff, doub
                                         BUF_NT);
 }
 
+// Buffer access through a resource made once (buf_rsrc) with a wave-uniform
+// byte offset in the scalar operand: the address is base + soff + voff, and
+// only voff is range-checked, so a lane with voff = BUF_NONE still does no
+// access. Callers keep soff below 2^31 (and voff of a taking-part lane below
+// 2^31), so base + soff + voff never wraps.
+template <typename real>
+__device__ __forceinline__ real blds(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float blds<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double blds<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <int AUX = 0>
+__device__ __forceinline__ void bsts(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, AUX);
+}
+template <int AUX = 0>
+__device__ __forceinline__ void bsts(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double v) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, AUX);
+}
+
 // Value of lane l+1 / l-1 of the wavefront (DPP wavefront shifts; the lane
 // with no source keeps its own value).
 __device__ __forceinline__ float lane_next(float v) {

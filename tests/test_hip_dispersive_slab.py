@@ -40,7 +40,8 @@ def oracle_run():
 def _gpu(precision, exact, env, monkeypatch):
     import photonfdtd_amd as pf
     from photonfdtd_amd.hip_backend import HipEngine
-    for k in ("PHOTONFDTD_HIP_ZPAD", "PHOTONFDTD_HIP_SPLIT", "PHOTONFDTD_HIP_PAIR"):
+    for k in ("PHOTONFDTD_HIP_ZPAD", "PHOTONFDTD_HIP_SPLIT", "PHOTONFDTD_HIP_PAIR",
+              "PHOTONFDTD_HIP_MIX", "PHOTONFDTD_HIP_EVEN_X", "PHOTONFDTD_HIP_XCHUNK"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
@@ -97,6 +98,64 @@ def test_other_layouts_same_bits_as_split_march(precision, exact, zpad, monkeypa
     ib, fb, db = _gpu(precision, exact, {"PHOTONFDTD_HIP_ZPAD": zpad}, monkeypatch)
     assert ia["tiles_bulk"] > 0 and ib["zpad"] == int(zpad), (ia, ib)
     assert (ib["tiles_bulk"] > 0) == (zpad != "0"), ib
+    for c in FIELDS:
+        assert np.array_equal(fa[c], fb[c]), c
+    for nm in ("in", "out"):
+        for c in da[nm]:
+            assert np.array_equal(da[nm][c], db[nm][c]), (nm, c)
+
+
+@pytest.mark.parametrize("mix", ["4", "12", "28"])
+@pytest.mark.parametrize("precision,exact", [("float64", True), ("float32", False)])
+def test_mixed_launch_same_bits_as_per_list_launches(precision, exact, mix, monkeypatch):
+    """The z-only (mix 4), z- and y-only (mix 12) or z-, y- and x-only (mix
+    28) CPML tiles launched with the psi-free tiles in one spatially ordered
+    launch, over x chunks of nearly equal length, give the bits of one launch
+    per list over 48-plane chunks."""
+    print(f"[case] dispersive slab mixed launch {mix} {precision}", flush=True)
+    # one launch per list, x chunks of at most 48 planes with a short last one
+    env = dict(M64, PHOTONFDTD_HIP_MIX="0", PHOTONFDTD_HIP_EVEN_X="0", PHOTONFDTD_HIP_XCHUNK="48")
+    ia, fa, da = _gpu(precision, exact, env, monkeypatch)
+    env = dict(M64, PHOTONFDTD_HIP_MIX=mix)   # x chunks of nearly equal length
+    ib, fb, db = _gpu(precision, exact, env, monkeypatch)
+    assert ia["cells_bulk"] > 0 and ib["cells_bulk"] > 0 and ib["cpml_lists"] == 0b1111, (ia, ib)
+    for c in FIELDS:
+        assert np.array_equal(fa[c], fb[c]), c
+    for nm in ("in", "out"):
+        for c in da[nm]:
+            assert np.array_equal(da[nm][c], db[nm][c]), (nm, c)
+
+
+def test_full_size_64_lane_march_same_bits_as_padded_march(monkeypatch):
+    """At 512^3 float32 the 64-lane march's scalar offsets (plane, component
+    and psi-slot offsets) reach beyond 1 GiB; the padded march addresses the
+    same nodes with per-plane base pointers. 60 steps of the benchmark scene
+    on both layouts give the same bits (fields and both DFT planes)."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.hip_backend import HipEngine
+
+    def run(zpad):
+        for k in ("PHOTONFDTD_HIP_SPLIT", "PHOTONFDTD_HIP_PAIR", "PHOTONFDTD_HIP_MIX"):
+            monkeypatch.delenv(k, raising=False)
+        monkeypatch.setenv("PHOTONFDTD_HIP_ZPAD", zpad)
+        sim = dispersive_slab(pf, "hip", shape=(512, 512, 512), steps=60, precision="float32")
+        eng = HipEngine(sim, np.float32, False)
+        info = eng.info()
+        eng.advance(0, sim.n_steps)
+        out = {c: eng.get(c) for c in FIELDS}
+        dft = {nm: eng.dft(nm) for nm in ("in", "out")}
+        eng.close()
+        return info, out, dft
+
+    print("[case] dispersive slab 512^3 float32, 64-lane vs padded march, 60 steps", flush=True)
+    ia, fa, da = run("64")
+    assert ia["zpad"] == 64 and ia["tiles_bulk"] > 0, ia
+    ib, fb, db = run("62")
+    assert ib["zpad"] == 62, ib
+    assert float(np.abs(fa["Ey"]).max()) > 0
+    # the pulse reaches the "in" plane (27 cells from the dipole) within the
+    # 60 steps, not the "out" plane
+    assert float(np.abs(da["in"]["Ey"]).max()) > 0
     for c in FIELDS:
         assert np.array_equal(fa[c], fb[c]), c
     for nm in ("in", "out"):
