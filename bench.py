@@ -880,6 +880,91 @@ def bench_surrogate(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_nn_gemm(args):
+    """Training GEMMs of the surrogate network: every layer's forward, data-
+    gradient and weight-gradient pass at the training batch size, timed with
+    HIP events on the stream the kernels run on. FLOPs are the convolution's
+    own (2 x output pixels x cout x (taps x cin) per pass), not the zero taps
+    the data-gradient GEMM of a stride-2 layer multiplies."""
+    import ctypes as C
+    import torch
+    from photonfdtd_amd import _native as N
+    from photonfdtd_amd.surrogate import layer_table
+    lib = N.load_library()
+    dev = torch.device("cuda")
+    B = args.nn_batch
+    reps = 20
+    st = torch.cuda.current_stream()
+    sp = C.c_void_p(st.cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    layers, h = [], 128
+    table = layer_table(16)
+    rows, tot_flops, tot_ms = [], 0.0, {"forward": 0.0, "data_grad": 0.0, "weight_grad": 0.0}
+    input_grad_us = 0.0
+    g = torch.Generator(device="cpu").manual_seed(0)
+    for li, (ci, co, k, s, relu) in enumerate(table):
+        if li == len(table) - 2:
+            h = 1                               # head: 1x1 layers on a 1x1 map
+        ho = (h - 1) // s + 1
+        d = N.PfNnConv()
+        d.n, d.h, d.w, d.cin, d.cout, d.ksize, d.stride, d.relu = B, h, h, ci, co, k, s, relu
+        d.out_f32 = int(li == len(table) - 1)
+        x = torch.randn(B, h, h, ci, generator=g).to(torch.bfloat16).to(dev)
+        w = (torch.randn(co, k, k, ci, generator=g) / np.sqrt(k * k * ci)).to(torch.bfloat16).to(dev)
+        b = torch.zeros(co, device=dev)
+        y = torch.empty(B, ho, ho, co, dtype=torch.float32 if d.out_f32 else torch.bfloat16, device=dev)
+        dy = torch.randn(B, ho, ho, co, generator=g).to(dev)
+        dx = torch.empty(B, h, h, ci, device=dev)
+        dw = torch.zeros(co, k, k, ci, device=dev)
+        flops = 2.0 * B * ho * ho * co * k * k * ci
+        times = {}
+        for name, fn in (
+                ("forward", lambda: lib.pf_nn_conv_forward(C.byref(d), ptr(x), ptr(w), ptr(b), ptr(y), sp)),
+                ("data_grad", lambda: lib.pf_nn_conv_backward(C.byref(d), ptr(x), ptr(w), ptr(y), ptr(dy),
+                                                             ptr(dx), None, None, sp)),
+                ("weight_grad", lambda: lib.pf_nn_conv_backward(C.byref(d), ptr(x), ptr(w), ptr(y), ptr(dy),
+                                                               None, ptr(dw), None, sp))):
+            if li == 0 and name == "data_grad":
+                # the input map needs no gradient in training (the design
+                # loop's ascent on the network takes it): timed, not counted
+                for _ in range(3):
+                    assert fn() == 0
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                for _ in range(reps):
+                    fn()
+                e1.record(st)
+                e1.synchronize()
+                input_grad_us = e0.elapsed_time(e1) / reps * 1e3
+                continue
+            for _ in range(3):
+                assert fn() == 0
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(reps):
+                fn()
+            e1.record(st)
+            e1.synchronize()
+            times[name] = e0.elapsed_time(e1) / reps
+            tot_ms[name] += times[name]
+            tot_flops += flops
+        rows.append({"layer": li, "shape": [B, h, h, ci, co, k, s],
+                     "gflop_per_pass": round(flops / 1e9, 3),
+                     **{f"{nm}_us": round(t * 1e3, 1) for nm, t in times.items()},
+                     **{f"{nm}_tflops": round(flops / (t * 1e-3) / 1e12, 1) for nm, t in times.items()}})
+        h = ho
+    total_ms = sum(tot_ms.values())
+    out = {"metric": "surrogate training GEMM TFLOP/s (bf16 MFMA, FP32 accumulation)",
+           "value": round(tot_flops / (total_ms * 1e-3) / 1e12, 2), "unit": "TFLOP/s",
+           "batch": B, "ms_per_training_pass": round(total_ms, 3),
+           "per_pass_ms": {k: round(v, 3) for k, v in tot_ms.items()},
+           "input_grad_us": round(input_grad_us, 1),
+           "roofline": {"bound": "mfma", "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": round(tot_flops / (total_ms * 1e-3) / 1e12 / 2500.0, 4)},
+           "layers": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -893,10 +978,12 @@ def main():
     ap.add_argument("--fd-voxels", type=int, default=20, help="adjoint3d: central-difference voxels")
     ap.add_argument("--workload", default="slab3d",
                     choices=("slab3d", "batched2d", "batched3d", "surrogate", "adjoint3d",
-                             "stream3d"),
+                             "stream3d", "nn_gemm"),
                     help="slab3d: the 3-D dispersive slab (headline); batched2d: 64 uneven "
                          "2-D TE scenes in one launch; batched3d: 16 uneven 3-D scenes in one "
                          "batch")
+    ap.add_argument("--nn-batch", type=int, default=256,
+                    help="nn_gemm: design maps per training batch")
     ap.add_argument("--tile-rows", type=int, default=64,
                     help="stream3d: owned x planes per streamed tile")
     ap.add_argument("--batch-config", default=None, metavar="MODE,WAVES,ROWS",
@@ -965,6 +1052,9 @@ def main():
         return
     if args.workload == "stream3d":
         bench_stream3d(args)
+        return
+    if args.workload == "nn_gemm":
+        bench_nn_gemm(args)
         return
 
     import photonfdtd_amd as pf

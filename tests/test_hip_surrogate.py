@@ -65,6 +65,9 @@ def run_layer(n, h, w, ci, co, k, s, relu, out_f32, seed=0):
     (3, 9, 9, 64, 40, 3, 1, False, True),
     (5, 1, 1, 256, 512, 1, 1, True, False),
     (5, 1, 1, 512, 16, 1, 1, False, True),
+    (3, 15, 13, 1, 64, 3, 2, True, False),    # one input channel, stride 2
+    (2, 9, 8, 16, 24, 1, 2, True, False),     # 1x1 stride 2: odd pixels get no gradient
+    (4, 12, 12, 128, 256, 3, 2, True, False), # 128-wide tiles
 ])
 def test_layer_matches_cpu(cfg):
     print(f"\n[case] layer {cfg}", flush=True)
