@@ -133,6 +133,12 @@ typedef struct pf_desc {
  * media, flux monitors, subpixel coefficients, checkpoints / adjoint,
  * pf_field_ptr (the fields are not on the device). */
 #define PF_FLAG_STREAM 2
+/* Step with the 64-lane march on dense z rows (nz a multiple of 64, so a
+ * field's layout is the generic kernels' C order): pf_checkpoint_enable /
+ * pf_adjoint then work on the fused stepper, whose forward run is the march
+ * (no reference counterpart; pf_adjoint's own steps are unchanged).
+ * pf_checkpoint_enable refuses a handle that did not end up in that layout. */
+#define PF_FLAG_DENSE64 4
 
 /* Cotangent of one DFT accumulator: g = dJ/d(Re F) + i dJ/d(Im F) for monitor
  * `mon`, component slot `slot`, complex128 interleaved [n_freq][box cells]. */

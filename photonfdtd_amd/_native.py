@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpfhip.so"
 ABI_VERSION = 9
 PF_FLAG_GENERIC = 1
 PF_FLAG_STREAM = 2
+PF_FLAG_DENSE64 = 4
 
 PF_F32, PF_F64 = 0, 1
 KERNEL_CLASSES = ("yee_h", "yee_e", "yee_fused", "aux", "yee_pair", "dft_fold")

@@ -57,6 +57,22 @@ def _with_eps(sim, eps: np.ndarray):
     return s
 
 
+def _checkpointed_engine(run, npdt, interval):
+    """A stepper that stores checkpoints for the adjoint: the 64-lane march on
+    dense z rows when nz is a multiple of 64 (its fields are then in the
+    generic layout the reverse sweep reads), else the generic kernels."""
+    if run.grid.shape[2] % 64 == 0:
+        eng = HipEngine(run, npdt, exact_mode(npdt), flags=N.PF_FLAG_DENSE64)
+        try:
+            eng.enable_checkpoints(interval)
+            return eng
+        except (RuntimeError, ValueError):
+            eng.close()    # the stepper did not end up in that layout
+    eng = HipEngine(run, npdt, exact_mode(npdt), flags=N.PF_FLAG_GENERIC)
+    eng.enable_checkpoints(interval)
+    return eng
+
+
 def _make_function():
     torch = _require_torch()
 
@@ -69,8 +85,7 @@ def _make_function():
             _check_sim(sim)
             run = _with_eps(sim, eps.detach().cpu().numpy())
             npdt = validate(run)
-            eng = HipEngine(run, npdt, exact_mode(npdt), flags=N.PF_FLAG_GENERIC)
-            eng.enable_checkpoints(checkpoint_interval)
+            eng = _checkpointed_engine(run, npdt, checkpoint_interval)
             res = run_loop(run, eng)
             ctx.engine = eng
             ctx.keys = _dft_keys(sim)

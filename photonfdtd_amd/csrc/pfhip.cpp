@@ -188,7 +188,7 @@ class Stepper final : public pf_handle {
     if (stream_) {
       (void)hipStreamSynchronize(stream_);
       for (Timed& t : pending_) {
-        (void)hipEventDestroy(t.a);
+        if (!t.shared_start) (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
       }
       for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
@@ -367,7 +367,9 @@ class Stepper final : public pf_handle {
   real* coef_ = nullptr;
   // March tile lists: [0] psi-free, [1] CPML on several axes (or any axis
   // with padded rows); with dense rows [2] z-only, [3] y-only, [4] x-only CPML
-  // tiles, each launched with the kernel variant of its axis set.
+  // tiles, each launched with the kernel variant of its axis set. (Lists of
+  // y+z and x+z tiles with their own variants, 80 VGPRs, measured slower than
+  // running them on the several-axis variant.)
   static constexpr int NLIST = 5;
   static constexpr int list_mask_[NLIST] = {0, 7, 4, 2, 1};
   int* tiles_[NLIST] = {};
@@ -427,12 +429,17 @@ class Stepper final : public pf_handle {
   struct Timed {
     hipEvent_t a, b;
     int cls;
+    bool shared_start;   // a is the previous entry's b
   };
   bool timing_ = false;
   std::vector<Timed> pending_;
   std::vector<hipEvent_t> pool_;
   int open_cls_ = -1;            // kernel class of the timed group being issued
   hipEvent_t open_ev_ = nullptr; // its start event
+  // close event of the last top-level group while nothing has been issued on
+  // the stream after it: the next group starts at that event instead of
+  // recording one of its own (two events per step fewer)
+  hipEvent_t chain_ev_ = nullptr;
   hipEvent_t record_event();
   double t_ms_[PF_NKCLASS] = {};
   int64_t t_count_[PF_NKCLASS] = {};
@@ -716,7 +723,8 @@ Stepper<real>::Stepper(const pf_desc& d, int zpad_default) {
     // most tiles touch a slab: 0.082 against 0.113 ms at 128^3); 0: dense
     // rows with the 62-node march
     const int zdef = zpad_default ? zpad_default
-                                  : ncell >= (int64_t(1) << 25) ? pfhip::ZPAD_BLOCK : 62;
+                     : (d.flags & PF_FLAG_DENSE64) && nz % 64 == 0 ? pfhip::ZPAD_BLOCK
+                     : ncell >= (int64_t(1) << 25) ? pfhip::ZPAD_BLOCK : 62;
     const int zenv = env_zpad ? std::atoi(env_zpad) : -1;
     zpad_ = variant == 2 && zenv != 0
                 ? (env_pair && env_pair[0] == '1'
@@ -1252,17 +1260,22 @@ void Stepper<real>::timed(int cls, F&& launch) {
     return;
   }
   const int outer = open_cls_;
-  if (outer >= 0) pending_.push_back({open_ev_, record_event(), outer});
+  if (outer >= 0) pending_.push_back({open_ev_, record_event(), outer, false});
   open_cls_ = cls;
-  open_ev_ = record_event();
+  const bool chained = outer < 0 && chain_ev_;
+  open_ev_ = chained ? chain_ev_ : record_event();
+  chain_ev_ = nullptr;
   launch();
-  pending_.push_back({open_ev_, record_event(), cls});
+  hipEvent_t close = record_event();
+  pending_.push_back({open_ev_, close, cls, chained});
   open_cls_ = outer;
   if (outer >= 0) open_ev_ = record_event();
+  else chain_ev_ = close;
 }
 
 template <typename real>
 void Stepper<real>::collect_timing() {
+  chain_ev_ = nullptr;
   if (pending_.empty()) return;
   HIP_CHECK(hipStreamSynchronize(stream_));
   for (Timed& t : pending_) {
@@ -1270,7 +1283,7 @@ void Stepper<real>::collect_timing() {
     HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
     t_ms_[t.cls] += ms;
     t_count_[t.cls] += 1;
-    pool_.push_back(t.a);
+    if (!t.shared_start) pool_.push_back(t.a);   // a shared start is the previous entry's end
     pool_.push_back(t.b);
   }
   pending_.clear();
@@ -1375,6 +1388,7 @@ void Stepper<real>::after_march(int64_t step) {
     ade_n_jobs1_ = (int)jobs.size();
     for (const pfhip::AdeJob<real>& J : jobs) ade_max_n1_ = std::max(ade_max_n1_, J.n);
     tab = arena_.upload(jobs.data(), jobs.size(), stream_);
+    chain_ev_ = nullptr;   // the upload is not part of the next group
   }
   timed(PF_K_AUX, [&] {
     if (tab && ade_n_jobs1_ > 0) {
@@ -1546,12 +1560,16 @@ void Stepper<real>::adopt_stream(hipStream_t s) {
 template <typename real>
 void Stepper<real>::save_state(real* dst) {
   const size_t nf = 6 * (size_t)(shape[0] * shape[1] * shape[2]);
-  HIP_CHECK(hipMemcpyAsync(dst, lvl_[0], nf * sizeof(real), hipMemcpyDeviceToDevice, stream_));
+  // the fused stepper: its current level (dense rows of nz slots, checked
+  // by checkpoint_enable) and that level's H-side psi
+  const real* lv = fused_ ? lvl_[cur_] : lvl_[0];
+  const real* ph = fused_ && cur_ == 1 ? psi_h_lvl1_ : psi_side_[1];
+  HIP_CHECK(hipMemcpyAsync(dst, lv, nf * sizeof(real), hipMemcpyDeviceToDevice, stream_));
   if (psi_len_e_)
     HIP_CHECK(hipMemcpyAsync(dst + nf, psi_side_[0], psi_len_e_ * sizeof(real),
                              hipMemcpyDeviceToDevice, stream_));
   if (psi_len_h_)
-    HIP_CHECK(hipMemcpyAsync(dst + nf + psi_len_e_, psi_side_[1], psi_len_h_ * sizeof(real),
+    HIP_CHECK(hipMemcpyAsync(dst + nf + psi_len_e_, ph, psi_len_h_ * sizeof(real),
                              hipMemcpyDeviceToDevice, stream_));
 }
 
@@ -1720,7 +1738,11 @@ void Stepper<real>::reset(hipStream_t s) {
 
 template <typename real>
 void Stepper<real>::checkpoint_enable(int interval) {
-  if (fused_) throw ArgError{"checkpoints need the generic kernels: create with PF_FLAG_GENERIC"};
+  // the fused stepper qualifies when its fields are in the generic layout
+  // (dense z rows of exactly nz slots) and it steps one step at a time
+  if (fused_ && !(fa_[0].variant == 2 && zpad_ == pfhip::ZPAD_BLOCK && zrow_ == shape[2] && !pair_))
+    throw ArgError{"checkpoints need the generic layout: create with PF_FLAG_GENERIC, or with "
+                   "PF_FLAG_DENSE64 on a grid whose nz is a multiple of 64"};
   if (args_.ce_y) throw ArgError{"the adjoint does not support subpixel smoothing"};
   if (interval < 1) throw ArgError{"checkpoint interval must be positive"};
   if (!ade_.empty()) throw ArgError{"the adjoint does not support dispersive media"};
@@ -1842,16 +1864,19 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
                                            args_.nyz, args_.nz, cot_dev[m], d.n_freq, d.phase_e,
                                            d.phase_h, st / d.interval, d.dt, stream_);
       }
-      // rebuild H^{n+1/2} from the stored pre-step state: H and its psi are
-      // copied back into the live arrays (the H update works in place), E^n
-      // and the E-side psi are read where the trajectory holds them
-      HIP_CHECK(hipMemcpyAsync(f_[0][3], T + 3 * (size_t)ncell, 3 * (size_t)ncell * sizeof(real),
-                               hipMemcpyDeviceToDevice, stream_));
-      if (psi_len_h_)
-        HIP_CHECK(hipMemcpyAsync(psi_side_[1], T + nf + psi_len_e_, psi_len_h_ * sizeof(real),
-                                 hipMemcpyDeviceToDevice, stream_));
+      // rebuild H^{n+1/2} from the stored pre-step state, out of place: E^n,
+      // H^{n-1/2} and the H-side psi are read where the trajectory holds them,
+      // the new H (every node) goes to the live arrays and the new H-side psi
+      // to the live psi, which nothing reads (the transposed kernels take
+      // both psi sides from the trajectory)
       pfhip::StepArgs<real> hs = args_;
-      for (int c = 0; c < 3; ++c) hs.f[c] = const_cast<real*>(T) + (size_t)c * ncell;
+      for (int c = 0; c < 6; ++c) hs.f[c] = const_cast<real*>(T) + (size_t)c * ncell;
+      for (int c = 3; c < 6; ++c) hs.fo[c] = f_[0][c];
+      for (int sl = 6; sl < pfhip::PSI_COUNT; ++sl)
+        if (args_.psi[sl]) {
+          hs.psi[sl] = const_cast<real*>(T) + nf + psi_off[sl];
+          hs.psio[sl] = args_.psi[sl];
+        }
       pfhip::launch_generic_h<real>(hs, exact_, stream_);
       pfhip::launch_inject<real>(fields_dev_[0], src_h_, n_steps_, st, stream_);
       for (int c = 0; c < 3; ++c) a.f[c] = T + (size_t)c * ncell;
@@ -1932,6 +1957,7 @@ void Stepper<real>::advance(int64_t step0, int64_t n) {
     if (ckpt_interval_ > 0 && s % ckpt_interval_ == 0) {
       if (s / ckpt_interval_ >= (int64_t)ckpts_.size()) throw ArgError{"step beyond the checkpoint table"};
       save_state(ckpts_[s / ckpt_interval_]);
+      chain_ev_ = nullptr;   // the copies are not part of the next group
     }
     if (pair_ && ckpt_interval_ == 0 && s + 2 <= step0 + n) {
       step_pair(s);

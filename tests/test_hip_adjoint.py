@@ -23,6 +23,20 @@ def small_3d(pf, precision="float64"):
                          precision=precision, backend="hip")
 
 
+def tall_3d(pf, precision="float64"):
+    """small_3d with 64 z cells: the adjoint's forward run takes the 64-lane
+    march on dense rows (PF_FLAG_DENSE64)."""
+    grid = pf.Grid(size=(12 * 80e-9, 10 * 80e-9, 64 * 80e-9), cell_size=80e-9, pml_layers=(3, 3, 3))
+    box = pf.Box(center=(0.1e-6, 0, 0.4e-6), size=(0.3e-6, 0.3e-6, 0.5e-6), medium=pf.Medium(3.0))
+    src = pf.PointDipole(position=(-0.2e-6, 0, 0.2e-6), component="Ey",
+                         waveform=pf.GaussianPulse(freq0=pf.C_0 / 1e-6, fwhm=4e-15))
+    dft = pf.DFTMonitor(name="p", components=("Ey", "Ez", "Hy", "Hz"),
+                        freqs=[pf.C_0 / 1.1e-6, pf.C_0 / 0.9e-6], plane_axis="x",
+                        plane_position=0.25e-6)
+    return pf.Simulation(grid, structures=[box], sources=[src], monitors=[dft], run_time=5e-15,
+                         precision=precision, backend="hip")
+
+
 def te_2d(pf):
     grid = pf.Grid(size=(1.2e-6, 1.0e-6), cell_size=(40e-9, 40e-9), pml_layers=(6, 6, 0))
     poly = pf.PolySlab(vertices=((-0.2e-6, -0.15e-6), (0.25e-6, -0.1e-6), (0.1e-6, 0.2e-6)),
@@ -55,12 +69,12 @@ def reference_grad(sim):
     return float(val.detach()), eps.grad.numpy()
 
 
-@pytest.mark.parametrize("scene", ["small_3d", "te_2d"])
+@pytest.mark.parametrize("scene", ["small_3d", "te_2d", "tall_3d"])
 def test_adjoint_matches_exact_autograd(scene):
     import photonfdtd_amd as pf
     from photonfdtd_amd.adjoint import value_and_grad_eps
     print(f"\n[case] adjoint vs torch autograd {scene}", flush=True)
-    build = small_3d if scene == "small_3d" else te_2d
+    build = {"small_3d": small_3d, "te_2d": te_2d, "tall_3d": tall_3d}[scene]
     v_ref, g_ref = reference_grad(build(pf))
     v, g = value_and_grad_eps(build(pf), loss_of, checkpoint_interval=7)
     rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
@@ -126,3 +140,34 @@ def test_dispersive_scene_is_rejected():
     sim._has_dispersion = True
     with pytest.raises(NotImplementedError):
         value_and_grad_eps(sim, loss_of)
+
+
+def test_dense64_forward_runs_the_march_and_matches_generic():
+    """With nz a multiple of 64 the checkpointed forward run is the 64-lane
+    march (fused path, dense rows of nz slots); its float64 gradient has the
+    bits of the generic-kernel run."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd import _native as N
+    from photonfdtd_amd.adjoint import _checkpointed_engine, value_and_grad_eps
+    from photonfdtd_amd.hip_backend import HipEngine, exact_mode, validate
+    print("\n[case] adjoint dense-64 march forward vs generic", flush=True)
+    sim = tall_3d(pf)
+    eng = _checkpointed_engine(sim, validate(sim), 7)
+    info = eng.info()
+    eng.close()
+    assert info["path"] == 2 and info["zpad"] == 64 and info["zrow"] == 64, info
+    _, g_march = value_and_grad_eps(tall_3d(pf), loss_of, checkpoint_interval=7)
+
+    import photonfdtd_amd.adjoint as adj
+
+    def generic_only(run, npdt, interval):
+        e = HipEngine(run, npdt, exact_mode(npdt), flags=N.PF_FLAG_GENERIC)
+        e.enable_checkpoints(interval)
+        return e
+    orig = adj._checkpointed_engine
+    adj._checkpointed_engine = generic_only
+    try:
+        _, g_gen = value_and_grad_eps(tall_3d(pf), loss_of, checkpoint_interval=7)
+    finally:
+        adj._checkpointed_engine = orig
+    assert np.array_equal(g_march, g_gen)
