@@ -788,7 +788,12 @@ def bench_surrogate(args):
     log(f"[surrogate] setup {time.perf_counter() - t0:.1f} s")
     X = D.training_designs(n_train + n_test)
     t1 = time.perf_counter()
-    Y = lab(X)
+    parts = []
+    for c0 in range(0, len(X), 2048):        # progress every 2048 designs
+        parts.append(lab(X[c0:c0 + 2048]))
+        log(f"[surrogate] labelled {min(c0 + 2048, len(X))}/{len(X)} designs "
+            f"({time.perf_counter() - t1:.1f} s)")
+    Y = np.concatenate(parts)
     t_label = time.perf_counter() - t1
     cells = lab.sim.grid.n_cells
     log(f"[surrogate] {len(X)} labels in {t_label:.1f} s "
@@ -822,7 +827,7 @@ def bench_surrogate(args):
     it_target = next(h["iter"] for h in hist if h["objective"] >= target)
     t_plain_target = t_plain * (it_target + 1) / len(hist)
     # candidates per round are verified in one batch of their own size
-    n_cand = 48
+    n_cand = args.candidates
     lab_v = D.Labeler(pf, ref, batch=n_cand, steps=steps)
     t4 = time.perf_counter()
     sur = D.surrogate_design(pf, net, lab_v, ref, rho0, rounds=args.rounds, target=target,
@@ -975,6 +980,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=12, help="surrogate training epochs")
     ap.add_argument("--adjoint-iters", type=int, default=12, help="plain adjoint iterations")
     ap.add_argument("--rounds", type=int, default=6, help="surrogate design rounds")
+    ap.add_argument("--candidates", type=int, default=48,
+                    help="surrogate: designs verified per round (one batched FDTD run)")
     ap.add_argument("--fd-voxels", type=int, default=20, help="adjoint3d: central-difference voxels")
     ap.add_argument("--workload", default="slab3d",
                     choices=("slab3d", "batched2d", "batched3d", "surrogate", "adjoint3d",
