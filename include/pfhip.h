@@ -248,6 +248,7 @@ typedef struct pf_info {
   int64_t pair_tiles_bulk;
   int64_t pair_tiles_cpml;
   int64_t dft_ring;
+  int64_t post_fused;   /* 1: ADE, E sources and DFT samples in one launch per step */
 } pf_info;
 int pf_stepper_info(pf_handle* h, pf_info* out);
 /* Accumulated flux of every flux monitor, [n_flux] doubles. */
@@ -303,8 +304,12 @@ int pf_batch_dft_read(pf_batch* b, int32_t sim, int32_t mon, int32_t slot, doubl
 int pf_batch_read_field(pf_batch* b, int32_t sim, int32_t comp, void* dst);
 /* Event timing (no reference counterpart; used by the benchmark): when on,
  * each kernel-class launch group of pf_advance is bracketed by HIP events on
- * the handle's stream. Enabling resets the totals. pf_timing_read returns,
- * per class, total milliseconds and launch counts ([PF_NKCLASS] each). */
+ * the handle's stream. on = 0: off; 1: every group; k > 1: the march and aux
+ * groups of every k-th single step only (their totals and counts cover those
+ * steps; the other classes are timed in every step), which keeps the event
+ * packets out of most steps. Enabling resets the totals. pf_timing_read
+ * returns, per class, total milliseconds and launch counts ([PF_NKCLASS]
+ * each). */
 int pf_timing_enable(pf_handle* h, int32_t on);
 int pf_timing_read(pf_handle* h, double* ms, int64_t* launches);
 /* Replaces CudaStepper.device_info (rust/src/cuda.rs:560-573). */

@@ -70,7 +70,7 @@ class PfInfo(C.Structure):
     _fields_ = [("path", _i32), ("zpad", _i32), ("zrow", _i32), ("march_nw", _i32),
                 ("pml_nw", _i32), ("cpml_lists", _i32), ("tiles_bulk", _i64), ("tiles_cpml", _i64),
                 ("cells_bulk", _i64), ("cells_cpml", _i64), ("pair_tiles_bulk", _i64),
-                ("pair_tiles_cpml", _i64), ("dft_ring", _i64)]
+                ("pair_tiles_cpml", _i64), ("dft_ring", _i64), ("post_fused", _i64)]
 
 
 class PfCotangent(C.Structure):

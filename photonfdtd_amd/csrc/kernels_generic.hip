@@ -343,19 +343,10 @@ __global__ void k_ade_apply(real* const* __restrict__ fields, const int64_t* __r
   }
 }
 
-// Copy a strided box of one or more components into a dense buffer:
-// out[(c * cstride) + flat box index] = field_c[node] (cstride = the box's
-// node count, or that of a larger box whose x slab this one is).
-template <typename real>
-__global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __restrict__ comps,
-                             int n_comp, BoxDev box, int64_t plane, int row, int zpad,
-                             int64_t cstride, real* __restrict__ out) {
-  // blockIdx.y = component (its field pointer is one scalar load per wave);
-  // x covers the box's nodes, z fastest
-  const int c = blockIdx.y;
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
-  if (r >= nbox) return;
+// Field index of flat box node r (z fastest): the element order of the
+// gathered planes.
+__device__ inline int64_t box_node(const BoxDev& box, int64_t r, int64_t nbox, int64_t plane,
+                                   int row, int zpad) {
   int64_t bi, bj, bk;
   if (nbox <= 0x7FFFFFFF) {
     const uint32_t u = (uint32_t)r, c2 = (uint32_t)box.count[2], c1 = (uint32_t)box.count[1];
@@ -369,11 +360,73 @@ __global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __
     bi = q / box.count[1];
     bj = q - bi * box.count[1];
   }
+  return (box.start[0] + bi * box.stride[0]) * plane + (box.start[1] + bj * box.stride[1]) * row +
+         zslot(box.start[2] + bk * box.stride[2], zpad);
+}
+
+// Copy a strided box of one or more components into a dense buffer:
+// out[(c * cstride) + flat box index] = field_c[node] (cstride = the box's
+// node count, or that of a larger box whose x slab this one is).
+template <typename real>
+__global__ void k_gather_box(real* const* __restrict__ fields, const int32_t* __restrict__ comps,
+                             int n_comp, BoxDev box, int64_t plane, int row, int zpad,
+                             int64_t cstride, real* __restrict__ out) {
+  // blockIdx.y = component (its field pointer is one scalar load per wave);
+  // x covers the box's nodes, z fastest
+  const int c = blockIdx.y;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nbox = box.count[0] * box.count[1] * box.count[2];
+  if (r >= nbox) return;
   const real* F = fields[comps[c]];
-  const int64_t node = (box.start[0] + bi * box.stride[0]) * plane +
-                       (box.start[1] + bj * box.stride[1]) * row +
-                       zslot(box.start[2] + bk * box.stride[2], zpad);
-  out[(int64_t)c * cstride + r] = F[node];
+  out[(int64_t)c * cstride + r] = F[box_node(box, r, nbox, plane, row, zpad)];
+}
+
+// The step's work after the march, in one launch: ADE correction of every
+// dispersive medium, the E-side soft sources and the DFT planes' samples.
+// The host takes this path only when no source cell is a dispersive cell and
+// no sampled E node is one of either, so the three parts touch disjoint E
+// nodes and their order does not matter; each part computes exactly what its
+// own kernel does (k_ade_jobs, k_inject, k_gather_box). Blocks
+// [0, ade_blk[n_ade]) are ADE nodes (job j owns [ade_blk[j], ade_blk[j+1])),
+// then src_blk source-group blocks, then the gathers (gather d owns
+// [dft_blk[d], dft_blk[d+1]), blocks_c per component).
+template <typename real>
+__global__ __launch_bounds__(256) void k_post_step(PostArgs<real> P) {
+  const int b = (int)blockIdx.x;
+  const int tid = (int)threadIdx.x;
+  if (b < P.ade_blk[P.n_ade]) {
+    int j = 0;
+    while (b >= P.ade_blk[j + 1]) ++j;
+    const AdeJob<real>& J = P.ade[j];
+    const int64_t t = (int64_t)(b - P.ade_blk[j]) * 256 + tid;
+    if (t < J.n && J.n_poles > 0)
+      ade_pp_node<real>(J.Ein, J.Eout, J.cells, J.n, J.Pcur, J.Pprev, J.abc, J.n_poles, J.inv_eps, t);
+    return;
+  }
+  const int bs = b - P.ade_blk[P.n_ade];
+  if (bs < P.src_blk) {
+    const int g = bs * 256 + tid;
+    if (g < P.src.n_groups) inject_group<real>(P.src_fields, P.src, P.n_steps, P.step, g);
+    return;
+  }
+  const int bd = bs - P.src_blk;
+  int d = 0;
+  while (d < P.n_dft && bd >= P.dft_blk[d + 1]) ++d;
+  if (d >= P.n_dft) return;
+  const PostGather<real>& G = P.dft[d];
+  const int local = bd - P.dft_blk[d];
+  const int c = local / G.blocks_c;
+  const int64_t r = (int64_t)(local - c * G.blocks_c) * 256 + tid;
+  if (r >= G.nbox) return;
+  const real* F = P.dft_fields[G.comps[c]];
+  G.out[(int64_t)c * G.nbox + r] = F[box_node(G.box, r, G.nbox, P.plane, P.row, P.zpad)];
+}
+
+template <typename real>
+void launch_post_step(const PostArgs<real>& P, hipStream_t s) {
+  const int blocks = P.ade_blk[P.n_ade] + P.src_blk + P.dft_blk[P.n_dft];
+  if (blocks <= 0) return;
+  k_post_step<real><<<blocks, 256, 0, s>>>(P);
 }
 
 // Fold `count` buffered samples into the complex128 accumulators, one sample
@@ -554,6 +607,7 @@ void launch_flux(real* const* fields_host, int axis, int64_t index, int nx, int 
                                        real*, const real*, int, real, hipStream_t);                \
   template void launch_gather_box<real>(real* const*, const int32_t*, int, const BoxDev&, int64_t, \
                                         int, int, real*, hipStream_t, int64_t);                    \
+  template void launch_post_step<real>(const PostArgs<real>&, hipStream_t);                        \
   template void launch_dft_fold<real>(const real*, int64_t, int, const int32_t*, int, int64_t,     \
                                       const double*, const double*, int, double, double*,          \
                                       hipStream_t);                                                \

@@ -212,7 +212,7 @@ class Stepper final : public pf_handle {
   void field_view(int comp, pf_view* out) override;
   void dft_view(int mon, pf_view* out) override;
   void flux_read(double* dst) override;
-  void timing_enable(bool on) override;
+  void timing_enable(int every) override;
   void checkpoint_enable(int interval) override;
   void adjoint(int n_cot, const pf_cotangent* cot, double* grad) override;
   // batched 2-D stepping support
@@ -254,6 +254,7 @@ class Stepper final : public pf_handle {
   int march_dist() const { return march_dist_; }
   int zpad() const { return zpad_; }
   void after_march(int64_t step);
+  void post_step(int64_t step, const pfhip::AdeJob<real>* tab);
   // Device tables of the ADE media for one launch of all media per step,
   // indexed [field level read][polarisation level] (the two-step march
   // swaps the polarisation buffers after every pair, so a single step can
@@ -263,6 +264,10 @@ class Stepper final : public pf_handle {
   pfhip::AdeJob<real>* ade_tab1_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int ade_n_jobs1_ = 0;
   int64_t ade_max_n1_ = 0;
+  std::vector<int64_t> ade_job_n_;   // nodes per job of those tables
+  // ADE media, E-side sources and DFT samples after the march in one launch
+  // (k_post_step): their E nodes are disjoint
+  bool post_ok_ = false;
   void adopt_stream(hipStream_t s);
   // The batched ADE and source passes of a 3-D batch: this simulation's
   // jobs for a step that reads level p; finish_step then flips the level
@@ -329,6 +334,7 @@ class Stepper final : public pf_handle {
   void step_once(int64_t step, bool monitors = true);
   void save_state(real* dst);
   void capture_dft(int64_t step);
+  bool dft_take(Dft& m, int64_t step);
   void fold(Dft& m);
   real* staging(size_t elems);
   const real* gather(int comp, const pf_box& box, int64_t* nbox);
@@ -379,6 +385,11 @@ class Stepper final : public pf_handle {
   // and 3, z- and y-only CPML): one launch over the union in spatial order,
   // so CPML tiles and their psi-free neighbours share the apron lines.
   int mix_mask_ = 0;
+  // lists launched on the side stream (bit q: list q): all-axis, y- and
+  // x-only; with the mixed launch the all-axis list only (the x-only list
+  // follows the mixed launch on the main stream: at 512^3 float32 the side
+  // stream's two lists outlasted the mixed launch by 170 us, one by 80 us)
+  int side_mask_ = -1;
   bool even_x_ = false;           // x chunks of nearly equal length (64-lane march)
   int* tiles_mix_ = nullptr;
   int ntiles_mix_ = 0;
@@ -432,6 +443,10 @@ class Stepper final : public pf_handle {
     bool shared_start;   // a is the previous entry's b
   };
   bool timing_ = false;
+  // time the step groups (march, aux) of every timing_every_-th step only;
+  // other classes (DFT folds, pairs) in every step
+  int timing_every_ = 1;
+  bool skip_step_groups_ = false;
   std::vector<Timed> pending_;
   std::vector<hipEvent_t> pool_;
   int open_cls_ = -1;            // kernel class of the timed group being issued
@@ -684,6 +699,46 @@ Stepper<real>::Stepper(const pf_desc& d, int zpad_default) {
     dft_.push_back(st);
   }
 
+  {
+    // One launch for the ADE media, E-side sources and DFT samples after the
+    // march needs disjoint E nodes: no source on a dispersive cell and no
+    // sampled E node that either of them writes (PHOTONFDTD_HIP_POST=0: one
+    // launch each). At 512^3 float32 (two media, one dipole, two planes of
+    // four components) 18 us per step against 33 us for four launches.
+    const char* env_post = std::getenv("PHOTONFDTD_HIP_POST");
+    int n_media = 0;
+    std::vector<int64_t> disp;
+    for (int m = 0; m < d.n_ade; ++m)
+      if (d.ade[m].n_cells > 0 && d.ade[m].n_poles > 0) {
+        ++n_media;
+        disp.insert(disp.end(), d.ade[m].cells, d.ade[m].cells + d.ade[m].n_cells);
+      }
+    std::sort(disp.begin(), disp.end());
+    bool ok = !(env_post && env_post[0] == '0') && n_media <= pfhip::POST_MAX_ADE &&
+              (int)dft_.size() <= pfhip::POST_MAX_DFT;
+    for (int64_t c : gcell[0])
+      if (ok && std::binary_search(disp.begin(), disp.end(), c)) ok = false;
+    auto in_box = [&](const BoxDev& b, int64_t c) {
+      const int64_t x[3] = {c / ((int64_t)ny * nz), (c / nz) % ny, c % nz};
+      for (int q = 0; q < 3; ++q) {
+        const int64_t u = x[q] - b.start[q];
+        if (u < 0 || u >= b.count[q] * b.stride[q] || u % b.stride[q] != 0) return false;
+      }
+      return true;
+    };
+    for (const Dft& m : dft_) {
+      bool comp_e[3] = {false, false, false};
+      for (int c = 0; c < m.n_comp; ++c)
+        if (m.comps[c] < 3) comp_e[m.comps[c]] = true;
+      if (!(comp_e[0] || comp_e[1] || comp_e[2])) continue;
+      for (size_t t = 0; ok && t < disp.size(); ++t)
+        if (in_box(m.box, disp[t])) ok = false;
+      for (size_t g = 0; ok && g < gcell[0].size(); ++g)
+        if (comp_e[gcomp[0][g]] && in_box(m.box, gcell[0][g])) ok = false;
+    }
+    post_ok_ = ok;
+  }
+
   for (int m = 0; m < d.n_flux; ++m) {
     const pf_flux_desc& s = d.flux[m];
     if (s.axis < 0 || s.axis > 2 || s.index < 0 || s.index >= d.shape[s.axis])
@@ -867,6 +922,8 @@ Stepper<real>::Stepper(const pf_desc& d, int zpad_default) {
   const char* env_pnw = std::getenv("PHOTONFDTD_HIP_PML_NW");
   if (split_ && env_pnw && (std::atoi(env_pnw) == 10 || std::atoi(env_pnw) == 12 || std::atoi(env_pnw) == 16))
     pml_nw_ = std::atoi(env_pnw);
+  const char* env_smask = std::getenv("PHOTONFDTD_HIP_SIDE");
+  side_mask_ = env_smask ? std::atoi(env_smask) & 0b11110 : -1;
   const char* env_side = std::getenv("PHOTONFDTD_HIP_OVERLAP");
   if (fused_ && split_ && !(env_side && env_side[0] == '0')) {
     HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
@@ -968,6 +1025,7 @@ void Stepper<real>::build_march_tiles(int xchunk) {
   if ((mix_mask_ & 0b10000) && !(mix_mask_ & 0b1000)) mix_mask_ &= ~0b10000;   // x only with y
   if (zpad_ != pfhip::ZPAD_BLOCK || !split_ || march_nw_ != 12 || pml_nw_ != 12 || lists[0].empty())
     mix_mask_ = 0;
+  if (side_mask_ < 0) side_mask_ = mix_mask_ ? 0b00010 : 0b11010;
   if (mix_mask_) {
     std::vector<pfhip::MarchTile> m;
     for (int q = 0; q < NLIST; ++q) {
@@ -1255,8 +1313,9 @@ hipEvent_t Stepper<real>::record_event() {
 template <typename real>
 template <typename F>
 void Stepper<real>::timed(int cls, F&& launch) {
-  if (!timing_) {
+  if (!timing_ || (skip_step_groups_ && (cls == PF_K_YEE_FUSED || cls == PF_K_AUX) && open_cls_ < 0)) {
     launch();
+    if (timing_) chain_ev_ = nullptr;   // untimed launches follow any nested group's end
     return;
   }
   const int outer = open_cls_;
@@ -1290,9 +1349,11 @@ void Stepper<real>::collect_timing() {
 }
 
 template <typename real>
-void Stepper<real>::timing_enable(bool on) {
+void Stepper<real>::timing_enable(int every) {
   collect_timing();
-  timing_ = on;
+  timing_ = every > 0;
+  timing_every_ = std::max(every, 1);
+  skip_step_groups_ = false;
   for (int c = 0; c < PF_NKCLASS; ++c) {
     t_ms_[c] = 0.0;
     t_count_[c] = 0;
@@ -1314,25 +1375,37 @@ void Stepper<real>::step_once(int64_t step, bool monitors) {
     const int in = cur_;
     timed(PF_K_YEE_FUSED, [&] {
       if (fa_[in].variant == 2) {
-        // The all-axis, x and y CPML lists on the side stream when the
-        // main stream has tiles too, so the launches overlap (they write
-        // disjoint tiles of level out and only read level in); the slowest
-        // list goes first.
+        // The CPML lists in side_mask_ on the side stream when the main
+        // stream has tiles too, so the launches overlap (they write disjoint
+        // tiles of level out and only read level in); the slowest list goes
+        // first. With the mixed launch, the main stream's other lists follow it.
         static constexpr int order[NLIST] = {1, 4, 3, 2, 0};
-        static constexpr bool on_side[NLIST] = {false, true, false, true, true};
+        auto on_side = [&](int q) { return (side_mask_ >> q & 1) != 0; };
         bool any_side = false, any_main = false;
-        for (int q = 0; q < NLIST; ++q) (on_side[q] ? any_side : any_main) |= ntiles_[q] > 0;
+        for (int q = 0; q < NLIST; ++q)
+          if (!(q != 0 && (mix_mask_ >> q & 1))) (on_side(q) ? any_side : any_main) |= ntiles_[q] > 0;
         const bool fork = side_ && any_side && any_main;
         if (fork) {
           HIP_CHECK(hipEventRecord(fork_ev_[0], stream_));
           HIP_CHECK(hipStreamWaitEvent(side_, fork_ev_[0], 0));
         }
-        for (int q : order) {
+        int seq[NLIST], nseq = 0;
+        if (fork && mix_mask_) {
+          for (int q : order)
+            if (on_side(q)) seq[nseq++] = q;
+          seq[nseq++] = 0;
+          for (int q : order)
+            if (q != 0 && !on_side(q)) seq[nseq++] = q;
+        } else {
+          for (int q : order) seq[nseq++] = q;
+        }
+        for (int n = 0; n < nseq; ++n) {
+          const int q = seq[n];
           if (ntiles_[q] == 0 || (mix_mask_ >> q & 1)) continue;
           pfhip::MarchArgs<real> ma = ma_[in];
           ma.tiles = tiles_[q];
           ma.ntiles = ntiles_[q];
-          hipStream_t s = fork && on_side[q] ? side_ : stream_;
+          hipStream_t s = fork && on_side(q) ? side_ : stream_;
           if (q == 0 && mix_mask_) {
             ma.tiles = tiles_mix_;
             ma.ntiles = ntiles_mix_;
@@ -1381,14 +1454,22 @@ template <typename real>
 void Stepper<real>::after_march(int64_t step) {
   const int in = cur_, out = cur_ ^ 1;
   pfhip::AdeJob<real>*& tab = ade_tab1_[in][p_lvl_];
-  if (ade_.size() > 1 && !tab) {
+  if ((ade_.size() > 1 || (post_ok_ && !ade_.empty())) && !tab) {
     // ade_jobs reads the polarisation pointers of the current P level
     std::vector<pfhip::AdeJob<real>> jobs;
     ade_jobs(in, jobs);
     ade_n_jobs1_ = (int)jobs.size();
-    for (const pfhip::AdeJob<real>& J : jobs) ade_max_n1_ = std::max(ade_max_n1_, J.n);
-    tab = arena_.upload(jobs.data(), jobs.size(), stream_);
+    ade_job_n_.clear();
+    for (const pfhip::AdeJob<real>& J : jobs) {
+      ade_max_n1_ = std::max(ade_max_n1_, J.n);
+      ade_job_n_.push_back(J.n);
+    }
+    tab = jobs.empty() ? nullptr : arena_.upload(jobs.data(), jobs.size(), stream_);
     chain_ev_ = nullptr;   // the upload is not part of the next group
+  }
+  if (post_ok_) {
+    timed(PF_K_AUX, [&] { post_step(step, tab); });
+    return;
   }
   timed(PF_K_AUX, [&] {
     if (tab && ade_n_jobs1_ > 0) {
@@ -1406,6 +1487,47 @@ void Stepper<real>::after_march(int64_t step) {
                                zpad_, f.dA, f.dt, f.partial, f.n_part, f.total, stream_);
     capture_dft(step);   // one timed group: two fewer events per step
   });
+}
+
+// after_march in one launch (k_post_step): ADE, E-side sources and the DFT
+// samples of the step, then the flux planes and any full DFT ring's fold.
+template <typename real>
+void Stepper<real>::post_step(int64_t step, const pfhip::AdeJob<real>* tab) {
+  const int out = cur_ ^ 1;
+  cur_ = out;
+  pfhip::PostArgs<real> P;
+  std::memset(&P, 0, sizeof(P));
+  auto blocks = [](int64_t n) { return (int)((n + 255) / 256); };
+  P.ade = tab;
+  P.n_ade = tab ? ade_n_jobs1_ : 0;
+  for (int j = 0; j < P.n_ade; ++j) P.ade_blk[j + 1] = P.ade_blk[j] + blocks(ade_job_n_[j]);
+  P.src_fields = fields_dev_[out];
+  P.src = src_e_;
+  P.n_steps = n_steps_;
+  P.step = step;
+  P.src_blk = blocks(src_e_.n_groups);
+  P.dft_fields = fields_dev_[out];
+  P.plane = (int64_t)args_.ny * zrow_;
+  P.row = zrow_;
+  P.zpad = zpad_;
+  Dft* taken[pfhip::POST_MAX_DFT];
+  for (Dft& m : dft_) {
+    if (!dft_take(m, step)) continue;
+    pfhip::PostGather<real>& G = P.dft[P.n_dft];
+    G.comps = m.comps_dev;
+    G.box = m.box;
+    G.nbox = m.nbox;
+    G.blocks_c = blocks(m.nbox);
+    G.out = m.ring + (int64_t)m.count * m.n_comp * m.nbox;
+    P.dft_blk[P.n_dft + 1] = P.dft_blk[P.n_dft] + m.n_comp * G.blocks_c;
+    taken[P.n_dft++] = &m;
+  }
+  pfhip::launch_post_step<real>(P, stream_);
+  for (Flux& f : flux_)
+    pfhip::launch_flux<real>(f_[cur_], f.axis, f.index, args_.nx, args_.ny, args_.nz, zrow_,
+                             zpad_, f.dA, f.dt, f.partial, f.n_part, f.total, stream_);
+  for (int d = 0; d < P.n_dft; ++d)
+    if (++taken[d]->count == taken[d]->K) fold(*taken[d]);
 }
 
 // A simulation can join a batched 3-D march when it runs the march kernel
@@ -1894,13 +2016,19 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
 }
 
 template <typename real>
+bool Stepper<real>::dft_take(Dft& m, int64_t step) {
+  if (step % m.interval != 0) return false;
+  const int64_t sample = step / m.interval;
+  if (sample >= m.n_samples) throw ArgError{"DFT sample beyond the phase table"};
+  if (m.count == 0) m.sample0 = sample;
+  else if (sample != m.sample0 + m.count) throw ArgError{"DFT samples must be consecutive"};
+  return true;
+}
+
+template <typename real>
 void Stepper<real>::capture_dft(int64_t step) {
   for (Dft& m : dft_) {
-    if (step % m.interval != 0) continue;
-    const int64_t sample = step / m.interval;
-    if (sample >= m.n_samples) throw ArgError{"DFT sample beyond the phase table"};
-    if (m.count == 0) m.sample0 = sample;
-    else if (sample != m.sample0 + m.count) throw ArgError{"DFT samples must be consecutive"};
+    if (!dft_take(m, step)) continue;
     pfhip::launch_gather_box<real>(fields_dev_[cur_], m.comps_dev, m.n_comp, m.box,
                                    (int64_t)args_.ny * zrow_, zrow_, zpad_,
                                    m.ring + (int64_t)m.count * m.n_comp * m.nbox, stream_);
@@ -1937,6 +2065,7 @@ void Stepper<real>::info(pf_info* out) const {
   out->pair_tiles_bulk = ntiles2_[0];
   out->pair_tiles_cpml = ntiles2_[1];
   for (const Dft& m : dft_) out->dft_ring = std::max<int64_t>(out->dft_ring, m.K);
+  out->post_fused = fused_ && post_ok_;
 }
 
 // Folds every buffered DFT sample into the complex128 accumulators and
@@ -1963,7 +2092,9 @@ void Stepper<real>::advance(int64_t step0, int64_t n) {
       step_pair(s);
       s += 2;
     } else {
+      skip_step_groups_ = timing_ && s % timing_every_ != 0;
       step_once(s);
+      skip_step_groups_ = false;
       s += 1;
     }
     if (timing_ && pending_.size() > 4096) collect_timing();
@@ -2777,7 +2908,8 @@ int pf_flux_read(pf_handle* h, double* dst) {
 
 int pf_timing_enable(pf_handle* h, int32_t on) {
   if (!h) return fail(PF_E_ARG, "null handle");
-  return guarded([&] { h->timing_enable(on != 0); });
+  if (on < 0) return fail(PF_E_ARG, "negative timing interval");
+  return guarded([&] { h->timing_enable(on); });
 }
 
 int pf_timing_read(pf_handle* h, double* ms, int64_t* launches) {

@@ -108,7 +108,7 @@ struct pf_handle {
   virtual void write_field(int comp, const void* src) = 0;
   virtual void dft_read(int mon, int slot, double* dst) = 0;
   virtual void flux_read(double* dst) = 0;
-  virtual void timing_enable(bool on) = 0;
+  virtual void timing_enable(int every) = 0;   // 0 off, k: step groups of every k-th step
   virtual void checkpoint_enable(int interval) = 0;
   virtual void adjoint(int n_cot, const pf_cotangent* cot, double* grad) = 0;
   virtual void timing_read(double* ms, int64_t* launches) = 0;

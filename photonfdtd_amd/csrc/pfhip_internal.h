@@ -324,6 +324,37 @@ struct DftJob {
 };
 template <typename real>
 void launch_ade_jobs(const AdeJob<real>* jobs, int n_jobs, int64_t max_n, hipStream_t s);
+// One step's ADE media, E-side sources and DFT samples in one launch
+// (k_post_step), for a single simulation whose parts touch disjoint E nodes.
+constexpr int POST_MAX_ADE = 8;
+constexpr int POST_MAX_DFT = 8;
+template <typename real>
+struct PostGather {
+  const int32_t* comps;   // device array of n_comp component codes
+  BoxDev box;
+  int64_t nbox;
+  int blocks_c;           // blocks per component: ceil(nbox / 256)
+  real* out;              // ring slot [n_comp][nbox]
+};
+template <typename real>
+struct PostArgs {
+  const AdeJob<real>* ade;   // device table of n_ade jobs
+  int n_ade;
+  int ade_blk[POST_MAX_ADE + 1];
+  real* const* src_fields;   // device table of the output level
+  SourceGroupsDev src;
+  int64_t n_steps, step;
+  int src_blk;
+  real* const* dft_fields;
+  int64_t plane;
+  int row, zpad;
+  int n_dft;
+  int dft_blk[POST_MAX_DFT + 1];
+  PostGather<real> dft[POST_MAX_DFT];
+};
+template <typename real>
+void launch_post_step(const PostArgs<real>& P, hipStream_t s);
+
 // The step is `step`, or *step_dev when step_dev is set (graph replays).
 template <typename real>
 void launch_dft_jobs(const DftJob<real>* jobs, int n_jobs, int64_t max_elems, int64_t step,

@@ -69,7 +69,7 @@ class Streamer final : public pf_handle {
   void write_field(int comp, const void* src) override;
   void dft_read(int mon, int slot, double* dst) override;
   void flux_read(double*) override {}
-  void timing_enable(bool) override {}
+  void timing_enable(int) override {}
   void checkpoint_enable(int) override {
     throw ArgError{"checkpoints need the device-resident stepper (PF_FLAG_STREAM is set)"};
   }

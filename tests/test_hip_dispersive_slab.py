@@ -37,15 +37,16 @@ def oracle_run():
     return run_oracle(dispersive_slab(pf, "hip"), return_state=True)
 
 
-def _gpu(precision, exact, env, monkeypatch):
+def _gpu(precision, exact, env, monkeypatch, **scene):
     import photonfdtd_amd as pf
     from photonfdtd_amd.hip_backend import HipEngine
     for k in ("PHOTONFDTD_HIP_ZPAD", "PHOTONFDTD_HIP_SPLIT", "PHOTONFDTD_HIP_PAIR",
-              "PHOTONFDTD_HIP_MIX", "PHOTONFDTD_HIP_EVEN_X", "PHOTONFDTD_HIP_XCHUNK"):
+              "PHOTONFDTD_HIP_MIX", "PHOTONFDTD_HIP_EVEN_X", "PHOTONFDTD_HIP_XCHUNK",
+              "PHOTONFDTD_HIP_POST", "PHOTONFDTD_HIP_SIDE"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
-    sim = dispersive_slab(pf, "hip", precision=precision)
+    sim = dispersive_slab(pf, "hip", precision=precision, **scene)
     eng = HipEngine(sim, np.float64 if precision == "float64" else np.float32, exact)
     info = eng.info()
     eng.advance(0, sim.n_steps)
@@ -124,6 +125,35 @@ def test_mixed_launch_same_bits_as_per_list_launches(precision, exact, mix, monk
     for nm in ("in", "out"):
         for c in da[nm]:
             assert np.array_equal(da[nm][c], db[nm][c]), (nm, c)
+
+
+@pytest.mark.parametrize("precision,exact", [("float64", True), ("float32", False)])
+def test_post_step_launch_and_main_stream_x_list_same_bits(precision, exact, monkeypatch):
+    """The ADE media, the E-side dipole and both DFT planes' samples in one
+    launch after the march, with the x-only CPML list on the main stream
+    after the mixed launch (the float32 defaults), give the bits of one
+    launch per part (PHOTONFDTD_HIP_POST=0) with the x-only list on the side
+    stream (PHOTONFDTD_HIP_SIDE=26).
+    At 160 cells in x the DFT planes miss the gold cube and the silicon box,
+    so the one-launch path is taken; on the 48-cell grid the "out" plane
+    crosses the cube and the stepper keeps one launch per part."""
+    print(f"[case] dispersive slab 160x72x140 post-step launch {precision}", flush=True)
+    scene = dict(shape=(160, 72, 140), steps=200)
+    ia, fa, da = _gpu(precision, exact, dict(M64, PHOTONFDTD_HIP_POST="0", PHOTONFDTD_HIP_SIDE="26"),
+                      monkeypatch, **scene)
+    ib, fb, db = _gpu(precision, exact, dict(M64, PHOTONFDTD_HIP_SIDE="2"), monkeypatch, **scene)
+    assert ia["post_fused"] == 0 and ib["post_fused"] == 1, (ia, ib)
+    assert ib["cpml_lists"] == 0b1111 and ib["tiles_bulk"] > 0, ib
+    assert float(np.abs(fb["Ey"]).max()) > 0
+    for c in FIELDS:
+        assert np.array_equal(fa[c], fb[c]), c
+    for nm in ("in", "out"):
+        assert float(np.abs(db[nm]["Ey"]).max()) > 0, nm
+        for c in da[nm]:
+            assert np.array_equal(da[nm][c], db[nm][c]), (nm, c)
+    print(f"[case] dispersive slab 48x72x140 post-step launch refused {precision}", flush=True)
+    ic, _, _ = _gpu(precision, exact, M64, monkeypatch)
+    assert ic["post_fused"] == 0, ic
 
 
 def test_full_size_64_lane_march_same_bits_as_padded_march(monkeypatch):
