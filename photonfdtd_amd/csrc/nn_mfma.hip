@@ -719,6 +719,97 @@ __global__ __launch_bounds__(256) void k_conv_c1_wgrad(Geom g, const uint16_t* x
   }
 }
 
+// Weight gradient, one input channel, cout a multiple of 4 (at most 64):
+// the workgroup sums output pixels [z PIX, (z + 1) PIX) into
+// part[z][cout][taps] like k_conv_c1_wgrad, with LP = 2^lp_log2 lanes per
+// pixel: lane q of a pixel's group holds channels 4q..4q+3 (one 16-byte dY
+// load, one 8- or 16-byte Y load per pixel), so a wave issues one dY load per
+// 64 / LP pixels instead of one per two. Thread t walks the pixels
+// r0 + t / LP, + 256 / LP, ... in order (U at a time, loads first); the
+// pixel slots are added in slot order through LDS, so the partials are the
+// same bits on every run. Indices fit 32 bits (host check).
+constexpr int C1W_U = 2;
+__global__ __launch_bounds__(256) void k_conv_c1_wgrad4(Geom g, int lp_log2, const uint16_t* x,
+                                                        const float* dy, const void* y, int relu,
+                                                        int out_f32, float* part) {
+  __shared__ float sums[256 * 4 * 9];
+  const int t = threadIdx.x;
+  const int LP = 1 << lp_log2, PS = 256 >> lp_log2;
+  const int c0 = 4 * (t & (LP - 1)), ps = t >> lp_log2;
+  const bool cok = c0 < g.cout;
+  const int taps = g.k * g.k;
+  const int R = g.n * g.ho * g.wo;
+  const int r0 = (int)blockIdx.x * C1_PIX, r1 = min(R, r0 + C1_PIX);
+  float acc[4][9];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int tp = 0; tp < 9; ++tp) acc[j][tp] = 0.f;
+  PixIt p;
+  p.set(min(r0 + ps, R - 1), g.ho, g.wo);
+  for (int rb = r0 + ps; rb < r1; rb += C1W_U * PS) {
+    int xi[C1W_U][9], di[C1W_U];
+    bool ok[C1W_U];
+#pragma unroll
+    for (int u = 0; u < C1W_U; ++u) {
+      const int r = rb + u * PS;
+      ok[u] = r < r1 && cok;
+      di[u] = ok[u] ? r * g.cout + c0 : 0;
+#pragma unroll
+      for (int tp = 0; tp < 9; ++tp) {
+        int ky, kx;
+        tap_yx(g, tp, ky, kx);
+        const int iy = p.oy * g.s - g.pad + ky, ix = p.ox * g.s - g.pad + kx;
+        xi[u][tp] = ok[u] && tp < taps && iy >= 0 && iy < g.h && ix >= 0 && ix < g.w
+                        ? (p.b * g.h + iy) * g.w + ix
+                        : -1;
+      }
+      p.step(PS, g.ho, g.wo);
+    }
+    // loads, unconditional (index 0 where there is nothing to read)
+    uint16_t xr[C1W_U][9];
+    float4 dv[C1W_U], yv[C1W_U];
+#pragma unroll
+    for (int u = 0; u < C1W_U; ++u) {
+#pragma unroll
+      for (int tp = 0; tp < 9; ++tp) xr[u][tp] = x[xi[u][tp] >= 0 ? xi[u][tp] : 0];
+      dv[u] = *reinterpret_cast<const float4*>(dy + di[u]);
+      if (!relu) {
+        yv[u] = make_float4(1.f, 1.f, 1.f, 1.f);
+      } else if (out_f32) {
+        yv[u] = *reinterpret_cast<const float4*>(static_cast<const float*>(y) + di[u]);
+      } else {
+        const uint2 h = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(y) + di[u]);
+        yv[u] = make_float4(bf2f((uint16_t)(h.x & 0xFFFF)), bf2f((uint16_t)(h.x >> 16)),
+                            bf2f((uint16_t)(h.y & 0xFFFF)), bf2f((uint16_t)(h.y >> 16)));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < C1W_U; ++u) {
+      const float dj[4] = {dv[u].x, dv[u].y, dv[u].z, dv[u].w};
+      const float yj[4] = {yv[u].x, yv[u].y, yv[u].z, yv[u].w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float d = ok[u] ? bf2f(f2bf(yj[j] > 0.f ? dj[j] : 0.f)) : 0.f;
+#pragma unroll
+        for (int tp = 0; tp < 9; ++tp) acc[j][tp] += (xi[u][tp] >= 0 ? bf2f(xr[u][tp]) : 0.f) * d;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int tp = 0; tp < 9; ++tp) sums[(t * 4 + j) * 9 + tp] = acc[j][tp];
+  __syncthreads();
+  const int width = 4 * LP;   // channels per pixel slot
+  for (int i = t; i < g.cout * taps; i += 256) {
+    const int co = i / taps, tp = i % taps;
+    float s = sums[co * 9 + tp];
+    for (int k = 1; k < PS; ++k) s += sums[(k * width + co) * 9 + tp];
+    part[((size_t)blockIdx.x * g.cout + co) * taps + tp] = s;
+  }
+}
+
 // dw[i] += sum over the splits of part[z][i], z = 0, 1, ... in order: the
 // weight gradient is the same bits on every run (no atomics).
 __global__ void k_reduce_splits(const float* part, int splits, int64_t count, float* dw) {
@@ -962,6 +1053,11 @@ constexpr int64_t SCRATCH_FLOATS = int64_t(4) << 20;
 // function of the shape only, so both backward entry points sum the same
 // partials in the same order.
 inline bool c1_wgrad(const Geom& g) { return g.cin == 1 && g.cout <= 64 && g.k <= 3; }
+// the four-channel form: 16-byte dY rows and 32-bit indices
+inline bool c1_wgrad4(const Geom& g) {
+  return c1_wgrad(g) && g.cout % 4 == 0 && (int64_t)g.n * g.ho * g.wo * g.cout < INT32_MAX &&
+         (int64_t)g.n * g.h * g.w < INT32_MAX;
+}
 inline int weight_grad_splits(const Geom& g) {
   const int64_t R = (int64_t)g.n * g.ho * g.wo;
   if (c1_wgrad(g)) return (int)((R + C1_PIX - 1) / C1_PIX);
@@ -1067,8 +1163,17 @@ void conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void
   }
   if (dw && pfnn::c1_wgrad(g)) {
     const int splits = pfnn::weight_grad_splits(g);
-    pfnn::k_conv_c1_wgrad<<<splits, 256, 0, s>>>(g, static_cast<const uint16_t*>(x), dy, y,
-                                                   c->relu, c->out_f32, part);
+    const bool aligned = reinterpret_cast<uintptr_t>(dy) % 16 == 0 &&
+                         reinterpret_cast<uintptr_t>(y) % (c->out_f32 ? 16 : 8) == 0;
+    if (pfnn::c1_wgrad4(g) && aligned) {
+      int lp_log2 = 0;
+      while ((1 << lp_log2) * 4 < g.cout) ++lp_log2;
+      pfnn::k_conv_c1_wgrad4<<<splits, 256, 0, s>>>(g, lp_log2, static_cast<const uint16_t*>(x), dy,
+                                                     y, c->relu, c->out_f32, part);
+    } else {
+      pfnn::k_conv_c1_wgrad<<<splits, 256, 0, s>>>(g, static_cast<const uint16_t*>(x), dy, y,
+                                                     c->relu, c->out_f32, part);
+    }
     pfnn::reduce_splits(part, splits, (int64_t)g.cout * g.kdim, dw, s);
   } else if (dw) {
     const int splits = pfnn::weight_grad_splits(g);
