@@ -332,6 +332,7 @@ class Stepper final : public pf_handle {
   };
 
   void step_once(int64_t step, bool monitors = true);
+  void launch_march_lists(const pfhip::MarchArgs<real>& base);
   void save_state(real* dst);
   void capture_dft(int64_t step);
   bool dft_take(Dft& m, int64_t step);
@@ -1369,57 +1370,65 @@ void Stepper<real>::timing_read(double* ms, int64_t* launches) {
   }
 }
 
+// One step of the march over every tile list, reading and writing the
+// levels `base` names (the stepper's ping-pong levels, or trajectory slots of
+// the adjoint's recompute).
+template <typename real>
+void Stepper<real>::launch_march_lists(const pfhip::MarchArgs<real>& base) {
+  // The CPML lists in side_mask_ on the side stream when the main
+  // stream has tiles too, so the launches overlap (they write disjoint
+  // tiles of level out and only read level in); the slowest list goes
+  // first. With the mixed launch, the main stream's other lists follow it.
+  static constexpr int order[NLIST] = {1, 4, 3, 2, 0};
+  auto on_side = [&](int q) { return (side_mask_ >> q & 1) != 0; };
+  bool any_side = false, any_main = false;
+  for (int q = 0; q < NLIST; ++q)
+    if (!(q != 0 && (mix_mask_ >> q & 1))) (on_side(q) ? any_side : any_main) |= ntiles_[q] > 0;
+  const bool fork = side_ && any_side && any_main;
+  if (fork) {
+    HIP_CHECK(hipEventRecord(fork_ev_[0], stream_));
+    HIP_CHECK(hipStreamWaitEvent(side_, fork_ev_[0], 0));
+  }
+  int seq[NLIST], nseq = 0;
+  if (fork && mix_mask_) {
+    for (int q : order)
+      if (on_side(q)) seq[nseq++] = q;
+    seq[nseq++] = 0;
+    for (int q : order)
+      if (q != 0 && !on_side(q)) seq[nseq++] = q;
+  } else {
+    for (int q : order) seq[nseq++] = q;
+  }
+  for (int n = 0; n < nseq; ++n) {
+    const int q = seq[n];
+    if (ntiles_[q] == 0 || (mix_mask_ >> q & 1)) continue;
+    pfhip::MarchArgs<real> ma = base;
+    ma.tiles = tiles_[q];
+    ma.ntiles = ntiles_[q];
+    hipStream_t s = fork && on_side(q) ? side_ : stream_;
+    if (q == 0 && mix_mask_) {
+      ma.tiles = tiles_mix_;
+      ma.ntiles = ntiles_mix_;
+      pfhip::launch_march64_mix<real>(ma, exact_, 12, (mix_mask_ >> 3 & 1) << 1 | (mix_mask_ >> 4 & 1), s);
+    } else if (zpad_ == pfhip::ZPAD_BLOCK)
+      pfhip::launch_march64<real>(ma, exact_, list_mask_[q], q == 0 ? march_nw_ : pml_nw_, s);
+    else
+      pfhip::launch_march<real>(ma, exact_, q == 1, q == 1 ? pml_nw_ : march_nw_,
+                                march_dist_, s);
+  }
+  if (fork) {
+    HIP_CHECK(hipEventRecord(fork_ev_[1], side_));
+    HIP_CHECK(hipStreamWaitEvent(stream_, fork_ev_[1], 0));
+  }
+}
+
 template <typename real>
 void Stepper<real>::step_once(int64_t step, bool monitors) {
   if (fused_) {
     const int in = cur_;
     timed(PF_K_YEE_FUSED, [&] {
       if (fa_[in].variant == 2) {
-        // The CPML lists in side_mask_ on the side stream when the main
-        // stream has tiles too, so the launches overlap (they write disjoint
-        // tiles of level out and only read level in); the slowest list goes
-        // first. With the mixed launch, the main stream's other lists follow it.
-        static constexpr int order[NLIST] = {1, 4, 3, 2, 0};
-        auto on_side = [&](int q) { return (side_mask_ >> q & 1) != 0; };
-        bool any_side = false, any_main = false;
-        for (int q = 0; q < NLIST; ++q)
-          if (!(q != 0 && (mix_mask_ >> q & 1))) (on_side(q) ? any_side : any_main) |= ntiles_[q] > 0;
-        const bool fork = side_ && any_side && any_main;
-        if (fork) {
-          HIP_CHECK(hipEventRecord(fork_ev_[0], stream_));
-          HIP_CHECK(hipStreamWaitEvent(side_, fork_ev_[0], 0));
-        }
-        int seq[NLIST], nseq = 0;
-        if (fork && mix_mask_) {
-          for (int q : order)
-            if (on_side(q)) seq[nseq++] = q;
-          seq[nseq++] = 0;
-          for (int q : order)
-            if (q != 0 && !on_side(q)) seq[nseq++] = q;
-        } else {
-          for (int q : order) seq[nseq++] = q;
-        }
-        for (int n = 0; n < nseq; ++n) {
-          const int q = seq[n];
-          if (ntiles_[q] == 0 || (mix_mask_ >> q & 1)) continue;
-          pfhip::MarchArgs<real> ma = ma_[in];
-          ma.tiles = tiles_[q];
-          ma.ntiles = ntiles_[q];
-          hipStream_t s = fork && on_side(q) ? side_ : stream_;
-          if (q == 0 && mix_mask_) {
-            ma.tiles = tiles_mix_;
-            ma.ntiles = ntiles_mix_;
-            pfhip::launch_march64_mix<real>(ma, exact_, 12, (mix_mask_ >> 3 & 1) << 1 | (mix_mask_ >> 4 & 1), s);
-          } else if (zpad_ == pfhip::ZPAD_BLOCK)
-            pfhip::launch_march64<real>(ma, exact_, list_mask_[q], q == 0 ? march_nw_ : pml_nw_, s);
-          else
-            pfhip::launch_march<real>(ma, exact_, q == 1, q == 1 ? pml_nw_ : march_nw_,
-                                      march_dist_, s);
-        }
-        if (fork) {
-          HIP_CHECK(hipEventRecord(fork_ev_[1], side_));
-          HIP_CHECK(hipStreamWaitEvent(stream_, fork_ev_[1], 0));
-        }
+        launch_march_lists(ma_[in]);
       } else {
         pfhip::launch_fused<real>(fa_[in], exact_, stream_);
       }
@@ -1901,7 +1910,9 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
   real* lam = scratch.zeros<real>(elems, stream_);
   real* lam_d = scratch.zeros<real>(6 * (size_t)ncell, stream_);
   double* grad = scratch.zeros<double>(ncell, stream_);
-  real* traj = static_cast<real*>(scratch.alloc((size_t)C * elems * sizeof(real)));
+  // C + 1 trajectory slots: the states before every step of a segment and
+  // after its last one
+  real* traj = static_cast<real*>(scratch.alloc((size_t)(C + 1) * elems * sizeof(real)));
   real* lam_tab[6];
   for (int c = 0; c < 6; ++c) lam_tab[c] = lam + (size_t)c * ncell;
   real** lam_dev = scratch.upload(lam_tab, 6, stream_);
@@ -1939,8 +1950,8 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
   a.ch = args_.ch;
 
   // field tables of the trajectory slots (the source kernels take one)
-  std::vector<real*> slot_ptrs((size_t)C * 6);
-  for (int k = 0; k < C; ++k)
+  std::vector<real*> slot_ptrs((size_t)(C + 1) * 6);
+  for (int k = 0; k <= C; ++k)
     for (int c = 0; c < 6; ++c) slot_ptrs[(size_t)k * 6 + c] = traj + (size_t)k * elems + (size_t)c * ncell;
   real** slot_tab = scratch.upload(slot_ptrs.data(), slot_ptrs.size(), stream_);
   // psi slot offsets inside a stored state (E side, then H side)
@@ -1949,14 +1960,22 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
     psi_off[sl] = sl < 6 ? (size_t)(args_.psi[sl] - args_.psi[0])
                          : psi_len_e_ + (size_t)(args_.psi[sl] - args_.psi[6]);
 
+  // The recompute runs the forward's own march when the stepper marches on
+  // dense rows of exactly nz slots (the trajectory's layout) with no H-side
+  // sources and no dispersive media: the slots are its levels, the E-side psi
+  // (updated in place by the march) copied into the output slot first.
+  const bool march_rc = fused_ && fa_[0].variant == 2 && zpad_ == pfhip::ZPAD_BLOCK &&
+                        zrow_ == shape[2] && !pair_ && src_h_.n_groups == 0 && ade_.empty() &&
+                        !(std::getenv("PHOTONFDTD_HIP_ADJ_MARCH") &&
+                          std::getenv("PHOTONFDTD_HIP_ADJ_MARCH")[0] == '0');
   const int64_t nseg = (n_steps_ + C - 1) / C;
   for (int64_t sg = nseg - 1; sg >= 0; --sg) {
     const int64_t s0 = sg * C, s1 = std::min<int64_t>(s0 + C, n_steps_);
-    // the segment's pre-step states: slot 0 is the checkpoint; each step then
-    // runs out of place from slot k into slot k + 1 (the state after the
-    // segment's last step is not needed)
+    // the segment's states: slot 0 is the checkpoint; each step then runs
+    // out of place from slot k into slot k + 1, the last one included (its
+    // H^{n+1/2} is what the reverse step of that step reads)
     HIP_CHECK(hipMemcpyAsync(traj, ckpts_[sg], elems * sizeof(real), hipMemcpyDeviceToDevice, stream_));
-    for (int64_t st = s0; st + 1 < s1; ++st) {
+    for (int64_t st = s0; st < s1; ++st) {
       const int k = (int)(st - s0);
       real* Tin = traj + (size_t)k * elems;
       real* Tout = Tin + elems;
@@ -1970,6 +1989,20 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
         sa.psio[sl] = args_.psi[sl] ? Tout + nf + psi_off[sl] : nullptr;
       }
       real* const* out_tab = slot_tab + (size_t)(k + 1) * 6;
+      if (march_rc) {
+        pfhip::MarchArgs<real> m = ma_[0];
+        m.Lin = Tin;
+        m.Lout = Tout;
+        m.psiE = psi_len_e_ ? Tout + nf : nullptr;
+        m.psiHin = psi_len_h_ ? Tin + nf + psi_len_e_ : nullptr;
+        m.psiHout = psi_len_h_ ? Tout + nf + psi_len_e_ : nullptr;
+        if (psi_len_e_)
+          HIP_CHECK(hipMemcpyAsync(Tout + nf, Tin + nf, psi_len_e_ * sizeof(real),
+                                   hipMemcpyDeviceToDevice, stream_));
+        launch_march_lists(m);
+        pfhip::launch_inject<real>(out_tab, src_e_, n_steps_, st, stream_);
+        continue;
+      }
       // the generic step's order: H, H sources, E, E sources (no dispersive media here)
       pfhip::launch_generic_h<real>(sa, exact_, stream_);
       pfhip::launch_inject<real>(out_tab, src_h_, n_steps_, st, stream_);
@@ -1986,22 +2019,11 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
                                            args_.nyz, args_.nz, cot_dev[m], d.n_freq, d.phase_e,
                                            d.phase_h, st / d.interval, d.dt, stream_);
       }
-      // rebuild H^{n+1/2} from the stored pre-step state, out of place: E^n,
-      // H^{n-1/2} and the H-side psi are read where the trajectory holds them,
-      // the new H (every node) goes to the live arrays and the new H-side psi
-      // to the live psi, which nothing reads (the transposed kernels take
-      // both psi sides from the trajectory)
-      pfhip::StepArgs<real> hs = args_;
-      for (int c = 0; c < 6; ++c) hs.f[c] = const_cast<real*>(T) + (size_t)c * ncell;
-      for (int c = 3; c < 6; ++c) hs.fo[c] = f_[0][c];
-      for (int sl = 6; sl < pfhip::PSI_COUNT; ++sl)
-        if (args_.psi[sl]) {
-          hs.psi[sl] = const_cast<real*>(T) + nf + psi_off[sl];
-          hs.psio[sl] = args_.psi[sl];
-        }
-      pfhip::launch_generic_h<real>(hs, exact_, stream_);
-      pfhip::launch_inject<real>(fields_dev_[0], src_h_, n_steps_, st, stream_);
+      // E^n from the pre-step slot, H^{n+1/2} (after the H sources) from the
+      // next slot, where the recompute wrote it; both psi sides from the
+      // pre-step slot
       for (int c = 0; c < 3; ++c) a.f[c] = T + (size_t)c * ncell;
+      for (int c = 3; c < 6; ++c) a.f[c] = T + elems + (size_t)c * ncell;
       for (int sl = 0; sl < 6; ++sl) {
         a.psi_fwd[sl] = T + nf + (size_t)(args_.psi[sl] - args_.psi[0]);
         a.psi_fwd[6 + sl] = T + nf + psi_len_e_ + (size_t)(args_.psi[6 + sl] - args_.psi[6]);

@@ -143,9 +143,10 @@ def test_dispersive_scene_is_rejected():
 
 
 def test_dense64_forward_runs_the_march_and_matches_generic():
-    """With nz a multiple of 64 the checkpointed forward run is the 64-lane
-    march (fused path, dense rows of nz slots); its float64 gradient has the
-    bits of the generic-kernel run."""
+    """With nz a multiple of 64 the checkpointed forward run and the
+    reverse sweep's segment recompute are the 64-lane march (fused path,
+    dense rows of nz slots, trajectory slots as its levels); the float64
+    gradient has the bits of the generic-kernel run."""
     import photonfdtd_amd as pf
     from photonfdtd_amd import _native as N
     from photonfdtd_amd.adjoint import _checkpointed_engine, value_and_grad_eps
