@@ -980,7 +980,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=12, help="surrogate training epochs")
     ap.add_argument("--adjoint-iters", type=int, default=12, help="plain adjoint iterations")
     ap.add_argument("--rounds", type=int, default=6, help="surrogate design rounds")
-    ap.add_argument("--candidates", type=int, default=48,
+    ap.add_argument("--candidates", type=int, default=16,
                     help="surrogate: designs verified per round (one batched FDTD run)")
     ap.add_argument("--fd-voxels", type=int, default=20, help="adjoint3d: central-difference voxels")
     ap.add_argument("--workload", default="slab3d",

@@ -288,7 +288,9 @@ def surrogate_design(pf, net, labeler: Labeler, ref_flux: np.ndarray, rho0: np.n
     the candidates with one batched FDTD run, keep the best verified design
     and fine-tune the network on the verified batch. If the network's
     relative error on the kept design exceeds `tol`, also take one plain
-    adjoint step from it."""
+    adjoint step from it. A round whose verified designs reach `target`
+    ends the loop right after the verification, and one whose adjoint step
+    reaches it ends before the fine-tuning (both only serve later rounds)."""
     best = np.array(rho0, dtype=np.float32)
     best_j = float(objective(labeler(best[None]))[0])
     if pool is not None:
@@ -315,8 +317,13 @@ def surrogate_design(pf, net, labeler: Labeler, ref_flux: np.ndarray, rho0: np.n
         fallback = err > tol
         if ver[k] > best_j:
             best, best_j = cand[k], float(ver[k])
-        train(net, cand, spec, epochs=finetune_epochs, batch=min(64, len(cand)), lr=5e-4,
-              seed=seed + r)
+        if target is not None and best_j >= target:
+            hist.append(dict(round=r, objective=best_j, predicted=float(pred[k]), error=float(err),
+                             fallback=False, wall_s=time.perf_counter() - t0))
+            if log:
+                log(f"[surrogate] round {r}: best J = {best_j:.4f} (candidate pred {pred[k]:.4f} vs "
+                    f"FDTD {ver[k]:.4f}, err {err:.1%}); target reached; {hist[-1]['wall_s']:.1f} s")
+            break
         if fallback:
             step_a = adjoint_design(pf, ref_flux, best, iters=1, steps=steps, precision=precision)
             rho_a = step_a["rho"].astype(np.float32)
@@ -325,6 +332,9 @@ def surrogate_design(pf, net, labeler: Labeler, ref_flux: np.ndarray, rho0: np.n
             PJ = np.concatenate([PJ, [ja]])
             if ja > best_j:
                 best, best_j = rho_a, ja
+        if target is None or best_j < target:
+            train(net, cand, spec, epochs=finetune_epochs, batch=min(64, len(cand)), lr=5e-4,
+                  seed=seed + r)
         hist.append(dict(round=r, objective=best_j, predicted=float(pred[k]), error=float(err),
                          fallback=bool(fallback), wall_s=time.perf_counter() - t0))
         if log:
