@@ -1909,6 +1909,7 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
   }
   real* lam = scratch.zeros<real>(elems, stream_);
   real* lam_d = scratch.zeros<real>(6 * (size_t)ncell, stream_);
+  real* lam_dh = scratch.zeros<real>(6 * (size_t)ncell, stream_);
   double* grad = scratch.zeros<double>(ncell, stream_);
   // C + 1 trajectory slots: the states before every step of a segment and
   // after its last one
@@ -1923,6 +1924,7 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
     a.f[c] = f_[0][c];
     a.lam_f[c] = lam_tab[c];
     a.lam_d[c] = lam_d + (size_t)c * ncell;
+    a.lam_dh[c] = lam_dh + (size_t)c * ncell;
   }
   a.ce = args_.ce;
   for (int sl = 0; sl < 12; ++sl) {
@@ -1969,6 +1971,7 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
                         !(std::getenv("PHOTONFDTD_HIP_ADJ_MARCH") &&
                           std::getenv("PHOTONFDTD_HIP_ADJ_MARCH")[0] == '0');
   const int64_t nseg = (n_steps_ + C - 1) / C;
+  bool first_reverse = true;   // no H-update terms to gather before the last step's transpose
   for (int64_t sg = nseg - 1; sg >= 0; --sg) {
     const int64_t s0 = sg * C, s1 = std::min<int64_t>(s0 + C, n_steps_);
     // the segment's states: slot 0 is the checkpoint; each step then runs
@@ -2028,7 +2031,12 @@ void Stepper<real>::adjoint(int n_cot, const pf_cotangent* cot, double* grad_out
         a.psi_fwd[sl] = T + nf + (size_t)(args_.psi[sl] - args_.psi[0]);
         a.psi_fwd[6 + sl] = T + nf + psi_len_e_ + (size_t)(args_.psi[6 + sl] - args_.psi[6]);
       }
-      for (int phase = 0; phase < 4; ++phase) pfhip::launch_adjoint_step<real>(a, stream_, phase);
+      // E transpose (after the H-update terms of step st + 1), then the H
+      // transpose (after the E-update terms of step st); the H-update terms
+      // of the first step are never gathered, since nothing reads lam_E after
+      pfhip::launch_adjoint_step<real>(a, stream_, 0, !first_reverse);
+      pfhip::launch_adjoint_step<real>(a, stream_, 1, true);
+      first_reverse = false;
     }
   }
   HIP_CHECK(hipGetLastError());

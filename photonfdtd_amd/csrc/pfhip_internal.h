@@ -210,7 +210,8 @@ struct AdjArgs {
   const real* psi_fwd[12];   // forward psi before the respective update
   real* lam_f[6];            // adjoint fields (updated in place)
   real* lam_psi[12];         // adjoint psi (updated in place)
-  real* lam_d[6];            // per-term scratch, one nx*ny*nz array each
+  real* lam_d[6];            // per-term scratch of the E update, one nx*ny*nz array each
+  real* lam_dh[6];           // per-term scratch of the H update
   double* grad;              // dJ/dce accumulator
   const real* b_e[3];
   const real* c_e[3];
@@ -535,7 +536,11 @@ template <typename real>
 void launch_ade_pp(real* const* Ein, real* const* Eout, const int64_t* cells, int64_t n, real* Pcur,
                    real* Pprev, const real* abc, int n_poles, real inv_eps, hipStream_t s);
 
-template <typename real> void launch_adjoint_step(const AdjArgs<real>& a, hipStream_t s, int phase);
+// phase 0: transpose of the E update, after (gather) adding the previous
+// reverse step's H-update terms to lam_E; phase 1: the E-update terms added
+// to lam_H, then the transpose of the H update.
+template <typename real>
+void launch_adjoint_step(const AdjArgs<real>& a, hipStream_t s, int phase, bool gather);
 template <typename real>
 void launch_adj_dft_source(real* const* lam_dev, const int32_t* comps, const int32_t* is_h,
                            int n_comp, const BoxDev& box, int64_t nyz, int nz, const double* cot,
