@@ -138,15 +138,24 @@ __device__ __forceinline__ u4 finish(const RawV& q) { return q.ok ? q.v : u4{0u,
 struct RawDz {
   u4 d0, d1, y0, y1;
   bool ok;
-  int relu, out_f32;
+  int relu, out_f32;   // relu < 0: d0 holds the finished BF16 values
 };
+// dZ[r][co..co+7] from FP32 dY and the layer's output Y, or from `dzb`, the
+// same values already rounded to BF16 by k_make_dz (when not null).
 __device__ __forceinline__ RawDz dz8_fetch(const Geom& g, const float* dy, const void* y, int relu,
-                                           int out_f32, int64_t r, int co, bool ok) {
+                                           int out_f32, int64_t r, int co, bool ok,
+                                           const uint16_t* dzb = nullptr) {
   RawDz q;
   q.ok = ok;
   q.relu = relu;
   q.out_f32 = out_f32;
   const size_t i = ok ? (size_t)r * g.cout + co : 0;
+  if (dzb) {
+    q.d0 = ld16(dzb + i);
+    q.d1 = q.y0 = q.y1 = u4{0u, 0u, 0u, 0u};
+    q.relu = -1;
+    return q;
+  }
   q.d0 = ld16(dy + i);
   q.d1 = ld16(dy + i + 4);
   q.y0 = q.y1 = u4{0u, 0u, 0u, 0u};
@@ -162,6 +171,7 @@ __device__ __forceinline__ RawDz dz8_fetch(const Geom& g, const float* dy, const
 }
 // As BF16 (round to nearest even).
 __device__ __forceinline__ u4 finish(const RawDz& q) {
+  if (q.relu < 0) return q.ok ? q.d0 : u4{0u, 0u, 0u, 0u};
   float v[8];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -345,6 +355,7 @@ struct DataGradOp {   // M = input pixels of the class, N = cin, K = (tap of the
   int relu, out_f32;
   float* dx;
   bool vec;   // cin and cout multiples of 8
+  const uint16_t* dzb;   // dZ as BF16 (k_make_dz), or null: from dy and y
   __device__ __forceinline__ int tap_of(int i) const { return (int)((c.taps >> (4 * i)) & 15u); }
   __device__ __forceinline__ int pixel(int m) const {
     const int ix2 = m % c.wc, t = m / c.wc;
@@ -383,7 +394,7 @@ struct DataGradOp {   // M = input pixels of the class, N = cin, K = (tap of the
     RawDz q;
     if (vec) {
       const int64_t r = it.ok && it.k < K ? out_of(it.b, it.iy, it.ix, tap_of(it.tc.tap)) : -1;
-      q = dz8_fetch(g, dy, y, relu, out_f32, r < 0 ? 0 : r, it.tc.c, r >= 0);
+      q = dz8_fetch(g, dy, y, relu, out_f32, r < 0 ? 0 : r, it.tc.c, r >= 0, dzb);
       it.tc.step(g.cout);
     } else {
       // element-wise: the finished values in d0/d1 as FP32, no mask
@@ -457,6 +468,7 @@ struct WeightGradOp {   // M = (tap, cin), N = cout, K = output pixels (split)
   int relu, out_f32;
   float* part;          // [split][cout][kdim]: the partial sum of each pixel split
   bool vec;             // cin and cout multiples of 8
+  const uint16_t* dzb;  // dZ as BF16 (k_make_dz), or null: from dy and y
   // A[m..m+7][r] = X~[r][m..m+7]
   struct AIt {
     int k, m, ky, kx, ci;
@@ -503,7 +515,7 @@ struct WeightGradOp {   // M = (tap, cin), N = cout, K = output pixels (split)
     RawDz q;
     if (vec) {
       const bool live = it.ok && it.k < K;
-      q = dz8_fetch(g, dy, y, relu, out_f32, live ? it.k : 0, it.n, live);
+      q = dz8_fetch(g, dy, y, relu, out_f32, live ? it.k : 0, it.n, live, dzb);
     } else {
       float v[8];
 #pragma unroll
@@ -947,6 +959,20 @@ __global__ __launch_bounds__(256) void k_gemm(Op op, int M, int N, int K, int kp
       }
 }
 
+// dZ = relu mask of dY, rounded to BF16 (the values every gradient pass
+// multiplies), written once per layer so both GEMM passes read 2 bytes per
+// value instead of 4 of dY plus Y. Thread t: values 8t..8t+7 (cout % 8 == 0).
+__global__ __launch_bounds__(256) void k_make_dz(Geom g, const float* dy, const void* y, int relu,
+                                                 int out_f32, uint16_t* dzb) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t R = (int64_t)g.n * g.ho * g.wo;
+  const int per = g.cout / 8;
+  if (v >= R * per) return;
+  const int64_t r = v / per;
+  const int co = (int)(v - r * per) * 8;
+  *reinterpret_cast<u4*>(dzb + r * g.cout + co) = finish(dz8_fetch(g, dy, y, relu, out_f32, r, co, true));
+}
+
 // dbias[co] += sum over output pixels of dZ: one workgroup per channel.
 __global__ __launch_bounds__(256) void k_bias_grad(Geom g, const float* dy, const void* y, int relu,
                                                    int out_f32, float* dbias) {
@@ -1140,12 +1166,33 @@ int pf_nn_conv_forward(const pf_nn_conv* c, const void* x, const void* w, const 
 
 namespace {
 
+// Whether the layer's gradient GEMMs read dZ as BF16 from a prepass
+// (vector operands on both channel counts, one of them a GEMM pass).
+bool gemm_dz(const pfnn::Geom& g, bool dx, bool dw) {
+  const bool vec = g.cin % 8 == 0 && g.cout % 8 == 0;
+  const bool dx_gemm = dx && !(g.cin == 1 && g.cout % 8 == 0 && g.cout <= 64);
+  const bool dw_gemm = dw && !pfnn::c1_wgrad(g);
+  return vec && (dx_gemm || dw_gemm);
+}
+size_t dz_bytes(const pfnn::Geom& g) {
+  return ((size_t)g.n * g.ho * g.wo * g.cout * sizeof(uint16_t) + 255) / 256 * 256;
+}
+
 // The backward kernels of one layer; `part` is the weight-gradient workspace
-// of weight_grad_splits(g) * cout * kdim floats (unused when dw is NULL).
+// of weight_grad_splits(g) * cout * kdim floats (unused when dw is NULL);
+// `dzb`, when not null, dz_bytes(g) of workspace for the BF16 dZ prepass
+// (gemm_dz(g)): the same values the passes would otherwise round from dY
+// and Y, so either way gives the same bits.
 void conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void* y,
                    const float* dy, float* dx, float* dw, float* dbias, float* part,
-                   hipStream_t s) {
+                   uint16_t* dzb, hipStream_t s) {
   const pfnn::Geom g = pfnn::make_geom(*c);
+  if (dzb && gemm_dz(g, dx != nullptr, dw != nullptr)) {
+    const int64_t nv = (int64_t)g.n * g.ho * g.wo * (g.cout / 8);
+    pfnn::k_make_dz<<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(g, dy, y, c->relu, c->out_f32, dzb);
+  } else {
+    dzb = nullptr;
+  }
   if (dx && g.cin == 1 && g.cout % 8 == 0 && g.cout <= 64) {
     const int64_t px = (int64_t)g.n * g.h * g.w, per = 256 / (g.cout / 8);
     pfnn::k_conv_c1_dgrad<<<(unsigned)((px + per - 1) / per), 256,
@@ -1157,7 +1204,7 @@ void conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void
     const int ncls = pfnn::dg_classes(g, cls);
     for (int q = 0; q < ncls; ++q) {
       pfnn::DataGradOp op{g, cls[q], static_cast<const uint16_t*>(w), dy, y, c->relu, c->out_f32,
-                          dx, g.cin % 8 == 0 && g.cout % 8 == 0};
+                          dx, g.cin % 8 == 0 && g.cout % 8 == 0, dzb};
       pfnn::launch_gemm(op, g.n * cls[q].hc * cls[q].wc, g.cin, cls[q].ntap * g.cout, 1, s);
     }
   }
@@ -1178,7 +1225,7 @@ void conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void
   } else if (dw) {
     const int splits = pfnn::weight_grad_splits(g);
     pfnn::WeightGradOp op{g, static_cast<const uint16_t*>(x), dy, y, c->relu, c->out_f32, part,
-                          g.cin % 8 == 0 && g.cout % 8 == 0};
+                          g.cin % 8 == 0 && g.cout % 8 == 0, dzb};
     pfnn::launch_gemm(op, g.kdim, g.cout, g.n * g.ho * g.wo, splits, s);
     pfnn::reduce_splits(part, splits, (int64_t)g.cout * g.kdim, dw, s);
   }
@@ -1193,17 +1240,19 @@ int pf_nn_conv_backward(const pf_nn_conv* c, const void* x, const void* w, const
                         const float* dy, float* dx, float* dw, float* dbias, void* stream) {
   if (!valid(c) || !x || !w || !y || !dy) return nn_fail("pf_nn_conv_backward: bad arguments");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  float* part = nullptr;
-  if (dw) {
-    // stream-ordered workspace: allocated, used and released in order on s
-    const pfnn::Geom g = pfnn::make_geom(*c);
-    const size_t bytes = (size_t)pfnn::weight_grad_splits(g) * g.cout * g.kdim * sizeof(float);
-    if (hipMallocAsync(reinterpret_cast<void**>(&part), bytes, s) != hipSuccess)
-      return nn_status("pf_nn_conv_backward: workspace allocation");
-  }
-  conv_backward(c, x, w, y, dy, dx, dw, dbias, part, s);
+  // stream-ordered workspace (allocated, used and released in order on s):
+  // the BF16 dZ, then the weight gradient's split partials
+  const pfnn::Geom g = pfnn::make_geom(*c);
+  const size_t zb = gemm_dz(g, dx != nullptr, dw != nullptr) ? dz_bytes(g) : 0;
+  const size_t pb = dw ? (size_t)pfnn::weight_grad_splits(g) * g.cout * g.kdim * sizeof(float) : 0;
+  char* ws = nullptr;
+  if (zb + pb > 0 && hipMallocAsync(reinterpret_cast<void**>(&ws), zb + pb, s) != hipSuccess)
+    return nn_status("pf_nn_conv_backward: workspace allocation");
+  uint16_t* dzb = zb ? reinterpret_cast<uint16_t*>(ws) : nullptr;
+  float* part = pb ? reinterpret_cast<float*>(ws + zb) : nullptr;
+  conv_backward(c, x, w, y, dy, dx, dw, dbias, part, dzb, s);
   const int st = nn_status("pf_nn_conv_backward");
-  if (part) (void)hipFreeAsync(part, s);
+  if (ws) (void)hipFreeAsync(ws, s);
   return st;
 }
 
@@ -1266,7 +1315,9 @@ int pf_nn_conv_backward_noalloc(const pf_nn_conv* c, const void* x, const void* 
     const int st = scratch_on_device(&part);
     if (st != PF_OK) return st;
   }
-  conv_backward(c, x, w, y, dy, dx, dw, dbias, part, static_cast<hipStream_t>(stream));
+  // no BF16 dZ workspace here: the passes round dZ from dY and Y themselves
+  // (the same values, so the same bits as pf_nn_conv_backward)
+  conv_backward(c, x, w, y, dy, dx, dw, dbias, part, nullptr, static_cast<hipStream_t>(stream));
   return nn_status("pf_nn_conv_backward_noalloc");
 }
 
