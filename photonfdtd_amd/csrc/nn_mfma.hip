@@ -1084,11 +1084,19 @@ inline bool c1_wgrad4(const Geom& g) {
   return c1_wgrad(g) && g.cout % 4 == 0 && (int64_t)g.n * g.ho * g.wo * g.cout < INT32_MAX &&
          (int64_t)g.n * g.h * g.w < INT32_MAX;
 }
+// Weight-gradient tiles of 288 rows (3x3 taps x 32 channels) where kdim is a
+// multiple of 288 and cout <= 64: no padded rows, and dZ read once per 288
+// rows of kdim instead of once per 128.
+constexpr int WG_BM288 = 288;
+inline bool wgrad_bm288(const Geom& g) {
+  return g.kdim % WG_BM288 == 0 && g.cout <= 64 && g.cin % 8 == 0 && g.cout % 8 == 0;
+}
 inline int weight_grad_splits(const Geom& g) {
   const int64_t R = (int64_t)g.n * g.ho * g.wo;
   if (c1_wgrad(g)) return (int)((R + C1_PIX - 1) / C1_PIX);
   const int64_t per = (int64_t)g.cout * g.kdim;
-  const int64_t tiles = (int64_t)((g.kdim + 127) / 128) * ((g.cout + tile_n(g.cout) - 1) / tile_n(g.cout));
+  const int bm = wgrad_bm288(g) ? WG_BM288 : 128;
+  const int64_t tiles = (int64_t)((g.kdim + bm - 1) / bm) * ((g.cout + tile_n(g.cout) - 1) / tile_n(g.cout));
   int64_t s = std::max<int64_t>((512 + tiles - 1) / tiles, (R + 8191) / 8192);
   s = std::min<int64_t>(s, std::max<int64_t>(1, R / 256));
   s = std::min<int64_t>(s, std::max<int64_t>(1, SCRATCH_FLOATS / per));
@@ -1226,7 +1234,14 @@ void conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void
     const int splits = pfnn::weight_grad_splits(g);
     pfnn::WeightGradOp op{g, static_cast<const uint16_t*>(x), dy, y, c->relu, c->out_f32, part,
                           g.cin % 8 == 0 && g.cout % 8 == 0, dzb};
-    pfnn::launch_gemm(op, g.kdim, g.cout, g.n * g.ho * g.wo, splits, s);
+    const int K = g.n * g.ho * g.wo;
+    if (pfnn::wgrad_bm288(g)) {
+      const int kper = std::max(pfnn::BK, ((K + splits - 1) / splits + pfnn::BK - 1) / pfnn::BK * pfnn::BK);
+      const int z = std::max(1, (K + kper - 1) / kper);
+      pfnn::launch_gemm_bm<pfnn::WeightGradOp, pfnn::WG_BM288>(op, g.kdim, g.cout, K, kper, z, s);
+    } else {
+      pfnn::launch_gemm(op, g.kdim, g.cout, K, splits, s);
+    }
     pfnn::reduce_splits(part, splits, (int64_t)g.cout * g.kdim, dw, s);
   }
   if (dbias) pfnn::k_bias_grad<<<g.cout, 256, 0, s>>>(g, dy, y, c->relu, c->out_f32, dbias);
