@@ -156,6 +156,35 @@ def test_post_step_launch_and_main_stream_x_list_same_bits(precision, exact, mon
     assert ic["post_fused"] == 0, ic
 
 
+def test_sampled_event_timing_counts(monkeypatch):
+    """pf_timing_enable(h, k) brackets the march and aux groups of every k-th
+    step only (k = 1: every step); a negative interval is refused."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd import _native as N
+    from photonfdtd_amd.hip_backend import HipEngine
+    print("[case] dispersive slab sampled event timing", flush=True)
+    for k in ("PHOTONFDTD_HIP_ZPAD", "PHOTONFDTD_HIP_POST", "PHOTONFDTD_HIP_SIDE"):
+        monkeypatch.delenv(k, raising=False)
+    monkeypatch.setenv("PHOTONFDTD_HIP_ZPAD", "64")
+    sim = dispersive_slab(pf, "hip", steps=60, precision="float32")
+    eng = HipEngine(sim, np.float32, False)
+    lib = eng.lib
+    fused, aux = N.KERNEL_CLASSES.index("yee_fused"), N.KERNEL_CLASSES.index("aux")
+    try:
+        assert lib.pf_timing_enable(eng.h, -1) == -1   # PF_E_ARG
+        for every, step0, n, want in ((10, 0, 40, 4), (1, 40, 20, 20)):
+            N.check(lib, lib.pf_timing_enable(eng.h, every), "pf_timing_enable")
+            eng.advance(step0, n)
+            ms = np.zeros(len(N.KERNEL_CLASSES))
+            cnt = np.zeros(len(N.KERNEL_CLASSES), dtype=np.int64)
+            N.check(lib, lib.pf_timing_read(eng.h, N.as_ptr(ms, N.C.c_double),
+                                            N.as_ptr(cnt, N.C.c_int64)), "pf_timing_read")
+            assert cnt[fused] == want and cnt[aux] == want, (every, cnt)
+            assert ms[fused] > 0 and ms[aux] > 0, (every, ms)
+    finally:
+        eng.close()
+
+
 def test_full_size_64_lane_march_same_bits_as_padded_march(monkeypatch):
     """At 512^3 float32 the 64-lane march's scalar offsets (plane, component
     and psi-slot offsets) reach beyond 1 GiB; the padded march addresses the
