@@ -1079,6 +1079,8 @@ constexpr int64_t SCRATCH_FLOATS = int64_t(4) << 20;
 // function of the shape only, so both backward entry points sum the same
 // partials in the same order.
 inline bool c1_wgrad(const Geom& g) { return g.cin == 1 && g.cout <= 64 && g.k <= 3; }
+// the one-input-channel data gradient (k_conv_c1_dgrad)
+inline bool c1_dgrad(const Geom& g) { return g.cin == 1 && g.cout % 8 == 0 && g.cout <= 64; }
 // the four-channel form: 16-byte dY rows and 32-bit indices
 inline bool c1_wgrad4(const Geom& g) {
   return c1_wgrad(g) && g.cout % 4 == 0 && (int64_t)g.n * g.ho * g.wo * g.cout < INT32_MAX &&
@@ -1178,7 +1180,7 @@ namespace {
 // (vector operands on both channel counts, one of them a GEMM pass).
 bool gemm_dz(const pfnn::Geom& g, bool dx, bool dw) {
   const bool vec = g.cin % 8 == 0 && g.cout % 8 == 0;
-  const bool dx_gemm = dx && !(g.cin == 1 && g.cout % 8 == 0 && g.cout <= 64);
+  const bool dx_gemm = dx && !pfnn::c1_dgrad(g);
   const bool dw_gemm = dw && !pfnn::c1_wgrad(g);
   return vec && (dx_gemm || dw_gemm);
 }
@@ -1201,7 +1203,7 @@ void conv_backward(const pf_nn_conv* c, const void* x, const void* w, const void
   } else {
     dzb = nullptr;
   }
-  if (dx && g.cin == 1 && g.cout % 8 == 0 && g.cout <= 64) {
+  if (dx && pfnn::c1_dgrad(g)) {
     const int64_t px = (int64_t)g.n * g.h * g.w, per = 256 / (g.cout / 8);
     pfnn::k_conv_c1_dgrad<<<(unsigned)((px + per - 1) / per), 256,
                             (g.cout * g.k * g.k + 256) * sizeof(float), s>>>(
