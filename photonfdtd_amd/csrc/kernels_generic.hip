@@ -437,6 +437,7 @@ void launch_post_step(const PostArgs<real>& P, hipStream_t s) {
 // factors are wave-uniform scalar loads), and reads each buffered sample once
 // for all of them.
 constexpr int FOLD_FB = 8;
+constexpr int FOLD_SU = 4;   // samples whose ring loads are in flight together
 
 template <typename real>
 __global__ __launch_bounds__(256) void k_dft_fold(const real* __restrict__ ring, int64_t nbox,
@@ -460,8 +461,27 @@ __global__ __launch_bounds__(256) void k_dft_fold(const real* __restrict__ ring,
   }
   const real* src = ring + (int64_t)c * nbox + r;
   const int64_t stride = (int64_t)n_comp * nbox;
-  for (int s = 0; s < count; ++s) {
-    const double v = (double)src[s * stride];
+  // samples in groups of FOLD_SU, their ring loads issued together, then
+  // accumulated one sample after another (the per-sample order)
+  int s = 0;
+  for (; s + FOLD_SU <= count; s += FOLD_SU) {
+    double v[FOLD_SU];
+#pragma unroll
+    for (int u = 0; u < FOLD_SU; ++u) v[u] = (double)src[(int64_t)(s + u) * stride];
+#pragma unroll
+    for (int u = 0; u < FOLD_SU; ++u) {
+      const double* p = ph + 2 * ((sample0 + s + u) * n_freq + f0);
+#pragma unroll
+      for (int q = 0; q < FOLD_FB; ++q) {
+        if (q < nf) {
+          re[q] = re[q] + (p[2 * q] * v[u]) * dt;
+          im[q] = im[q] + (p[2 * q + 1] * v[u]) * dt;
+        }
+      }
+    }
+  }
+  for (; s < count; ++s) {
+    const double v = (double)src[(int64_t)s * stride];
     const double* p = ph + 2 * ((sample0 + s) * n_freq + f0);
 #pragma unroll
     for (int q = 0; q < FOLD_FB; ++q) {
