@@ -1464,7 +1464,9 @@ template <typename real>
 void Stepper<real>::after_march(int64_t step) {
   const int in = cur_, out = cur_ ^ 1;
   pfhip::AdeJob<real>*& tab = ade_tab1_[in][p_lvl_];
-  if ((ade_.size() > 1 || (post_ok_ && !ade_.empty())) && !tab) {
+  bool any_poles = false;
+  for (const Ade& a : ade_) any_poles |= a.n > 0 && a.n_poles > 0;
+  if ((ade_.size() > 1 || post_ok_) && any_poles && !tab) {
     // ade_jobs reads the polarisation pointers of the current P level
     std::vector<pfhip::AdeJob<real>> jobs;
     ade_jobs(in, jobs);
