@@ -172,3 +172,24 @@ def test_dense64_forward_runs_the_march_and_matches_generic():
     finally:
         adj._checkpointed_engine = orig
     assert np.array_equal(g_march, g_gen)
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_march_recompute_same_gradient_as_generic_recompute(precision, monkeypatch):
+    """On the dense 64-slot rows the reverse sweep recomputes each checkpoint
+    segment with the forward's march (trajectory slots as its levels, the
+    E-side psi copied into the output slot first); with
+    PHOTONFDTD_HIP_ADJ_MARCH=0 it takes the generic kernels from the same
+    checkpoints. Both give the same gradient bits (float64 in reference order
+    and float32 in the fast order, checkpoint interval 7: a short last
+    segment)."""
+    import photonfdtd_amd as pf
+    from photonfdtd_amd.adjoint import value_and_grad_eps
+    print(f"\n[case] adjoint march vs generic recompute {precision}", flush=True)
+    monkeypatch.delenv("PHOTONFDTD_HIP_ADJ_MARCH", raising=False)
+    _, g_march = value_and_grad_eps(tall_3d(pf, precision), loss_of, checkpoint_interval=7)
+    monkeypatch.setenv("PHOTONFDTD_HIP_ADJ_MARCH", "0")
+    _, g_gen = value_and_grad_eps(tall_3d(pf, precision), loss_of, checkpoint_interval=7)
+    g_march, g_gen = np.asarray(g_march), np.asarray(g_gen)
+    assert np.abs(g_march).max() > 0
+    assert np.array_equal(g_march, g_gen)
